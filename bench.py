@@ -1,0 +1,182 @@
+"""RHS throughput on one MI355X.
+
+Workload (the metric's configuration on one GPU): Sedov-Taylor 128^3 = 2,097,152
+particles, SPH only, plain h-iteration (band 50-80, target 64), cubic spline, ideal gas,
+FP64 -- one step = one full Miluphpc::rhs() (bbox, keys, sort, tree, h-iteration,
+neighbours, density, EOS, forces) with inputs resident in HBM.  The Plummer
+16,777,216-particle Barnes-Hut RHS (theta 0.6, quadrupole, FP32 + Kahan) is timed beside
+it and reported under "plummer".
+
+Prints ONE JSON line.  value = RHS particles/s of the Sedov workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def sph_bytes_per_particle(nnb: float) -> dict:
+    """Algorithmic (compulsory) HBM bytes per particle of each SPH kernel."""
+    return {
+        "ms_nn": 32 + 4 + 4 * nnb,           # read x,y,z,h; write list + count
+        "ms_density": 40 + 4 + 4 * nnb + 8,  # read x,y,z,h,m + list; write rho
+        "ms_eos": 24 + 16,                   # read rho,e,p; write p,cs
+        "ms_forces": 88 + 4 + 4 * nnb + 48,  # read 11 fields + list; write 6 fields
+        "ms_hiter": 32 + 8,                  # per active particle and pass: read x,y,z,h; write h
+    }
+
+
+def time_rhs(ctx, lib, flags, steps, warmup, label):
+    import torch
+
+    for i in range(warmup):
+        ctx.rhs(flags)
+        log(f"{label}: warmup {i + 1}/{warmup} done")
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        stats.append(ctx.rhs(flags))
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    log(f"{label}: {steps} timed steps, {wall * 1e3:.3f} ms/step")
+    return wall, stats
+
+
+def median_stats(stats):
+    keys = [k for k in stats[0] if k.startswith("ms_")]
+    out = {k: float(np.median([s[k] for s in stats])) for k in keys}
+    for k in stats[0]:
+        if not k.startswith("ms_"):
+            out[k] = stats[-1][k]
+    return out
+
+
+def cpu_baseline_sedov(s) -> dict:
+    """The oracle (FP64, OpenMP) on the same Sedov 128^3 input: one full SPH RHS."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import pyoracle as orc
+
+    cores = len(os.sched_getaffinity(0))
+    orc.set_threads(cores)
+    t0 = time.perf_counter()
+    orc.sph_rhs(s)
+    dt = time.perf_counter() - t0
+    n = len(s["x"])
+    return {"value": n / dt, "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": f"one full SPH RHS (h-iteration, neighbours, density, EOS, forces) of the "
+                      f"Sedov 128^3 input, {dt:.2f} s on {cores} host threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nside", type=int, default=128)
+    ap.add_argument("--plummer-n", type=int, default=16777216)
+    ap.add_argument("--no-plummer", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU decomposition is out of scope for this build)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    from miluphpc_amd import ics, lib
+
+    # ---- Sedov 128^3, SPH, plain h-iteration, FP64 ------------------------------------
+    s = ics.sedov_lattice(args.nside)
+    n = len(s["x"])
+    log(f"sedov {args.nside}^3 = {n} particles generated")
+    p = ics.params_sedov()
+    flags = lib.RHS_SPH | lib.RHS_VARIABLE_SML
+    with lib.Context(n, p) as ctx:
+        ctx.load(**{k: v for k, v in s.items() if k in lib.FIELD_ID})
+        wall, stats = time_rhs(ctx, lib, flags, args.steps, args.warmup, "sedov")
+    med = median_stats(stats)
+    nnb = med["n_neighbours"] / n
+    bpp = sph_bytes_per_particle(nnb)
+    kern = {k: med[k] for k in bpp}
+    dom = max(kern, key=kern.get)
+    units = n if dom != "ms_hiter" else max(1, med["n_h_counts"])
+    traffic_bytes = bpp[dom] * units
+    achieved = traffic_bytes / (med[dom] * 1e-3) / 1e9
+    result = {
+        "metric": "RHS particles/s: Sedov 2M SPH, Plummer 16M BH; learned-h vs plain, same GPU",
+        "value": n / wall,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Sedov-Taylor {args.nside}^3 lattice SPH RHS, plain h-iteration "
+                               f"(target 64, band 50-80), cubic spline, ideal gas, FP64",
+                   "n_particles": n, "path": "plain"},
+        "roofline": {"bound": "hbm", "kernel": dom[3:], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_particle": bpp[dom], "kernel_ms": med[dom]},
+        "phases_ms": {k[3:]: round(v, 4) for k, v in med.items() if k.startswith("ms_")},
+        "mean_neighbours": nnb,
+        "mean_h_counts": med["n_h_counts"] / n,
+    }
+    log(f"sedov: {n / wall:.4e} particles/s, dominant {dom} {med[dom]:.3f} ms, {achieved:.1f} GB/s")
+
+    # ---- Plummer 16M, Barnes-Hut, theta 0.6, quadrupole, FP32 + Kahan ----------------
+    if not args.no_plummer:
+        pl = ics.plummer(args.plummer_n)
+        log(f"plummer {args.plummer_n} generated")
+        pp = ics.params_plummer_large()
+        with lib.Context(args.plummer_n, pp) as ctx:
+            ctx.load(**pl)
+            pw, pst = time_rhs(ctx, lib, lib.RHS_GRAVITY, max(1, args.steps // 2), 1, "plummer")
+        pm = median_stats(pst)
+        flops = 23.0 * pm["n_part_interactions"] + 68.0 * pm["n_cell_interactions"]
+        tflops = flops / (pm["ms_gravity"] * 1e-3) / 1e12
+        result["plummer"] = {
+            "workload": f"Plummer {args.plummer_n} Barnes-Hut RHS, theta 0.6, eps 2e-4, monopole + "
+                        "quadrupole, FP32 + Kahan far field, FP64 positions and keys",
+            "value": args.plummer_n / pw, "unit": "particles/s", "ms_per_step": pw * 1e3,
+            "phases_ms": {k[3:]: round(v, 4) for k, v in pm.items() if k.startswith("ms_")},
+            "interactions_per_particle": (pm["n_part_interactions"] + pm["n_cell_interactions"]) / args.plummer_n,
+            "roofline": {"bound": "fp32", "kernel": "gravity", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS,
+                         "flops_per_interaction": {"particle": 23, "cell_quadrupole": 68}},
+        }
+        log(f"plummer: {args.plummer_n / pw:.4e} particles/s, gravity {pm['ms_gravity']:.2f} ms, {tflops:.2f} TFLOP/s")
+        del pl
+
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        result["cpu_baseline"] = cpu_baseline_sedov(s)
+        log(f"cpu baseline {result['cpu_baseline']['value']:.4e} particles/s")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

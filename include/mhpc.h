@@ -1,0 +1,197 @@
+/*
+ * mhpc.h -- C-ABI of the MI355X-native single-subdomain RHS of milupHPC.
+ *
+ * One context owns one subdomain on one GPU: device-resident SoA particle fields in the
+ * caller's index order, the key-sorted working copies, the octree and the neighbour
+ * lists, one hipStream_t and a sticky device error word.  All entry points return an
+ * mhpc_status (0 = MHPC_OK) and never exit the process; stage entry points write the
+ * elapsed milliseconds of their kernels (hipEvents on the context stream) to `ms`
+ * when it is non-NULL, which keeps the reference's "Launch::f(...) returns time"
+ * convention.  Contexts are not shared between host threads.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * milupHPC source tree).
+ */
+#ifndef MHPC_H
+#define MHPC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHPC_API_VERSION 1
+
+typedef struct mhpc_ctx mhpc_ctx;
+
+/* Error codes.  Replace gpuErrorcheck's exit (cuda_utilities.cuh:41-46), cudaTerminate
+ * (cuda_utilities.cuh:70-74) and the capacity exits (miluphpc.cpp:974-981, 2001-2006,
+ * 2049-2055, 2600-2606, 2802-2807). */
+typedef enum {
+    MHPC_OK = 0,
+    MHPC_E_CAP_NODES = 1,        /* more cells than the node capacity */
+    MHPC_E_CAP_NEIGHBOURS = 2,   /* a particle has >= max_nb neighbours, or 0 after h-iteration */
+    MHPC_E_DEPTH = 3,            /* two particles share a 21-level key (duplicate positions) */
+    MHPC_E_RHO_NONPOSITIVE = 4,  /* density <= 0 (density.cu:68-70) */
+    MHPC_E_STACK = 5,            /* traversal stack overflow */
+    MHPC_E_ARG = 6,              /* bad argument */
+    MHPC_E_HIP = 7,              /* HIP runtime error */
+    MHPC_E_STATE = 8             /* call order (e.g. forces before density) */
+} mhpc_status;
+
+/* Space-filling curves (reference Curve::Type, subdomain.cuh). */
+enum { MHPC_CURVE_LEBESGUE = 0, MHPC_CURVE_HILBERT = 1 };
+
+/* Equations of state (reference EquationOfStates ids, material.cuh).  100 is a build
+ * extension: isothermal gas with a constant caller-given sound speed, p = cs^2 rho. */
+enum {
+    MHPC_EOS_POLYTROPIC = 0,
+    MHPC_EOS_ISOTHERMAL = 3,
+    MHPC_EOS_IDEAL_GAS = 9,
+    MHPC_EOS_LOCALLY_ISOTHERMAL = 12,
+    MHPC_EOS_ISOTHERMAL_CONST_CS = 100
+};
+
+/* Barnes-Hut acceptance forms: s/d < theta (computeForces_v1, gravity.cu:252-396) or
+ * s^2/theta <= d^2 + eps^2 (computeForces_v2, gravity.cu:726-903), both evaluated for a
+ * group of consecutive key-sorted particles against the group's bounding box. */
+enum { MHPC_MAC_CLASSIC = 0, MHPC_MAC_SQRT_THETA = 1 };
+
+/* Pair-sum precision. */
+enum { MHPC_PREC_FP64 = 0, MHPC_PREC_FP32_KAHAN = 1 };
+
+/* Runtime parameters.  Field names follow SimulationParameters (parameter.h:122-159)
+ * and Material (material.cuh:123-160) where they have a counterpart. */
+typedef struct {
+    /* gravity (SimulationParameters::theta, ::smoothing = eps^2, ::gravityForceVersion) */
+    double theta;
+    double smoothing;
+    int mac;
+    int quadrupole;         /* 0: monopole (reference), 1: monopole + quadrupole */
+    double G;               /* 1 in N-body units (SI_UNITS 0) */
+    /* keys (SimulationParameters::sfcSelection) */
+    int curve;
+    /* one material (Material::eos, ::artificialViscosity, ::interactions) */
+    int eos;
+    double gamma;           /* eos.polytropic_gamma */
+    double K;               /* eos.polytropic_K */
+    double cs_const;        /* sound speed for MHPC_EOS_ISOTHERMAL_CONST_CS */
+    double alpha, beta;     /* Monaghan artificial viscosity */
+    int n_target;           /* Material::interactions (target neighbour count) */
+    int band_lo, band_hi;   /* accepted neighbour-count band of the h-iteration */
+    int max_counts;         /* counts per particle of the h-iteration (reference: 7) */
+    int max_nb;             /* MAX_NUM_INTERACTIONS (parameter.h, 180) */
+    int sml_avg;            /* kernel length (h_i + h_j)/2 (VARIABLE_SML || INTEGRATE_SML) */
+    int precision;          /* MHPC_PREC_* for gravity and SPH pair sums */
+    int cs_before_p;        /* 1: sound speed from the previous p, then p (miluphpc.cpp:2938-2951) */
+} mhpc_params;
+
+/* Field ids for upload / download / device pointers (reference Entry::Name, particles.cuh).
+ * All particle fields are in the caller's (original) index order. */
+typedef enum {
+    MHPC_F_X = 0, MHPC_F_Y, MHPC_F_Z,
+    MHPC_F_VX, MHPC_F_VY, MHPC_F_VZ,
+    MHPC_F_MASS, MHPC_F_SML, MHPC_F_E, MHPC_F_RHO, MHPC_F_P, MHPC_F_CS,
+    MHPC_F_AX, MHPC_F_AY, MHPC_F_AZ,            /* SPH accelerations */
+    MHPC_F_G_AX, MHPC_F_G_AY, MHPC_F_G_AZ,      /* gravitational accelerations */
+    MHPC_F_G_AX_OLD, MHPC_F_G_AY_OLD, MHPC_F_G_AZ_OLD,
+    MHPC_F_DRHODT, MHPC_F_DEDT, MHPC_F_DSMLDT,
+    MHPC_F_U,                                   /* gravitational potential energy 1/2 m phi */
+    MHPC_F_NOI,                                 /* int32: number of interactions */
+    MHPC_F_KEY,                                 /* uint64: SFC key of each particle */
+    MHPC_F_SORTED,                              /* int32: key-sorted order -> particle index */
+    MHPC_F_LEVEL,                               /* int32: leaf level of each particle */
+    MHPC_F_H_ITERS,                             /* int32: h-iteration counts of the last RHS */
+    MHPC_F_COUNT_
+} mhpc_field;
+
+/* RHS flags (GRAVITY_SIM, SPH_SIM, VARIABLE_SML switches of parameter.h). */
+enum {
+    MHPC_RHS_GRAVITY = 1u << 0,
+    MHPC_RHS_SPH = 1u << 1,
+    MHPC_RHS_VARIABLE_SML = 1u << 2,   /* plain h-iteration before the neighbour search */
+    MHPC_RHS_POTENTIAL = 1u << 3,      /* also accumulate the gravitational potential */
+    MHPC_RHS_TIMING = 1u << 4,         /* per-phase hipEvent timing (adds no sync) */
+    MHPC_RHS_LEARNED_H = 1u << 5       /* MLP predicts h; the plain iteration corrects */
+};
+
+typedef struct {
+    float ms_total;
+    float ms_bbox, ms_keys, ms_sort, ms_gather, ms_tree, ms_com;
+    float ms_gravity, ms_mlp, ms_hiter, ms_nn, ms_density, ms_eos, ms_forces, ms_scatter;
+    int64_t num_cells;
+    int64_t n_cell_interactions;       /* BH: particle-cell interactions */
+    int64_t n_part_interactions;       /* BH: particle-particle interactions */
+    int64_t n_neighbours;              /* sum of noi */
+    int64_t n_h_counts;                /* neighbour counts done by the h-iteration */
+    int32_t max_noi;
+    int32_t h_passes;
+} mhpc_rhs_stats;
+
+/* -------- context (replaces the Miluphpc ctor allocations, miluphpc.cpp:3-93, and the
+ * ParticleHandler / TreeHandler / HelperHandler allocations, particle_handler.cpp:3-205,
+ * tree_handler.cpp:3-41, helper_handler.cpp) -------- */
+void mhpc_default_params(mhpc_params *p);
+int mhpc_create(const mhpc_params *p, int64_t cap_particles, int64_t cap_cells, mhpc_ctx **out);
+int mhpc_destroy(mhpc_ctx *ctx);
+int mhpc_set_params(mhpc_ctx *ctx, const mhpc_params *p);
+int mhpc_get_params(mhpc_ctx *ctx, mhpc_params *p);
+int mhpc_set_num_particles(mhpc_ctx *ctx, int64_t n);
+int64_t mhpc_num_particles(mhpc_ctx *ctx);
+const char *mhpc_last_error(mhpc_ctx *ctx);
+const char *mhpc_status_string(int status);
+int mhpc_synchronize(mhpc_ctx *ctx);
+
+/* -------- data movement (ParticleHandler::copy*(To::Target), particle_handler.h:391-455) -- */
+int mhpc_upload(mhpc_ctx *ctx, int field, const void *host, int64_t n);
+int mhpc_download(mhpc_ctx *ctx, int field, void *host, int64_t n);
+int mhpc_field_ptr(mhpc_ctx *ctx, int field, void **device_ptr);
+/* neighbour lists in the reference layout nnl[i * max_nb + k] (original indices) */
+int mhpc_download_neighbours(mhpc_ctx *ctx, int32_t *nnl, int32_t *noi, int64_t n, int max_nb);
+/* octree in key-sorted numbering: particles [0,n) (sorted positions), cells n + c */
+int mhpc_tree_size(mhpc_ctx *ctx, int64_t *num_cells);
+int mhpc_download_tree(mhpc_ctx *ctx, int32_t *cell_level, int32_t *cell_start,
+                       int32_t *cell_end, int32_t *cell_child, double *cell_mass,
+                       double *cell_x, double *cell_y, double *cell_z, double *cell_quad);
+int mhpc_get_box(mhpc_ctx *ctx, double box[6]);
+
+/* -------- RHS: Miluphpc::rhs(step, selfGravity, assign) (miluphpc.h:478,
+ * miluphpc.cpp:387-562) -------- */
+int mhpc_rhs(mhpc_ctx *ctx, uint32_t flags, mhpc_rhs_stats *stats);
+
+/* -------- stages (the Kernel::Launch wrappers of the reference) -------- */
+/* TreeNS::Kernel::Launch::computeBoundingBox (tree.cu:1949-1955) */
+int mhpc_bbox(mhpc_ctx *ctx, double box[6], float *ms);
+/* SubDomainKeyTreeNS::Kernel::Launch::getParticleKeys (subdomain.cu:1564-1570) */
+int mhpc_keys(mhpc_ctx *ctx, float *ms);
+/* TreeNS::Kernel::Launch::prepareSorting + HelperNS::sortArray (tree.cu:1404-1413,
+ * helper.cu:150-167) */
+int mhpc_sort(mhpc_ctx *ctx, float *ms);
+/* TreeNS::Kernel::Launch::buildTree (tree.cu:1929-1937), on the key-sorted order */
+int mhpc_build_tree(mhpc_ctx *ctx, float *ms);
+/* TreeNS::Kernel::Launch::calculateCentersOfMass (tree.cu:1416-1497) for all levels */
+int mhpc_centres_of_mass(mhpc_ctx *ctx, float *ms);
+/* Gravity::Kernel::Launch::computeForces_v2 (gravity.cu:2652-2663) */
+int mhpc_gravity(mhpc_ctx *ctx, int potential, float *ms);
+/* SPH::Kernel::Launch::fixedRadiusNN_variableSML (sph.cu:3128-3135) */
+int mhpc_h_iterate(mhpc_ctx *ctx, float *ms);
+/* SPH::Kernel::Launch::fixedRadiusNN (sph.cu:3094-3101) */
+int mhpc_neighbours(mhpc_ctx *ctx, float *ms);
+/* SPH::Kernel::Launch::calculateDensity (density.cu:74-79) */
+int mhpc_density(mhpc_ctx *ctx, float *ms);
+/* SPH::Kernel::Launch::calculateSoundSpeed + calculatePressure (soundspeed.cu,
+ * pressure.cu:93) */
+int mhpc_eos(mhpc_ctx *ctx, float *ms);
+/* SPH::Kernel::Launch::internalForces (internal_forces.cu:671-676) */
+int mhpc_internal_forces(mhpc_ctx *ctx, float *ms);
+
+/* -------- integrator kernels (LeapfrogNS::Kernel::Launch::updateX / updateV,
+ * device_leapfrog.cu:4-58) -------- */
+int mhpc_leapfrog_update_x(mhpc_ctx *ctx, double dt);
+int mhpc_leapfrog_update_v(mhpc_ctx *ctx, double dt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,612 @@
+// api.hip -- the C-ABI: context lifetime, field transfer, stage entry points and the
+// RHS sequence of Miluphpc::rhs (miluphpc.cpp:387-562) for one subdomain on one GPU.
+//
+// Asynchronous launches on the context stream; one synchronisation at the end of each
+// entry point, where the sticky device error word is read and mapped to mhpc_status
+// (replacing the reference's exits and cudaTerminate).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "mhpc_internal.h"
+
+#define HIPC(expr)                                                            \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) {                                               \
+            c->last_error = std::string(#expr) + ": " + hipGetErrorString(e_); \
+            return MHPC_E_HIP;                                                \
+        }                                                                     \
+    } while (0)
+
+extern "C" void mhpc_default_params(mhpc_params *p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->theta = 0.5;
+    p->smoothing = 0.0;
+    p->mac = MHPC_MAC_CLASSIC;
+    p->quadrupole = 0;
+    p->G = 1.0;
+    p->curve = MHPC_CURVE_LEBESGUE;
+    p->eos = MHPC_EOS_IDEAL_GAS;
+    p->gamma = 5.0 / 3.0;
+    p->K = 0.0;
+    p->cs_const = 0.0;
+    p->alpha = 1.0;
+    p->beta = 2.0;
+    p->n_target = 64;
+    p->band_lo = 50;
+    p->band_hi = 80;
+    p->max_counts = 7;
+    p->max_nb = 180;
+    p->sml_avg = 1;
+    p->precision = MHPC_PREC_FP64;
+    p->cs_before_p = 1;
+}
+
+extern "C" const char *mhpc_status_string(int s) {
+    switch (s) {
+        case MHPC_OK: return "ok";
+        case MHPC_E_CAP_NODES: return "node capacity exceeded";
+        case MHPC_E_CAP_NEIGHBOURS: return "neighbour capacity exceeded (or no neighbours after h-iteration)";
+        case MHPC_E_DEPTH: return "duplicate 21-level keys (tree depth)";
+        case MHPC_E_RHO_NONPOSITIVE: return "non-positive density";
+        case MHPC_E_STACK: return "traversal stack overflow";
+        case MHPC_E_ARG: return "bad argument";
+        case MHPC_E_HIP: return "HIP runtime error";
+        case MHPC_E_STATE: return "call order";
+        default: return "unknown";
+    }
+}
+
+static int check_params(const mhpc_params *p) {
+    if (!p) return 0;
+    if (!(p->theta > 0.0) || p->smoothing < 0.0) return 0;
+    if (p->curve != MHPC_CURVE_LEBESGUE && p->curve != MHPC_CURVE_HILBERT) return 0;
+    if (p->mac != MHPC_MAC_CLASSIC && p->mac != MHPC_MAC_SQRT_THETA) return 0;
+    if (p->max_nb < 1 || p->max_counts < 1 || p->max_counts > 64) return 0;
+    if (p->n_target < 1 || p->band_lo > p->band_hi) return 0;
+    if (p->precision != MHPC_PREC_FP64 && p->precision != MHPC_PREC_FP32_KAHAN) return 0;
+    return 1;
+}
+
+template <typename T>
+static hipError_t dalloc(T **p, int64_t count) {
+    if (count <= 0) count = 1;
+    return hipMalloc((void **)p, sizeof(T) * (size_t)count);
+}
+
+static void upload_htable(mhpc_ctx *c) {
+    std::vector<double> t(MHPC_HTABLE_MAX + 1);
+    t[0] = 0.0;
+    for (int k = 1; k <= MHPC_HTABLE_MAX; k++)
+        t[k] = 0.5 * (1.0 + pow((double)c->p.n_target / (double)k, 1.0 / 3.0));
+    (void)hipMemcpy(c->htable, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice);
+}
+
+static int ensure_nbr(mhpc_ctx *c) {
+    if (c->nbr && c->nbr_rows >= c->p.max_nb) return MHPC_OK;
+    if (c->nbr) (void)hipFree(c->nbr);
+    c->nbr = nullptr;
+    HIPC(dalloc(&c->nbr, (int64_t)c->p.max_nb * c->cap_pad));
+    c->nbr_rows = c->p.max_nb;
+    return MHPC_OK;
+}
+
+static void free_all(mhpc_ctx *c) {
+    void *ptrs[] = {c->noi, c->level, c->h_iters, c->key, c->lkey, c->skey, c->keyB, c->perm,
+                    c->valB, c->tile_hist, c->tile_off, c->digit_hist, c->slkey, c->sx, c->sy,
+                    c->sz, c->sm, c->sh, c->sh2, c->svx, c->svy, c->svz, c->se, c->srho, c->sp,
+                    c->scs, c->snoi, c->siters, c->sdone, c->lcp, c->tcnt, c->toff,
+                    c->level_count, c->cell_base, c->cstart, c->cend, c->clevel, c->cchild,
+                    c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax,
+                    c->nbr, c->htable, c->bbox_partial, c->box, c->err, c->counters};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int f = 0; f < MHPC_F_NOI; f++)
+        if (c->f[f]) (void)hipFree(c->f[f]);
+    if (c->h_err) (void)hipHostFree(c->h_err);
+    if (c->h_counters) (void)hipHostFree(c->h_counters);
+    for (int e = 0; e < MHPC_NUM_EVENTS; e++)
+        if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
+    HIPC(hipGetDevice(&c->device));
+    HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int e = 0; e < MHPC_NUM_EVENTS; e++) HIPC(hipEventCreate(&c->ev[e]));
+    c->cap = cap;
+    c->cap_pad = ceil_div(cap, 64) * 64;
+    c->cap_cells = cap_cells;
+    for (int f = 0; f < MHPC_F_NOI; f++) {
+        HIPC(dalloc(&c->f[f], cap));
+        HIPC(hipMemset(c->f[f], 0, sizeof(double) * (size_t)cap));
+    }
+    HIPC(dalloc(&c->noi, cap));
+    HIPC(dalloc(&c->level, cap));
+    HIPC(dalloc(&c->h_iters, cap));
+    HIPC(hipMemset(c->noi, 0, sizeof(int32_t) * (size_t)cap));
+    HIPC(hipMemset(c->level, 0, sizeof(int32_t) * (size_t)cap));
+    HIPC(hipMemset(c->h_iters, 0, sizeof(int32_t) * (size_t)cap));
+    HIPC(dalloc(&c->key, cap));
+    HIPC(dalloc(&c->lkey, cap));
+    HIPC(dalloc(&c->skey, cap));
+    HIPC(dalloc(&c->keyB, cap));
+    HIPC(dalloc(&c->slkey, cap));
+    HIPC(dalloc(&c->perm, cap));
+    HIPC(dalloc(&c->valB, cap));
+    c->sort_tiles_cap = ceil_div(cap, MHPC_SORT_TILE);
+    HIPC(dalloc(&c->tile_hist, MHPC_SORT_BUCKETS * c->sort_tiles_cap));
+    HIPC(dalloc(&c->tile_off, MHPC_SORT_BUCKETS * c->sort_tiles_cap));
+    HIPC(dalloc(&c->digit_hist, MHPC_SORT_PASSES * MHPC_SORT_BUCKETS));
+    double **sorted[] = {&c->sx, &c->sy, &c->sz, &c->sm, &c->sh, &c->sh2, &c->svx,
+                         &c->svy, &c->svz, &c->se, &c->srho, &c->sp, &c->scs};
+    for (double **p : sorted) HIPC(dalloc(p, cap));
+    HIPC(dalloc(&c->snoi, cap));
+    HIPC(dalloc(&c->siters, cap));
+    HIPC(hipMemset(c->siters, 0, sizeof(int32_t) * (size_t)cap));
+    HIPC(dalloc(&c->sdone, cap));
+    HIPC(dalloc(&c->lcp, cap));
+    c->tree_tiles_cap = ceil_div(cap, MHPC_TREE_TILE);
+    HIPC(dalloc(&c->tcnt, (MHPC_LEVELS + 1) * c->tree_tiles_cap));
+    HIPC(dalloc(&c->toff, (MHPC_LEVELS + 1) * c->tree_tiles_cap));
+    HIPC(dalloc(&c->level_count, MHPC_LEVELS + 2));
+    HIPC(hipMemset(c->level_count, 0, sizeof(int32_t) * (MHPC_LEVELS + 2)));
+    HIPC(dalloc(&c->cell_base, MHPC_LEVELS + 3));
+    HIPC(hipMemset(c->cell_base, 0, sizeof(int32_t) * (MHPC_LEVELS + 3)));
+    HIPC(dalloc(&c->cstart, cap_cells));
+    HIPC(dalloc(&c->cend, cap_cells));
+    HIPC(dalloc(&c->clevel, cap_cells));
+    HIPC(dalloc(&c->cchild, 8 * cap_cells));
+    HIPC(dalloc(&c->cmass, cap_cells));
+    HIPC(dalloc(&c->cx, cap_cells));
+    HIPC(dalloc(&c->cy, cap_cells));
+    HIPC(dalloc(&c->cz, cap_cells));
+    HIPC(dalloc(&c->cquad, 6 * cap_cells));
+    HIPC(dalloc(&c->cquadf, 2 * cap_cells));
+    HIPC(dalloc(&c->cbmin, 3 * cap_cells));
+    HIPC(dalloc(&c->cbmax, 3 * cap_cells));
+    HIPC(dalloc(&c->htable, MHPC_HTABLE_MAX + 1));
+    upload_htable(c);
+    HIPC(dalloc(&c->bbox_partial, 2 * 1024));
+    HIPC(dalloc(&c->box, 6));
+    HIPC(hipMemset(c->box, 0, sizeof(double) * 6));
+    HIPC(dalloc(&c->err, 1));
+    HIPC(hipMemset(c->err, 0, sizeof(uint32_t)));
+    HIPC(dalloc(&c->counters, CNT_COUNT_));
+    HIPC(hipMemset(c->counters, 0, sizeof(unsigned long long) * CNT_COUNT_));
+    HIPC(hipHostMalloc((void **)&c->h_err, sizeof(uint32_t), hipHostMallocDefault));
+    HIPC(hipHostMalloc((void **)&c->h_counters, sizeof(unsigned long long) * CNT_COUNT_,
+                       hipHostMallocDefault));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_create(const mhpc_params *p, int64_t cap_particles, int64_t cap_cells,
+                           mhpc_ctx **out) {
+    if (!out) return MHPC_E_ARG;
+    *out = nullptr;
+    if (cap_particles < 1 || cap_particles > (int64_t)INT32_MAX / 2) return MHPC_E_ARG;
+    if (cap_cells <= 0) cap_cells = cap_particles + cap_particles / 2 + 65536;
+    if (cap_cells > (int64_t)INT32_MAX - cap_particles) return MHPC_E_ARG;
+    mhpc_params def;
+    mhpc_default_params(&def);
+    if (!p) p = &def;
+    if (!check_params(p)) return MHPC_E_ARG;
+    mhpc_ctx *c = new mhpc_ctx();
+    c->p = *p;
+    int st = create_impl(c, cap_particles, cap_cells);
+    if (st != MHPC_OK) {
+        fprintf(stderr, "mhpc_create: %s\n", c->last_error.c_str());
+        free_all(c);
+        delete c;
+        return st;
+    }
+    *out = c;
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_destroy(mhpc_ctx *c) {
+    if (!c) return MHPC_E_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_set_params(mhpc_ctx *c, const mhpc_params *p) {
+    if (!c || !check_params(p)) return MHPC_E_ARG;
+    const int old_target = c->p.n_target;
+    c->p = *p;
+    if (p->n_target != old_target) upload_htable(c);
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_get_params(mhpc_ctx *c, mhpc_params *p) {
+    if (!c || !p) return MHPC_E_ARG;
+    *p = c->p;
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_set_num_particles(mhpc_ctx *c, int64_t n) {
+    if (!c || n < 0 || n > c->cap) return MHPC_E_ARG;
+    c->n = n;
+    c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
+    c->nn_valid = c->rho_valid = c->eos_valid = false;
+    return MHPC_OK;
+}
+
+extern "C" int64_t mhpc_num_particles(mhpc_ctx *c) { return c ? c->n : -1; }
+
+extern "C" const char *mhpc_last_error(mhpc_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
+
+extern "C" int mhpc_synchronize(mhpc_ctx *c) {
+    if (!c) return MHPC_E_ARG;
+    HIPC(hipStreamSynchronize(c->stream));
+    return MHPC_OK;
+}
+
+struct FieldDesc {
+    void *ptr;
+    size_t elem;
+};
+
+static FieldDesc field_desc(mhpc_ctx *c, int field) {
+    if (field >= 0 && field < MHPC_F_NOI) return {c->f[field], sizeof(double)};
+    switch (field) {
+        case MHPC_F_NOI: return {c->noi, sizeof(int32_t)};
+        case MHPC_F_KEY: return {c->key, sizeof(uint64_t)};
+        case MHPC_F_SORTED: return {c->perm, sizeof(int32_t)};
+        case MHPC_F_LEVEL: return {c->level, sizeof(int32_t)};
+        case MHPC_F_H_ITERS: return {c->h_iters, sizeof(int32_t)};
+        default: return {nullptr, 0};
+    }
+}
+
+extern "C" int mhpc_upload(mhpc_ctx *c, int field, const void *host, int64_t n) {
+    if (!c || !host || n < 0 || n > c->n) return MHPC_E_ARG;
+    FieldDesc d = field_desc(c, field);
+    if (!d.ptr) return MHPC_E_ARG;
+    HIPC(hipMemcpyAsync(d.ptr, host, d.elem * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
+    c->nn_valid = c->rho_valid = c->eos_valid = false;
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_download(mhpc_ctx *c, int field, void *host, int64_t n) {
+    if (!c || !host || n < 0 || n > c->n) return MHPC_E_ARG;
+    FieldDesc d = field_desc(c, field);
+    if (!d.ptr) return MHPC_E_ARG;
+    HIPC(hipMemcpyAsync(host, d.ptr, d.elem * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_field_ptr(mhpc_ctx *c, int field, void **device_ptr) {
+    if (!c || !device_ptr) return MHPC_E_ARG;
+    FieldDesc d = field_desc(c, field);
+    if (!d.ptr) return MHPC_E_ARG;
+    *device_ptr = d.ptr;
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_get_box(mhpc_ctx *c, double box[6]) {
+    if (!c || !box) return MHPC_E_ARG;
+    HIPC(hipMemcpyAsync(box, c->box, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return MHPC_OK;
+}
+
+// ------------------------------------------------------------------------------------
+static void reset_flags(mhpc_ctx *c) {
+    launch_fill_u32(c, c->err, 0u, 1);
+    launch_fill_u64(c, c->counters, 0ull, CNT_COUNT_);
+}
+
+static int finish(mhpc_ctx *c) {
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(c->h_err, c->err, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipMemcpyAsync(c->h_counters, c->counters, sizeof(unsigned long long) * CNT_COUNT_,
+                        hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    const uint32_t e = *c->h_err;
+    if (e == 0) return MHPC_OK;
+    int st = MHPC_OK;
+    if (e & ERRF_DEPTH) st = MHPC_E_DEPTH;
+    else if (e & ERRF_CAP_NODES) st = MHPC_E_CAP_NODES;
+    else if (e & ERRF_STACK) st = MHPC_E_STACK;
+    else if (e & ERRF_CAP_NEIGHBOURS) st = MHPC_E_CAP_NEIGHBOURS;
+    else if (e & ERRF_RHO) st = MHPC_E_RHO_NONPOSITIVE;
+    c->last_error = mhpc_status_string(st);
+    return st;
+}
+
+static float elapsed(mhpc_ctx *c, int a, int b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) ms = 0.f;
+    return ms;
+}
+
+// one stage: reset flags, launch, time with events, sync, map errors
+template <typename F>
+static int run_stage(mhpc_ctx *c, float *ms, F &&fn) {
+    reset_flags(c);
+    HIPC(hipEventRecord(c->ev[0], c->stream));
+    int st = fn();
+    if (st != MHPC_OK) return st;
+    HIPC(hipEventRecord(c->ev[1], c->stream));
+    st = finish(c);
+    if (ms) *ms = elapsed(c, 0, 1);
+    return st;
+}
+
+extern "C" int mhpc_bbox(mhpc_ctx *c, double box[6], float *ms) {
+    if (!c) return MHPC_E_ARG;
+    int st = run_stage(c, ms, [&] {
+        launch_bbox(c);
+        return (int)MHPC_OK;
+    });
+    if (st == MHPC_OK && box) st = mhpc_get_box(c, box);
+    return st;
+}
+
+extern "C" int mhpc_keys(mhpc_ctx *c, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    int st = run_stage(c, ms, [&] {
+        launch_keys(c);
+        return (int)MHPC_OK;
+    });
+    c->keys_valid = st == MHPC_OK;
+    return st;
+}
+
+extern "C" int mhpc_sort(mhpc_ctx *c, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    if (!c->keys_valid) return MHPC_E_STATE;
+    int st = run_stage(c, ms, [&] {
+        launch_sort(c);
+        return (int)MHPC_OK;
+    });
+    c->sorted_valid = st == MHPC_OK;
+    return st;
+}
+
+extern "C" int mhpc_build_tree(mhpc_ctx *c, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    if (!c->sorted_valid) return MHPC_E_STATE;
+    int st = run_stage(c, ms, [&] {
+        launch_gather(c, true, true);
+        launch_tree(c);
+        launch_bottom_up(c, false);
+        return (int)MHPC_OK;
+    });
+    c->tree_valid = st == MHPC_OK;
+    return st;
+}
+
+extern "C" int mhpc_centres_of_mass(mhpc_ctx *c, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    if (!c->tree_valid) return MHPC_E_STATE;
+    int st = run_stage(c, ms, [&] {
+        launch_bottom_up(c, true);
+        return (int)MHPC_OK;
+    });
+    c->com_valid = st == MHPC_OK;
+    return st;
+}
+
+extern "C" int mhpc_gravity(mhpc_ctx *c, int potential, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    if (!c->com_valid) return MHPC_E_STATE;
+    return run_stage(c, ms, [&] {
+        launch_gravity(c, potential != 0);
+        return (int)MHPC_OK;
+    });
+}
+
+extern "C" int mhpc_h_iterate(mhpc_ctx *c, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    if (!c->tree_valid) return MHPC_E_STATE;
+    int st = run_stage(c, ms, [&] {
+        launch_h_iterate(c, c->p.max_counts);
+        launch_scatter_h(c);
+        return (int)MHPC_OK;
+    });
+    c->nn_valid = false;
+    return st;
+}
+
+extern "C" int mhpc_neighbours(mhpc_ctx *c, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    if (!c->tree_valid) return MHPC_E_STATE;
+    int st = ensure_nbr(c);
+    if (st != MHPC_OK) return st;
+    st = run_stage(c, ms, [&] {
+        launch_neighbours(c);
+        return (int)MHPC_OK;
+    });
+    c->nn_valid = st == MHPC_OK;
+    return st;
+}
+
+extern "C" int mhpc_density(mhpc_ctx *c, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    if (!c->nn_valid) return MHPC_E_STATE;
+    int st = run_stage(c, ms, [&] {
+        launch_density(c);
+        return (int)MHPC_OK;
+    });
+    c->rho_valid = st == MHPC_OK;
+    return st;
+}
+
+extern "C" int mhpc_eos(mhpc_ctx *c, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    if (!c->rho_valid) return MHPC_E_STATE;
+    int st = run_stage(c, ms, [&] {
+        launch_eos(c);
+        return (int)MHPC_OK;
+    });
+    c->eos_valid = st == MHPC_OK;
+    return st;
+}
+
+extern "C" int mhpc_internal_forces(mhpc_ctx *c, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    if (!c->eos_valid) return MHPC_E_STATE;
+    return run_stage(c, ms, [&] {
+        launch_forces(c);
+        return (int)MHPC_OK;
+    });
+}
+
+extern "C" int mhpc_leapfrog_update_x(mhpc_ctx *c, double dt) {
+    if (!c) return MHPC_E_ARG;
+    launch_leapfrog_x(c, dt);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->stream));
+    c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
+    c->nn_valid = c->rho_valid = c->eos_valid = false;
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_leapfrog_update_v(mhpc_ctx *c, double dt) {
+    if (!c) return MHPC_E_ARG;
+    launch_leapfrog_v(c, dt);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->stream));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_download_neighbours(mhpc_ctx *c, int32_t *nnl, int32_t *noi, int64_t n, int max_nb) {
+    if (!c || !nnl || !noi || n != c->n || max_nb < 1) return MHPC_E_ARG;
+    if (!c->nn_valid) return MHPC_E_STATE;
+    int32_t *d_nnl = nullptr, *d_noi = nullptr;
+    HIPC(dalloc(&d_nnl, n * (int64_t)max_nb));
+    HIPC(dalloc(&d_noi, n));
+    HIPC(hipMemsetAsync(d_nnl, 0xff, sizeof(int32_t) * (size_t)(n * max_nb), c->stream));
+    launch_neighbours_to_caller(c, d_nnl, d_noi, max_nb);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(nnl, d_nnl, sizeof(int32_t) * (size_t)(n * max_nb), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipMemcpyAsync(noi, d_noi, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    (void)hipFree(d_nnl);
+    (void)hipFree(d_noi);
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_tree_size(mhpc_ctx *c, int64_t *num_cells) {
+    if (!c || !num_cells) return MHPC_E_ARG;
+    if (!c->tree_valid) return MHPC_E_STATE;
+    int32_t cb[MHPC_LEVELS + 3];
+    HIPC(hipMemcpyAsync(cb, c->cell_base, sizeof(cb), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    *num_cells = cb[MHPC_LEVELS + 1];
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_download_tree(mhpc_ctx *c, int32_t *cell_level, int32_t *cell_start,
+                                  int32_t *cell_end, int32_t *cell_child, double *cell_mass,
+                                  double *cell_x, double *cell_y, double *cell_z,
+                                  double *cell_quad) {
+    int64_t nc = 0;
+    int st = mhpc_tree_size(c, &nc);
+    if (st != MHPC_OK) return st;
+    if (nc > c->cap_cells) return MHPC_E_CAP_NODES;
+    const size_t i4 = sizeof(int32_t) * (size_t)nc, d8 = sizeof(double) * (size_t)nc;
+    if (cell_level) HIPC(hipMemcpyAsync(cell_level, c->clevel, i4, hipMemcpyDeviceToHost, c->stream));
+    if (cell_start) HIPC(hipMemcpyAsync(cell_start, c->cstart, i4, hipMemcpyDeviceToHost, c->stream));
+    if (cell_end) HIPC(hipMemcpyAsync(cell_end, c->cend, i4, hipMemcpyDeviceToHost, c->stream));
+    if (cell_child) HIPC(hipMemcpyAsync(cell_child, c->cchild, 8 * i4, hipMemcpyDeviceToHost, c->stream));
+    if (cell_mass) HIPC(hipMemcpyAsync(cell_mass, c->cmass, d8, hipMemcpyDeviceToHost, c->stream));
+    if (cell_x) HIPC(hipMemcpyAsync(cell_x, c->cx, d8, hipMemcpyDeviceToHost, c->stream));
+    if (cell_y) HIPC(hipMemcpyAsync(cell_y, c->cy, d8, hipMemcpyDeviceToHost, c->stream));
+    if (cell_z) HIPC(hipMemcpyAsync(cell_z, c->cz, d8, hipMemcpyDeviceToHost, c->stream));
+    if (cell_quad)
+        for (int e = 0; e < 6; e++)
+            HIPC(hipMemcpyAsync(cell_quad + e * nc, c->cquad + e * c->cap_cells, d8, hipMemcpyDeviceToHost,
+                                c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return MHPC_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Miluphpc::rhs (miluphpc.cpp:387-562) for one subdomain: reset -> bounding box -> keys ->
+// sort -> tree -> [COM -> Barnes-Hut] -> [h-iteration -> neighbours -> density -> sound
+// speed / pressure -> internal forces].  No host synchronisation until the end.
+extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
+    if (!c) return MHPC_E_ARG;
+    const bool grav = flags & MHPC_RHS_GRAVITY;
+    const bool sph = flags & MHPC_RHS_SPH;
+    const bool hiter = sph && (flags & MHPC_RHS_VARIABLE_SML);
+    const bool pot = grav && (flags & MHPC_RHS_POTENTIAL);
+    if (sph) {
+        int st = ensure_nbr(c);
+        if (st != MHPC_OK) return st;
+    }
+    int e = 0;
+    auto mark = [&](int idx) { (void)hipEventRecord(c->ev[idx], c->stream); };
+    reset_flags(c);
+    mark(0);
+    launch_bbox(c);
+    mark(1);
+    launch_keys(c);
+    mark(2);
+    launch_sort(c);
+    mark(3);
+    launch_gather(c, grav, sph);
+    mark(4);
+    launch_tree(c);
+    mark(5);
+    launch_bottom_up(c, grav);
+    mark(6);
+    if (grav) launch_gravity(c, pot);
+    mark(7);
+    if (hiter) {
+        launch_h_iterate(c, c->p.max_counts);
+        launch_scatter_h(c);
+    }
+    mark(8);
+    if (sph) launch_neighbours(c);
+    mark(9);
+    if (sph) launch_density(c);
+    mark(10);
+    if (sph) launch_eos(c);
+    mark(11);
+    if (sph) launch_forces(c);
+    mark(12);
+    (void)e;
+    int st = finish(c);
+    c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = (st == MHPC_OK);
+    c->com_valid = c->com_valid && grav;
+    c->nn_valid = c->rho_valid = c->eos_valid = (st == MHPC_OK) && sph;
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->ms_bbox = elapsed(c, 0, 1);
+        stats->ms_keys = elapsed(c, 1, 2);
+        stats->ms_sort = elapsed(c, 2, 3);
+        stats->ms_gather = elapsed(c, 3, 4);
+        stats->ms_tree = elapsed(c, 4, 5);
+        stats->ms_com = elapsed(c, 5, 6);
+        stats->ms_gravity = elapsed(c, 6, 7);
+        stats->ms_hiter = elapsed(c, 7, 8);
+        stats->ms_nn = elapsed(c, 8, 9);
+        stats->ms_density = elapsed(c, 9, 10);
+        stats->ms_eos = elapsed(c, 10, 11);
+        stats->ms_forces = elapsed(c, 11, 12);
+        stats->ms_total = elapsed(c, 0, 12);
+        int32_t cb[MHPC_LEVELS + 3];
+        if (hipMemcpy(cb, c->cell_base, sizeof(cb), hipMemcpyDeviceToHost) == hipSuccess)
+            stats->num_cells = cb[MHPC_LEVELS + 1];
+        stats->n_cell_interactions = (int64_t)c->h_counters[CNT_CELL_INT];
+        stats->n_part_interactions = (int64_t)c->h_counters[CNT_PART_INT];
+        stats->n_neighbours = (int64_t)c->h_counters[CNT_NEIGHBOURS];
+        stats->max_noi = (int32_t)c->h_counters[CNT_MAX_NOI];
+        stats->n_h_counts = (int64_t)c->h_counters[CNT_H_COUNTS];
+        stats->h_passes = hiter ? c->p.max_counts : 0;
+    }
+    return st;
+}

@@ -1,0 +1,368 @@
+// k_gravity.hip -- Barnes-Hut force, wavefront-cooperative.
+//
+// The reference walks the tree once per particle (computeForces_v1, gravity.cu:252-396)
+// or once per warp of 32 sorted particles with a warp vote on the acceptance test
+// (computeForces_v2, gravity.cu:726-903).  Here one wavefront owns a GROUP of 64
+// consecutive key-sorted particles and walks the tree once for all of them:
+//   * the 64 lanes test up to 64 stacked nodes at a time against the group's bounding
+//     box (FP64, exact, the acceptance test the oracle restates), so the traversal
+//     itself is parallel across lanes;
+//   * leaves and accepted cells go to two LDS interaction lists shared by the wave;
+//   * each time a list holds 64 entries all 64 lanes evaluate it, every lane reading
+//     the same LDS entry (broadcast) for its own particle: FP32 pair math on FP64
+//     position differences, Kahan-compensated sums (or plain FP64 for the FP64 mode).
+// The stack lives in LDS; the number of nodes popped per round shrinks as the stack
+// fills so it cannot overflow (a hard bound still sets ERRF_STACK).
+#include "mhpc_internal.h"
+
+#define BH_WAVES 2                       // waves (groups) per block
+#define BH_STACK 768                     // soft stack capacity per wave
+#define BH_STACK_PHYS (BH_STACK + 192)   // hard capacity
+#define BH_LIST 128
+
+struct BHWaveLds {
+    int stack[BH_STACK_PHYS];
+    // particle list
+    double px[BH_LIST], py[BH_LIST], pz[BH_LIST];
+    double pm[BH_LIST];
+    int pid[BH_LIST];
+    // cell list
+    double cx[BH_LIST], cy[BH_LIST], cz[BH_LIST];
+    double cm[BH_LIST];
+    float4 q0[BH_LIST];
+    float2 q1[BH_LIST];
+};
+
+template <bool FP64>
+struct BHAcc;
+
+template <>
+struct BHAcc<false> {
+    Kahan x, y, z, phi;
+};
+
+template <>
+struct BHAcc<true> {
+    Plain64 x, y, z, phi;
+};
+
+// particle-particle (monopole) batch
+template <bool FP64, bool POT>
+__device__ __forceinline__ void eval_particles(const BHWaveLds &L, int cnt, double px, double py,
+                                               double pz, int self, double eps2, BHAcc<FP64> &acc) {
+    for (int e = 0; e < cnt; e++) {
+        if constexpr (FP64) {
+            const double dx = L.px[e] - px, dy = L.py[e] - py, dz = L.pz[e] - pz;
+            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            const double rinv = r2 > 0.0 ? 1.0 / sqrt(r2) : 0.0;
+            const double mr3 = L.pm[e] * rinv * rinv * rinv;
+            acc.x.add(mr3 * dx);
+            acc.y.add(mr3 * dy);
+            acc.z.add(mr3 * dz);
+            if (POT && L.pid[e] != self) acc.phi.add(-L.pm[e] * rinv);
+        } else {
+            const float dx = (float)(L.px[e] - px), dy = (float)(L.py[e] - py),
+                        dz = (float)(L.pz[e] - pz);
+            const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, (float)eps2)));
+            const float rinv = r2 > 0.f ? rsqrtf(r2) : 0.f;
+            const float mr = (float)L.pm[e] * rinv;
+            const float mr3 = mr * rinv * rinv;
+            acc.x.add(mr3 * dx);
+            acc.y.add(mr3 * dy);
+            acc.z.add(mr3 * dz);
+            if (POT && L.pid[e] != self) acc.phi.add(-mr);
+        }
+    }
+}
+
+// particle-cell batch: monopole (+ traceless quadrupole)
+template <bool FP64, bool POT, bool QUAD>
+__device__ __forceinline__ void eval_cells(const BHWaveLds &L, int cnt, double px, double py,
+                                           double pz, double eps2, BHAcc<FP64> &acc) {
+    for (int e = 0; e < cnt; e++) {
+        if constexpr (FP64) {
+            const double dx = L.cx[e] - px, dy = L.cy[e] - py, dz = L.cz[e] - pz;
+            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            const double rinv = r2 > 0.0 ? 1.0 / sqrt(r2) : 0.0;
+            const double rinv2 = rinv * rinv;
+            const double m = L.cm[e];
+            const double mr3 = m * rinv * rinv2;
+            double fx = mr3 * dx, fy = mr3 * dy, fz = mr3 * dz;
+            double ph = -m * rinv;
+            if (QUAD) {
+                const float4 a = L.q0[e];
+                const float2 b = L.q1[e];
+                const double qxx = a.x, qxy = a.y, qxz = a.z, qyy = a.w, qyz = b.x, qzz = b.y;
+                const double qdx = qxx * dx + qxy * dy + qxz * dz;
+                const double qdy = qxy * dx + qyy * dy + qyz * dz;
+                const double qdz = qxz * dx + qyz * dy + qzz * dz;
+                const double dqd = dx * qdx + dy * qdy + dz * qdz;
+                const double rinv5 = rinv2 * rinv2 * rinv;
+                const double t = 2.5 * dqd * rinv5 * rinv2;
+                fx += -qdx * rinv5 + t * dx;
+                fy += -qdy * rinv5 + t * dy;
+                fz += -qdz * rinv5 + t * dz;
+                ph += -0.5 * dqd * rinv5;
+            }
+            acc.x.add(fx);
+            acc.y.add(fy);
+            acc.z.add(fz);
+            if (POT) acc.phi.add(ph);
+        } else {
+            const float dx = (float)(L.cx[e] - px), dy = (float)(L.cy[e] - py),
+                        dz = (float)(L.cz[e] - pz);
+            const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, (float)eps2)));
+            const float rinv = r2 > 0.f ? rsqrtf(r2) : 0.f;
+            const float rinv2 = rinv * rinv;
+            const float m = (float)L.cm[e];
+            const float mr = m * rinv;
+            const float mr3 = mr * rinv2;
+            float fx = mr3 * dx, fy = mr3 * dy, fz = mr3 * dz;
+            float ph = -mr;
+            if (QUAD) {
+                const float4 a = L.q0[e];
+                const float2 b = L.q1[e];
+                const float qdx = __builtin_fmaf(a.x, dx, __builtin_fmaf(a.y, dy, a.z * dz));
+                const float qdy = __builtin_fmaf(a.y, dx, __builtin_fmaf(a.w, dy, b.x * dz));
+                const float qdz = __builtin_fmaf(a.z, dx, __builtin_fmaf(b.x, dy, b.y * dz));
+                const float dqd = __builtin_fmaf(dx, qdx, __builtin_fmaf(dy, qdy, dz * qdz));
+                const float rinv5 = rinv2 * rinv2 * rinv;
+                const float t = 2.5f * dqd * rinv5 * rinv2;
+                fx = __builtin_fmaf(t, dx, __builtin_fmaf(-qdx, rinv5, fx));
+                fy = __builtin_fmaf(t, dy, __builtin_fmaf(-qdy, rinv5, fy));
+                fz = __builtin_fmaf(t, dz, __builtin_fmaf(-qdz, rinv5, fz));
+                ph = __builtin_fmaf(-0.5f * dqd, rinv5, ph);
+            }
+            acc.x.add(fx);
+            acc.y.add(fy);
+            acc.z.add(fz);
+            if (POT) acc.phi.add(ph);
+        }
+    }
+}
+
+struct BHArgs {
+    int64_t n;
+    const int32_t *perm;
+    const double *sx, *sy, *sz, *sm;
+    const int32_t *clevel, *cchild;
+    const double *cmass, *cx, *cy, *cz;
+    const float4 *cquadf;
+    const double *box;
+    double theta, eps2, G;
+    int mac;
+    double *gax, *gay, *gaz, *u;
+    uint32_t *err;
+    unsigned long long *counters;
+};
+
+template <bool FP64, bool POT, bool QUAD>
+__global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
+    __shared__ BHWaveLds lds_all[BH_WAVES];
+    const int w = threadIdx.x / 64, lane = wave_lane();
+    BHWaveLds &L = lds_all[w];
+    const int64_t group = (int64_t)blockIdx.x * BH_WAVES + w;
+    const int64_t k = group * 64 + lane;
+    const int64_t n = A.n;
+    if (group * 64 >= n) return;   // whole wave exits together
+    const bool valid = k < n;
+    const double px = valid ? A.sx[k] : 0.0, py = valid ? A.sy[k] : 0.0, pz = valid ? A.sz[k] : 0.0;
+    // group bounding box (exact)
+    const double glo_x = wave_min(valid ? px : 1e300), ghi_x = wave_max(valid ? px : -1e300);
+    const double glo_y = wave_min(valid ? py : 1e300), ghi_y = wave_max(valid ? py : -1e300);
+    const double glo_z = wave_min(valid ? pz : 1e300), ghi_z = wave_max(valid ? pz : -1e300);
+    const int nvalid = popc64(wave_ballot(valid));
+    // acceptance constants, same expressions as the oracle
+    const double R = 0.5 * (A.box[1] - A.box[0]);
+    const double s2_0 = 4.0 * R * R;
+    const double dp_1 = R * R / A.theta;
+    const double theta2 = A.theta * A.theta;
+    const double eps2 = A.eps2;
+
+    BHAcc<FP64> acc;
+    int top = 1, pl_n = 0, cl_n = 0;
+    long long n_cells = 0, n_parts = 0;
+    if (lane == 0) L.stack[0] = (int)n;   // root
+    wave_sync();
+    const uint64_t lt = lanemask_lt();
+    bool overflow = false;
+    while (top > 0) {
+        int kpop = (BH_STACK - top) / 7;
+        kpop = kpop < 1 ? 1 : kpop;
+        kpop = kpop > 64 ? 64 : kpop;
+        kpop = kpop > top ? top : kpop;
+        const int node = lane < kpop ? L.stack[top - kpop + lane] : -1;
+        top -= kpop;
+        wave_sync();
+        const bool isPart = node >= 0 && node < n;
+        const bool isCell = node >= n;
+        bool accept = false, open = false;
+        double ccx = 0, ccy = 0, ccz = 0;
+        int64_t c = 0;
+        if (isCell) {
+            c = node - n;
+            ccx = A.cx[c];
+            ccy = A.cy[c];
+            ccz = A.cz[c];
+            const int lev = A.clevel[c];
+            double dx = 0.0, dy = 0.0, dz = 0.0;
+            if (ccx < glo_x) dx = glo_x - ccx; else if (ccx > ghi_x) dx = ccx - ghi_x;
+            if (ccy < glo_y) dy = glo_y - ccy; else if (ccy > ghi_y) dy = ccy - ghi_y;
+            if (ccz < glo_z) dz = glo_z - ccz; else if (ccz > ghi_z) dz = ccz - ghi_z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (A.mac == MHPC_MAC_CLASSIC) {
+                const double s2 = s2_0 * ldexp(1.0, -2 * lev);
+                accept = (d2 + eps2) * theta2 > s2;
+            } else {
+                const double dp = dp_1 * ldexp(1.0, -2 * (lev - 1));
+                accept = dp <= d2 + eps2;
+            }
+            open = !accept;
+        }
+        const uint64_t mP = wave_ballot(isPart), mA = wave_ballot(accept);
+        if (isPart) {
+            const int pos = pl_n + popc64(mP & lt);
+            L.px[pos] = A.sx[node];
+            L.py[pos] = A.sy[node];
+            L.pz[pos] = A.sz[node];
+            L.pm[pos] = A.sm[node];
+            L.pid[pos] = node;
+        }
+        if (accept) {
+            const int pos = cl_n + popc64(mA & lt);
+            L.cx[pos] = ccx;
+            L.cy[pos] = ccy;
+            L.cz[pos] = ccz;
+            L.cm[pos] = A.cmass[c];
+            if (QUAD) {
+                const float4 a = A.cquadf[2 * c], b = A.cquadf[2 * c + 1];
+                L.q0[pos] = a;
+                L.q1[pos] = make_float2(b.x, b.y);
+            }
+        }
+        pl_n += popc64(mP);
+        cl_n += popc64(mA);
+        n_parts += popc64(mP);
+        n_cells += popc64(mA);
+        // push the children of opened cells
+        int ch[8];
+        int nc = 0;
+        if (open) {
+            const int4 *cp = reinterpret_cast<const int4 *>(A.cchild + 8 * c);
+            const int4 a = cp[0], b = cp[1];
+            ch[0] = a.x; ch[1] = a.y; ch[2] = a.z; ch[3] = a.w;
+            ch[4] = b.x; ch[5] = b.y; ch[6] = b.z; ch[7] = b.w;
+#pragma unroll
+            for (int s = 0; s < 8; s++) nc += ch[s] != -1;
+        }
+        const int incl = wave_incl_scan(nc);
+        const int tot = __shfl(incl, 63);
+        if (top + tot > BH_STACK_PHYS) {
+            overflow = true;
+            break;
+        }
+        if (open) {
+            int pos = top + incl - nc;
+#pragma unroll
+            for (int s = 7; s >= 0; s--)
+                if (ch[s] != -1) L.stack[pos++] = ch[s];
+        }
+        top += tot;
+        wave_sync();
+        if (pl_n >= 64) {
+            eval_particles<FP64, POT>(L, 64, px, py, pz, (int)k, eps2, acc);
+            wave_sync();
+            const int rest = pl_n - 64;
+            if (lane < rest) {
+                const double a = L.px[64 + lane], b = L.py[64 + lane], d = L.pz[64 + lane];
+                const double m = L.pm[64 + lane];
+                const int id = L.pid[64 + lane];
+                L.px[lane] = a; L.py[lane] = b; L.pz[lane] = d; L.pm[lane] = m; L.pid[lane] = id;
+            }
+            pl_n = rest;
+            wave_sync();
+        }
+        if (cl_n >= 64) {
+            eval_cells<FP64, POT, QUAD>(L, 64, px, py, pz, eps2, acc);
+            wave_sync();
+            const int rest = cl_n - 64;
+            if (lane < rest) {
+                const double a = L.cx[64 + lane], b = L.cy[64 + lane], d = L.cz[64 + lane];
+                const double m = L.cm[64 + lane];
+                float4 q0 = make_float4(0, 0, 0, 0);
+                float2 q1 = make_float2(0, 0);
+                if (QUAD) {
+                    q0 = L.q0[64 + lane];
+                    q1 = L.q1[64 + lane];
+                }
+                L.cx[lane] = a; L.cy[lane] = b; L.cz[lane] = d; L.cm[lane] = m;
+                if (QUAD) {
+                    L.q0[lane] = q0;
+                    L.q1[lane] = q1;
+                }
+            }
+            cl_n = rest;
+            wave_sync();
+        }
+    }
+    if (overflow) {
+        if (lane == 0) set_err(A.err, ERRF_STACK);
+        return;
+    }
+    if (pl_n > 0) eval_particles<FP64, POT>(L, pl_n, px, py, pz, (int)k, eps2, acc);
+    if (cl_n > 0) eval_cells<FP64, POT, QUAD>(L, cl_n, px, py, pz, eps2, acc);
+    if (valid) {
+        const int32_t i = A.perm[k];
+        A.gax[i] = A.G * acc.x.value();
+        A.gay[i] = A.G * acc.y.value();
+        A.gaz[i] = A.G * acc.z.value();
+        if (POT) A.u[i] = 0.5 * A.sm[k] * A.G * acc.phi.value();
+    }
+    if (lane == 0) {
+        atomicAdd(&A.counters[CNT_CELL_INT], (unsigned long long)(n_cells * nvalid));
+        atomicAdd(&A.counters[CNT_PART_INT], (unsigned long long)(n_parts * nvalid));
+    }
+}
+
+template <bool FP64, bool POT, bool QUAD>
+static void launch_bh(mhpc_ctx *c, const BHArgs &a) {
+    const int64_t groups = ceil_div(c->n, 64);
+    hipLaunchKernelGGL((k_bh_walk<FP64, POT, QUAD>), dim3((unsigned)ceil_div(groups, BH_WAVES)),
+                       dim3(64 * BH_WAVES), 0, c->stream, a);
+}
+
+void launch_gravity(mhpc_ctx *c, bool potential) {
+    if (c->n <= 0) return;
+    BHArgs a;
+    a.n = c->n;
+    a.perm = c->perm;
+    a.sx = c->sx; a.sy = c->sy; a.sz = c->sz; a.sm = c->sm;
+    a.clevel = c->clevel;
+    a.cchild = c->cchild;
+    a.cmass = c->cmass; a.cx = c->cx; a.cy = c->cy; a.cz = c->cz;
+    a.cquadf = c->cquadf;
+    a.box = c->box;
+    a.theta = c->p.theta;
+    a.eps2 = c->p.smoothing;
+    a.G = c->p.G;
+    a.mac = c->p.mac;
+    a.gax = c->f[MHPC_F_G_AX]; a.gay = c->f[MHPC_F_G_AY]; a.gaz = c->f[MHPC_F_G_AZ];
+    a.u = c->f[MHPC_F_U];
+    a.err = c->err;
+    a.counters = c->counters;
+    const bool fp64 = c->p.precision == MHPC_PREC_FP64;
+    const bool quad = c->p.quadrupole != 0;
+    if (fp64) {
+        if (potential) {
+            if (quad) launch_bh<true, true, true>(c, a); else launch_bh<true, true, false>(c, a);
+        } else {
+            if (quad) launch_bh<true, false, true>(c, a); else launch_bh<true, false, false>(c, a);
+        }
+    } else {
+        if (potential) {
+            if (quad) launch_bh<false, true, true>(c, a); else launch_bh<false, true, false>(c, a);
+        } else {
+            if (quad) launch_bh<false, false, true>(c, a); else launch_bh<false, false, false>(c, a);
+        }
+    }
+}

@@ -1,0 +1,180 @@
+// k_sort.hip -- stable LSD radix sort of (64-bit key, 32-bit index) pairs.
+//
+// Replaces TreeNS::prepareSorting + HelperNS::sortArray = cub::DeviceRadixSort::SortPairs
+// (tree.cu:1404-1413, helper.cu:150-167): same result, a stable sort by key in which
+// ties keep ascending index order.  Eight passes of 8-bit digits.  Per pass:
+//   count   -- digit histogram of each 4096-key tile (LDS atomics);
+//   scan    -- one workgroup per digit scans the tile histograms (digit-major), adding
+//              the digit's global offset from the one-read histogram of all passes;
+//   scatter -- each wave ranks its 16 slots of 64 keys with a ballot multisplit (peers
+//              = lanes holding the same digit), the block stages the tile in LDS in
+//              digit order and writes each digit run contiguously.
+// HBM bytes per pass: count reads 8 B/key, scatter reads 12 and writes 12.
+#include "mhpc_internal.h"
+
+#define SORT_BLOCK 256
+#define SORT_ITEMS (MHPC_SORT_TILE / SORT_BLOCK)   // 16
+#define SORT_WAVES (SORT_BLOCK / 64)
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_hist_all(const uint64_t *__restrict__ keys,
+                                                              int64_t n,
+                                                              uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[MHPC_SORT_PASSES * MHPC_SORT_BUCKETS];
+    for (int t = threadIdx.x; t < MHPC_SORT_PASSES * MHPC_SORT_BUCKETS; t += SORT_BLOCK) h[t] = 0;
+    __syncthreads();
+    int64_t i = blockIdx.x * (int64_t)SORT_BLOCK + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * SORT_BLOCK;
+    for (; i < n; i += stride) {
+        const uint64_t k = keys[i];
+#pragma unroll
+        for (int p = 0; p < MHPC_SORT_PASSES; p++)
+            atomicAdd(&h[p * MHPC_SORT_BUCKETS + (int)((k >> (MHPC_SORT_BITS * p)) & 255u)], 1u);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < MHPC_SORT_PASSES * MHPC_SORT_BUCKETS; t += SORT_BLOCK)
+        if (h[t]) atomicAdd(&hist[t], h[t]);
+}
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_count(const uint64_t *__restrict__ keys,
+                                                           int64_t n, int shift,
+                                                           uint32_t *__restrict__ tile_hist,
+                                                           int64_t ntiles) {
+    __shared__ uint32_t h[MHPC_SORT_BUCKETS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * MHPC_SORT_TILE;
+#pragma unroll 4
+    for (int s = 0; s < SORT_ITEMS; s++) {
+        int64_t idx = base + s * SORT_BLOCK + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(int)((keys[idx] >> shift) & 255u)], 1u);
+    }
+    __syncthreads();
+    tile_hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_scan(const uint32_t *__restrict__ tile_hist,
+                                                          uint32_t *__restrict__ tile_off,
+                                                          int64_t ntiles,
+                                                          const uint32_t *__restrict__ digit_hist) {
+    __shared__ uint32_t lds4[SORT_WAVES];
+    const int d = blockIdx.x;
+    // global offset of digit d = number of keys with a smaller digit in this pass
+    uint32_t part = (threadIdx.x < (unsigned)d) ? digit_hist[threadIdx.x] : 0u;
+    uint32_t dummy;
+    uint32_t carry = block_excl_scan256(part, &dummy, lds4);
+    block_scan_row(tile_hist + (int64_t)d * ntiles, tile_off + (int64_t)d * ntiles, ntiles, carry,
+                   lds4);
+}
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_scatter(const uint64_t *__restrict__ kin,
+                                                             const int32_t *__restrict__ vin,
+                                                             uint64_t *__restrict__ kout,
+                                                             int32_t *__restrict__ vout, int64_t n,
+                                                             int shift,
+                                                             const uint32_t *__restrict__ tile_off,
+                                                             int64_t ntiles) {
+    __shared__ uint64_t skeys[MHPC_SORT_TILE];
+    __shared__ int32_t svals[MHPC_SORT_TILE];
+    __shared__ uint32_t wcnt[SORT_WAVES][MHPC_SORT_BUCKETS];
+    __shared__ uint32_t dstart[MHPC_SORT_BUCKETS];
+    __shared__ uint32_t goff[MHPC_SORT_BUCKETS];
+    __shared__ uint32_t lds4[SORT_WAVES];
+
+    const int tid = threadIdx.x, w = tid / 64, lane = wave_lane();
+    const int64_t base = (int64_t)blockIdx.x * MHPC_SORT_TILE;
+#pragma unroll
+    for (int k = 0; k < SORT_WAVES; k++) wcnt[k][tid] = 0;
+    goff[tid] = tile_off[(int64_t)tid * ntiles + blockIdx.x];
+    __syncthreads();
+
+    uint64_t kr[SORT_ITEMS];
+    int32_t vr[SORT_ITEMS];
+    uint32_t rank[SORT_ITEMS];
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int s = 0; s < SORT_ITEMS; s++) {
+        const int64_t idx = base + (int64_t)w * (64 * SORT_ITEMS) + s * 64 + lane;
+        const bool valid = idx < n;
+        const uint64_t key = valid ? kin[idx] : 0ull;
+        const int32_t val = valid ? (vin ? vin[idx] : (int32_t)idx) : 0;
+        const uint32_t d = (uint32_t)((key >> shift) & 255u);
+        uint64_t peers = wave_ballot(valid);
+#pragma unroll
+        for (int b = 0; b < MHPC_SORT_BITS; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = wave_ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t r = (uint32_t)popc64(peers & lt);
+        uint32_t prior = 0;
+        if (valid) prior = wcnt[w][d];
+        wave_sync();
+        if (valid && r == 0) wcnt[w][d] = prior + (uint32_t)popc64(peers);
+        wave_sync();
+        kr[s] = key;
+        vr[s] = val;
+        rank[s] = prior + r;
+    }
+    __syncthreads();
+    {
+        const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
+        const uint32_t tot = c0 + c1 + c2 + c3;
+        __syncthreads();
+        wcnt[0][tid] = 0;
+        wcnt[1][tid] = c0;
+        wcnt[2][tid] = c0 + c1;
+        wcnt[3][tid] = c0 + c1 + c2;
+        uint32_t excl;
+        block_excl_scan256(tot, &excl, lds4);
+        dstart[tid] = excl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SORT_ITEMS; s++) {
+        const int64_t idx = base + (int64_t)w * (64 * SORT_ITEMS) + s * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (uint32_t)((kr[s] >> shift) & 255u);
+            const uint32_t pos = dstart[d] + wcnt[w][d] + rank[s];
+            skeys[pos] = kr[s];
+            svals[pos] = vr[s];
+        }
+    }
+    __syncthreads();
+    const int64_t rem = n - base;
+    const int cnt = rem < MHPC_SORT_TILE ? (int)rem : MHPC_SORT_TILE;
+    for (int t = tid; t < cnt; t += SORT_BLOCK) {
+        const uint64_t key = skeys[t];
+        const uint32_t d = (uint32_t)((key >> shift) & 255u);
+        const uint32_t o = goff[d] + ((uint32_t)t - dstart[d]);
+        if (o < (uint64_t)n) {
+            kout[o] = key;
+            vout[o] = svals[t];
+        }
+    }
+}
+
+void launch_sort(mhpc_ctx *c) {
+    const int64_t n = c->n;
+    if (n <= 0) return;
+    const int64_t ntiles = ceil_div(n, MHPC_SORT_TILE);
+    launch_fill_u32(c, c->digit_hist, 0u, MHPC_SORT_PASSES * MHPC_SORT_BUCKETS);
+    int hb = (int)(ceil_div(n, SORT_BLOCK * 16) < 1024 ? ceil_div(n, SORT_BLOCK * 16) : 1024);
+    hipLaunchKernelGGL(k_sort_hist_all, dim3(hb), dim3(SORT_BLOCK), 0, c->stream, c->key, n,
+                       c->digit_hist);
+    const uint64_t *kin = c->key;
+    const int32_t *vin = nullptr;
+    for (int pass = 0; pass < MHPC_SORT_PASSES; pass++) {
+        uint64_t *kout = (pass & 1) ? c->skey : c->keyB;
+        int32_t *vout = (pass & 1) ? c->perm : c->valB;
+        const int shift = MHPC_SORT_BITS * pass;
+        hipLaunchKernelGGL(k_sort_count, dim3((unsigned)ntiles), dim3(SORT_BLOCK), 0, c->stream, kin, n,
+                           shift, c->tile_hist, ntiles);
+        hipLaunchKernelGGL(k_sort_scan, dim3(MHPC_SORT_BUCKETS), dim3(SORT_BLOCK), 0, c->stream,
+                           c->tile_hist, c->tile_off, ntiles,
+                           c->digit_hist + pass * MHPC_SORT_BUCKETS);
+        hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)ntiles), dim3(SORT_BLOCK), 0, c->stream, kin,
+                           vin, kout, vout, n, shift, c->tile_off, ntiles);
+        kin = kout;
+        vin = vout;
+    }
+}

@@ -1,0 +1,313 @@
+// k_tree.hip -- octree from sorted keys, and the bottom-up centre-of-mass pass.
+//
+// The reference inserts particles with locks (TreeNS::Kernel::buildTree, tree.cu:768-1082)
+// and gets a scheduling-dependent numbering of a deterministic structure: a cell for
+// every key prefix (level L, first L octant digits) shared by at least two particles,
+// each leaf at the first level where its prefix is unique.  With the particles in key
+// order every such cell is a contiguous run [start, end].  Let l[k] be the number of
+// leading octant digits that sorted keys k and k+1 share.  A level-L cell starts at k iff
+// l[k-1] < L <= l[k] and ends at k iff l[k] < L <= l[k-1].  So:
+//   count  -- per 1024-position tile, the number of cell starts at each level;
+//   scan   -- per level, exclusive scan over tiles (one workgroup per level);
+//   cells  -- every position ranks its start / end events among the level's starts
+//             (ballot prefix sums) and writes cell start, end, level: cells come out in
+//             level-major order, sorted by start within a level;
+//   link   -- every cell and particle finds its parent by binary search over the
+//             parent level's starts and fills the parent's child slot (octant = Lebesgue
+//             digit, the reference's slot convention).
+// No locks, no retries, and the same input always gives the same numbering.
+// The bottom-up pass then runs level 21..0: mass, centre of mass (children in slot
+// order, FP64, as calculateCentersOfMass, tree.cu:1416-1497), traceless quadrupole and
+// the particles' bounding box per cell.
+#include "mhpc_internal.h"
+
+#define TREE_BLOCK 256
+#define TREE_ITEMS (MHPC_TREE_TILE / TREE_BLOCK)   // 4
+
+__device__ __forceinline__ int lcp_levels(uint64_t a, uint64_t b) {
+    const uint64_t x = a ^ b;
+    if (x == 0) return MHPC_LEVELS;
+    return (__clzll((long long)x) - 1) / 3;   // keys use bits 0..62
+}
+
+__global__ __launch_bounds__(TREE_BLOCK) void k_tree_count(const uint64_t *__restrict__ skey,
+                                                           int64_t n, uint8_t *__restrict__ lcp,
+                                                           uint32_t *__restrict__ tcnt,
+                                                           int64_t ntiles, uint32_t *err) {
+    __shared__ uint32_t cnt[MHPC_LEVELS + 1];
+    if (threadIdx.x <= MHPC_LEVELS) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * MHPC_TREE_TILE;
+#pragma unroll
+    for (int s = 0; s < TREE_ITEMS; s++) {
+        const int64_t k = base + s * TREE_BLOCK + threadIdx.x;
+        if (k < n) {
+            const uint64_t kk = skey[k];
+            const int a = (k > 0) ? lcp_levels(skey[k - 1], kk) : 0;
+            const int b = (k + 1 < n) ? lcp_levels(kk, skey[k + 1]) : 0;
+            lcp[k] = (uint8_t)b;
+            if (b >= MHPC_LEVELS) set_err(err, ERRF_DEPTH);
+            for (int L = a + 1; L <= b; L++) atomicAdd(&cnt[L], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x >= 1 && threadIdx.x <= MHPC_LEVELS)
+        tcnt[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(TREE_BLOCK) void k_tree_scan(const uint32_t *__restrict__ tcnt,
+                                                          uint32_t *__restrict__ toff,
+                                                          int64_t ntiles,
+                                                          int32_t *__restrict__ level_count) {
+    __shared__ uint32_t lds4[4];
+    const int L = blockIdx.x + 1;
+    uint32_t tot = block_scan_row(tcnt + (int64_t)L * ntiles, toff + (int64_t)L * ntiles, ntiles, 0u,
+                                  lds4);
+    if (threadIdx.x == 0) level_count[L] = (int32_t)tot;
+}
+
+__global__ __launch_bounds__(TREE_BLOCK) void k_tree_cells(
+    const uint8_t *__restrict__ lcp, int64_t n, const uint32_t *__restrict__ toff, int64_t ntiles,
+    const int32_t *__restrict__ level_count, int32_t *__restrict__ cell_base_out,
+    int32_t *__restrict__ cstart, int32_t *__restrict__ cend, int32_t *__restrict__ clevel,
+    int32_t *__restrict__ cchild, int64_t cap_cells, uint32_t *err) {
+    __shared__ int32_t cb[MHPC_LEVELS + 3];
+    __shared__ uint32_t pre[TREE_ITEMS][4][MHPC_LEVELS + 1];
+    const int tid = threadIdx.x, w = tid / 64, lane = wave_lane();
+    if (tid == 0) {
+        cb[0] = 0;
+        cb[1] = 1;
+        for (int L = 1; L <= MHPC_LEVELS; L++) cb[L + 1] = cb[L] + level_count[L];
+        cb[MHPC_LEVELS + 2] = cb[MHPC_LEVELS + 1];
+        if (blockIdx.x == 0) {
+            for (int L = 0; L <= MHPC_LEVELS + 2; L++) cell_base_out[L] = cb[L];
+            if ((int64_t)cb[MHPC_LEVELS + 1] > cap_cells) set_err(err, ERRF_CAP_NODES);
+            // root: level 0, all particles
+            cstart[0] = 0;
+            cend[0] = (int32_t)(n - 1);
+            clevel[0] = 0;
+            for (int s = 0; s < 8; s++) cchild[s] = -1;
+        }
+    }
+    const int64_t base = (int64_t)blockIdx.x * MHPC_TREE_TILE;
+    int a[TREE_ITEMS], b[TREE_ITEMS];
+    int lmax = 0;
+#pragma unroll
+    for (int s = 0; s < TREE_ITEMS; s++) {
+        const int64_t k = base + s * TREE_BLOCK + tid;
+        a[s] = (k < n && k > 0) ? (int)lcp[k - 1] : 0;
+        b[s] = (k < n) ? (int)lcp[k] : 0;
+        lmax = max(lmax, max(a[s], b[s]));
+    }
+    lmax = wave_max(lmax);
+    // per (slot, wave, level) start counts
+#pragma unroll
+    for (int s = 0; s < TREE_ITEMS; s++) {
+        for (int L = 1; L <= MHPC_LEVELS; L++) {
+            const uint64_t m = (L <= lmax) ? wave_ballot(a[s] < L && L <= b[s]) : 0ull;
+            if (lane == 0) pre[s][w][L] = (uint32_t)popc64(m);
+        }
+    }
+    __syncthreads();
+    if (tid >= 1 && tid <= MHPC_LEVELS) {
+        uint32_t run = 0;
+        for (int s = 0; s < TREE_ITEMS; s++)
+            for (int ww = 0; ww < 4; ww++) {
+                uint32_t v = pre[s][ww][tid];
+                pre[s][ww][tid] = run;
+                run += v;
+            }
+    }
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int s = 0; s < TREE_ITEMS; s++) {
+        const int64_t k = base + s * TREE_BLOCK + tid;
+        for (int L = 1; L <= lmax; L++) {
+            const bool st = a[s] < L && L <= b[s];
+            const bool en = b[s] < L && L <= a[s];
+            const uint64_t m = wave_ballot(st);
+            if (st || en) {
+                const int64_t before =
+                    (int64_t)toff[(int64_t)L * ntiles + blockIdx.x] + pre[s][w][L] + popc64(m & lt);
+                if (st) {
+                    const int64_t c = cb[L] + before;
+                    if (c < cap_cells) {
+                        cstart[c] = (int32_t)k;
+                        clevel[c] = L;
+#pragma unroll
+                        for (int q = 0; q < 8; q++) cchild[8 * c + q] = -1;
+                    }
+                } else {
+                    const int64_t c = cb[L] + before - 1;
+                    if (c < cap_cells) cend[c] = (int32_t)k;
+                }
+            }
+        }
+    }
+}
+
+// last cell of level P whose start is <= k (cells of a level are sorted by start)
+__device__ __forceinline__ int32_t find_cell(const int32_t *__restrict__ cstart,
+                                             const int32_t *__restrict__ cb, int P, int32_t k) {
+    int32_t lo = cb[P], hi = cb[P + 1] - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (cstart[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(TREE_BLOCK) void k_tree_link(
+    const uint8_t *__restrict__ lcp, const uint64_t *__restrict__ slkey,
+    const int32_t *__restrict__ perm, int64_t n, const int32_t *__restrict__ cell_base,
+    const int32_t *__restrict__ cstart, const int32_t *__restrict__ clevel,
+    int32_t *__restrict__ cchild, int32_t *__restrict__ plevel, int64_t cap_cells) {
+    __shared__ int32_t cb[MHPC_LEVELS + 3];
+    if (threadIdx.x < MHPC_LEVELS + 3) cb[threadIdx.x] = cell_base[threadIdx.x];
+    __syncthreads();
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t ncells = cb[MHPC_LEVELS + 1];
+    if (ncells > cap_cells) return;   // capacity error already flagged; cells are incomplete
+    if (t < n) {
+        const int32_t k = (int32_t)t;
+        const int a = (k > 0) ? (int)lcp[k - 1] : 0;
+        const int b = (int)lcp[k];
+        const int Lp = max(a, b) + 1;
+        plevel[perm[k]] = Lp;
+        if (Lp > MHPC_LEVELS) return;   // duplicate key, flagged by k_tree_count
+        const int P = Lp - 1;
+        const int32_t parent = (P == 0) ? 0 : find_cell(cstart, cb, P, k);
+        if (parent >= cap_cells) return;
+        const int digit = (int)((slkey[k] >> (3 * (MHPC_LEVELS - Lp))) & 7u);
+        cchild[8 * (int64_t)parent + digit] = k;
+    } else {
+        const int64_t c = t - n + 1;   // cells 1 .. ncells-1
+        if (c >= ncells) return;
+        const int L = clevel[c];
+        const int32_t s = cstart[c];
+        const int32_t parent = (L == 1) ? 0 : find_cell(cstart, cb, L - 1, s);
+        const int digit = (int)((slkey[s] >> (3 * (MHPC_LEVELS - L))) & 7u);
+        cchild[8 * (int64_t)parent + digit] = (int32_t)(n + c);
+    }
+}
+
+void launch_tree(mhpc_ctx *c) {
+    const int64_t n = c->n;
+    if (n <= 0) return;
+    const int64_t ntiles = ceil_div(n, MHPC_TREE_TILE);
+    const uint64_t *slkey = (c->p.curve == MHPC_CURVE_HILBERT) ? c->slkey : c->skey;
+    hipLaunchKernelGGL(k_tree_count, dim3((unsigned)ntiles), dim3(TREE_BLOCK), 0, c->stream, c->skey, n,
+                       c->lcp, c->tcnt, ntiles, c->err);
+    hipLaunchKernelGGL(k_tree_scan, dim3(MHPC_LEVELS), dim3(TREE_BLOCK), 0, c->stream, c->tcnt, c->toff,
+                       ntiles, c->level_count);
+    hipLaunchKernelGGL(k_tree_cells, dim3((unsigned)ntiles), dim3(TREE_BLOCK), 0, c->stream, c->lcp, n,
+                       c->toff, ntiles, c->level_count, c->cell_base, c->cstart, c->cend, c->clevel,
+                       c->cchild, c->cap_cells, c->err);
+    const int64_t items = n + c->cap_cells;
+    hipLaunchKernelGGL(k_tree_link, dim3((unsigned)ceil_div(items, TREE_BLOCK)), dim3(TREE_BLOCK), 0,
+                       c->stream, c->lcp, slkey, c->perm, n, c->cell_base, c->cstart, c->clevel,
+                       c->cchild, c->level, c->cap_cells);
+}
+
+// ------------------------------------------------------------------------------------
+// bottom-up, one launch per level (cells of a level are contiguous): FP64 mass and
+// centre of mass over the children in slot order (tree.cu:1416-1497), traceless
+// quadrupole about the COM with the parallel-axis shift, and the particles' AABB.
+// Each launch reads its level's range from cell_base on the device.
+__global__ __launch_bounds__(256) void k_bottom_up(
+    int L, int64_t n, const int32_t *__restrict__ cell_base, const int32_t *__restrict__ cchild,
+    const double *__restrict__ sx, const double *__restrict__ sy, const double *__restrict__ sz,
+    const double *__restrict__ sm, double *__restrict__ cmass, double *__restrict__ cx,
+    double *__restrict__ cy, double *__restrict__ cz, double *__restrict__ cquad,
+    float4 *__restrict__ cquadf, double *__restrict__ cbmin, double *__restrict__ cbmax,
+    int64_t cap_cells, int gravity, int quad) {
+    const int64_t c0 = cell_base[L], c1a = cell_base[L + 1];
+    const int64_t c1 = c1a < cap_cells ? c1a : cap_cells;
+    for (int64_t c = c0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < c1;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        int32_t ch[8];
+        const int4 *cp = reinterpret_cast<const int4 *>(cchild + 8 * c);
+        const int4 q0 = cp[0], q1 = cp[1];
+        ch[0] = q0.x; ch[1] = q0.y; ch[2] = q0.z; ch[3] = q0.w;
+        ch[4] = q1.x; ch[5] = q1.y; ch[6] = q1.z; ch[7] = q1.w;
+        double bmin[3] = {1e300, 1e300, 1e300}, bmax[3] = {-1e300, -1e300, -1e300};
+        double m = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int32_t k = ch[s];
+            if (k == -1) continue;
+            double px, py, pz, pm, lo[3], hi[3];
+            if (k < n) {
+                px = sx[k]; py = sy[k]; pz = sz[k]; pm = sm[k];
+                lo[0] = hi[0] = px; lo[1] = hi[1] = py; lo[2] = hi[2] = pz;
+            } else {
+                const int64_t cc = k - n;
+                px = cx[cc]; py = cy[cc]; pz = cz[cc]; pm = cmass[cc];
+                lo[0] = cbmin[cc]; lo[1] = cbmin[cap_cells + cc]; lo[2] = cbmin[2 * cap_cells + cc];
+                hi[0] = cbmax[cc]; hi[1] = cbmax[cap_cells + cc]; hi[2] = cbmax[2 * cap_cells + cc];
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                bmin[a] = fmin(bmin[a], lo[a]);
+                bmax[a] = fmax(bmax[a], hi[a]);
+            }
+            if (gravity) {
+                mx += pm * px;
+                my += pm * py;
+                mz += pm * pz;
+                m += pm;
+            }
+        }
+        cbmin[c] = bmin[0]; cbmin[cap_cells + c] = bmin[1]; cbmin[2 * cap_cells + c] = bmin[2];
+        cbmax[c] = bmax[0]; cbmax[cap_cells + c] = bmax[1]; cbmax[2 * cap_cells + c] = bmax[2];
+        if (!gravity) continue;
+        if (m > 0.0) { mx /= m; my /= m; mz /= m; }
+        cmass[c] = m;
+        cx[c] = mx;
+        cy[c] = my;
+        cz[c] = mz;
+        if (!quad) continue;
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int32_t k = ch[s];
+            if (k == -1) continue;
+            double px, py, pz, pm;
+            double q[6] = {0, 0, 0, 0, 0, 0};
+            if (k < n) {
+                px = sx[k]; py = sy[k]; pz = sz[k]; pm = sm[k];
+            } else {
+                const int64_t cc = k - n;
+                px = cx[cc]; py = cy[cc]; pz = cz[cc]; pm = cmass[cc];
+#pragma unroll
+                for (int e = 0; e < 6; e++) q[e] = cquad[e * cap_cells + cc];
+            }
+            const double dx = px - mx, dy = py - my, dz = pz - mz;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            a0 += q[0] + pm * (3.0 * dx * dx - d2);
+            a1 += q[1] + pm * (3.0 * dx * dy);
+            a2 += q[2] + pm * (3.0 * dx * dz);
+            a3 += q[3] + pm * (3.0 * dy * dy - d2);
+            a4 += q[4] + pm * (3.0 * dy * dz);
+            a5 += q[5] + pm * (3.0 * dz * dz - d2);
+        }
+        cquad[c] = a0; cquad[cap_cells + c] = a1; cquad[2 * cap_cells + c] = a2;
+        cquad[3 * cap_cells + c] = a3; cquad[4 * cap_cells + c] = a4; cquad[5 * cap_cells + c] = a5;
+        cquadf[2 * c] = make_float4((float)a0, (float)a1, (float)a2, (float)a3);
+        cquadf[2 * c + 1] = make_float4((float)a4, (float)a5, 0.f, 0.f);
+    }
+}
+
+void launch_bottom_up(mhpc_ctx *c, bool gravity) {
+    if (c->n <= 0) return;
+    int grid = (int)ceil_div(c->n, 256 * 4);
+    if (grid < 1) grid = 1;
+    if (grid > 2048) grid = 2048;
+    for (int L = MHPC_LEVELS; L >= 0; L--) {
+        hipLaunchKernelGGL(k_bottom_up, dim3(L <= 3 ? 1 : grid), dim3(256), 0, c->stream, L, c->n,
+                           c->cell_base, c->cchild, c->sx, c->sy, c->sz, c->sm, c->cmass, c->cx, c->cy,
+                           c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax, c->cap_cells,
+                           gravity ? 1 : 0, (gravity && c->p.quadrupole) ? 1 : 0);
+    }
+}

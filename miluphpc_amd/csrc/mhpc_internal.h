@@ -1,0 +1,269 @@
+// mhpc_internal.h -- context layout and wave64 device helpers shared by the kernels.
+//
+// Data layout in HBM (one context = one subdomain):
+//  * caller-order SoA fields (FP64, plus int32 noi / level / h_iters, uint64 key), the
+//    arrays the reference's Particles class holds (particles.cuh:50-264);
+//  * key-sorted working copies s* of the fields the tree, gravity and SPH kernels read,
+//    so every group of 64 consecutive particles is spatially compact and each octree
+//    cell is a contiguous range [cstart, cend] of sorted positions;
+//  * the octree in level-major cell order (cell 0 = root), child slots in Lebesgue
+//    octant order (tree.cu:768-1082), node ids: particles [0,n), cells n + c;
+//  * neighbour lists transposed, nbr[k * nbr_stride + i] (sorted indices), so a wave
+//    reading its k-th neighbours touches one contiguous 256 B row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/mhpc.h"
+
+#ifndef MHPC_WAVE
+#define MHPC_WAVE 64
+#endif
+
+#define MHPC_LEVELS 21          // key levels (MAX_LEVEL, parameter.h)
+#define MHPC_SORT_BITS 8
+#define MHPC_SORT_BUCKETS 256
+#define MHPC_SORT_PASSES 8
+#define MHPC_SORT_TILE 4096     // keys per scatter block (256 threads x 16)
+#define MHPC_TREE_TILE 1024     // positions per tree block (256 threads x 4)
+#define MHPC_HTABLE_MAX 4096    // h-iteration factor table size
+#define MHPC_NUM_EVENTS 24
+
+// device error flags (sticky word, mapped onto mhpc_status by the host)
+enum : uint32_t {
+    ERRF_CAP_NODES = 1u,
+    ERRF_CAP_NEIGHBOURS = 2u,
+    ERRF_DEPTH = 4u,
+    ERRF_RHO = 8u,
+    ERRF_STACK = 16u,
+};
+
+// device counters (uint64) for statistics
+enum {
+    CNT_CELL_INT = 0,
+    CNT_PART_INT = 1,
+    CNT_NEIGHBOURS = 2,
+    CNT_MAX_NOI = 3,
+    CNT_H_COUNTS = 4,
+    CNT_COUNT_ = 8
+};
+
+struct mhpc_ctx {
+    mhpc_params p;
+    int64_t cap = 0, cap_pad = 0, cap_cells = 0, n = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // caller-order fields
+    double *f[MHPC_F_COUNT_] = {};   // FP64 fields (ids < MHPC_F_NOI)
+    int32_t *noi = nullptr, *level = nullptr, *h_iters = nullptr;
+    uint64_t *key = nullptr;         // sort key per particle (caller order)
+    uint64_t *lkey = nullptr;        // Lebesgue key per particle (caller order)
+
+    // sort
+    uint64_t *skey = nullptr, *keyB = nullptr;   // sorted keys / ping-pong buffer
+    int32_t *perm = nullptr, *valB = nullptr;    // sorted position -> particle index
+    uint32_t *tile_hist = nullptr, *tile_off = nullptr;
+    uint32_t *digit_hist = nullptr;              // [passes][256]
+    int64_t sort_tiles_cap = 0;
+    uint64_t *slkey = nullptr;                   // Lebesgue keys in sorted order
+
+    // sorted working copies
+    double *sx = nullptr, *sy = nullptr, *sz = nullptr, *sm = nullptr;
+    double *sh = nullptr, *sh2 = nullptr;
+    double *svx = nullptr, *svy = nullptr, *svz = nullptr, *se = nullptr;
+    double *srho = nullptr, *sp = nullptr, *scs = nullptr;
+    int32_t *snoi = nullptr, *siters = nullptr;
+    uint8_t *sdone = nullptr;
+
+    // tree
+    uint8_t *lcp = nullptr;
+    uint32_t *tcnt = nullptr, *toff = nullptr;   // [22][tree tiles]
+    int64_t tree_tiles_cap = 0;
+    int32_t *level_count = nullptr;              // [22]
+    int32_t *cell_base = nullptr;                // [24] level-major offsets, [23] = total
+    int32_t *cstart = nullptr, *cend = nullptr, *clevel = nullptr, *cchild = nullptr;
+    double *cmass = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr;
+    double *cquad = nullptr;                     // [6][cap_cells]
+    float4 *cquadf = nullptr;                    // packed FP32 quadrupole (xx,xy,xz,yy),(yz,zz,-,-)
+    double *cbmin = nullptr, *cbmax = nullptr;   // [3][cap_cells] particle AABB per cell
+
+    // neighbours
+    int32_t *nbr = nullptr;                      // [max_nb][cap_pad]
+    int nbr_rows = 0;
+
+    // h-iteration
+    double *htable = nullptr;
+
+    // misc
+    double *bbox_partial = nullptr;              // [2][bbox blocks]
+    double *box = nullptr;                       // [6]
+    uint32_t *err = nullptr;
+    unsigned long long *counters = nullptr;
+    uint32_t *h_err = nullptr;                   // pinned
+    unsigned long long *h_counters = nullptr;    // pinned
+    hipEvent_t ev[MHPC_NUM_EVENTS] = {};
+
+    bool keys_valid = false, sorted_valid = false, tree_valid = false, com_valid = false;
+    bool nn_valid = false, rho_valid = false, eos_valid = false;
+    std::string last_error;
+};
+
+// ------------------------------------------------------------------------------------
+// host-side launchers (defined in the k_*.hip files)
+void launch_fill_u32(mhpc_ctx *c, uint32_t *p, uint32_t v, int64_t n);
+void launch_fill_u64(mhpc_ctx *c, unsigned long long *p, unsigned long long v, int64_t n);
+void launch_bbox(mhpc_ctx *c);
+void launch_keys(mhpc_ctx *c);
+void launch_sort(mhpc_ctx *c);
+void launch_gather(mhpc_ctx *c, bool gravity, bool sph);
+void launch_tree(mhpc_ctx *c);
+void launch_bottom_up(mhpc_ctx *c, bool gravity);
+void launch_gravity(mhpc_ctx *c, bool potential);
+void launch_h_iterate(mhpc_ctx *c, int passes);
+void launch_neighbours(mhpc_ctx *c);
+void launch_density(mhpc_ctx *c);
+void launch_eos(mhpc_ctx *c);
+void launch_forces(mhpc_ctx *c);
+void launch_scatter_h(mhpc_ctx *c);
+void launch_leapfrog_x(mhpc_ctx *c, double dt);
+void launch_leapfrog_v(mhpc_ctx *c, double dt);
+void launch_neighbours_to_caller(mhpc_ctx *c, int32_t *dst_nnl, int32_t *dst_noi, int max_nb);
+
+// ------------------------------------------------------------------------------------
+// device helpers
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(__gfx950__)
+#error "these kernels are written for gfx950 (CDNA4, wave64)"
+#endif
+
+static inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ int wave_lane() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return (uint64_t)__ballot(p); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int l = wave_lane();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }
+
+// orders LDS traffic of one wavefront against the compiler (the hardware keeps one
+// wave's LDS operations in order)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// inclusive prefix sum over the wave
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int l = wave_lane();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int w = __shfl_up(v, o);
+        if (l >= o) v += w;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void set_err(uint32_t *err, uint32_t f) { atomicOr(err, f); }
+
+// block-wide exclusive scan of one uint32 per thread (blockDim.x = 256); returns the total.
+// Contains two __syncthreads: every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t *excl, uint32_t *lds4) {
+    const int w = threadIdx.x / 64;
+    int inc = wave_incl_scan((int)v);
+    if (wave_lane() == 63) lds4[w] = (uint32_t)inc;
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k < w) off += lds4[k];
+        total += lds4[k];
+    }
+    *excl = off + (uint32_t)inc - v;
+    __syncthreads();
+    return total;
+}
+
+// exclusive scan of row[0..len) into out[0..len) plus `carry`, by one 256-thread block
+__device__ __forceinline__ uint32_t block_scan_row(const uint32_t *row, uint32_t *out, int64_t len,
+                                                   uint32_t carry, uint32_t *lds4) {
+    const int64_t chunk = 256 * 16;
+    for (int64_t c0 = 0; c0 < len; c0 += chunk) {
+        uint32_t v[16];
+        uint32_t s = 0;
+        const int64_t b = c0 + (int64_t)threadIdx.x * 16;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            v[k] = (b + k < len) ? row[b + k] : 0u;
+            s += v[k];
+        }
+        uint32_t excl;
+        uint32_t tot = block_excl_scan256(s, &excl, lds4);
+        uint32_t run = carry + excl;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            if (b + k < len) out[b + k] = run;
+            run += v[k];
+        }
+        carry += tot;
+    }
+    return carry;
+}
+
+// exact FP64 arithmetic helpers: the .hip files are compiled with -ffp-contract=off, so
+// a*b+c is two roundings, as in the reference build (nvcc -fmad=false, Makefile:33)
+
+// Kahan-compensated FP32 accumulator
+struct Kahan {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float x) {
+        float y = x - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
+// plain FP64 accumulator with the same interface
+struct Plain64 {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double x) { s += x; }
+    __device__ __forceinline__ double value() const { return s; }
+};
+
+#define HIP_CHECK_LAUNCH() (void)hipGetLastError()

@@ -1,0 +1,149 @@
+/*
+ * oracle.h -- CPU restatement (FP64, plain C + OpenMP) of milupHPC's single-subdomain
+ * right-hand side, used ONLY as test infrastructure: the checker of the HIP path in
+ * tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py.  Nothing in
+ * the product (miluphpc_amd/) links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the
+ * milupHPC source tree).  Functions that no golden vector, closed form, invariant or
+ * brute-force check pins are marked "parity unpinned" in their comment; at present
+ * none is (the tests under tests/test_oracle_*.py pin each one).
+ *
+ * Node numbering of the oracle's octree: indices [0, n) are particles, [n, n+ncells)
+ * are cells, and n is the root.  (The reference aliases the root with particle 0's
+ * child slots, tree.cu:768-1082; that is a storage artefact, not semantics.)
+ */
+#ifndef MHPC_ORACLE_H
+#define MHPC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORC_OK = 0,
+    ORC_E_CAP_NODES = 1,
+    ORC_E_CAP_NEIGHBOURS = 2,
+    ORC_E_DEPTH = 3,
+    ORC_E_RHO_NONPOSITIVE = 4,
+    ORC_E_STACK = 5,
+    ORC_E_ARG = 6
+};
+
+/* curve types (reference Curve::Type) */
+enum { ORC_CURVE_LEBESGUE = 0, ORC_CURVE_HILBERT = 1 };
+
+/* EOS types: reference EquationOfStates ids (material.cuh); 100 = isothermal gas with a
+ * constant, caller-given sound speed (p = cs^2 rho), a build extension. */
+enum { ORC_EOS_POLYTROPIC = 0, ORC_EOS_ISOTHERMAL = 3, ORC_EOS_IDEAL_GAS = 9,
+       ORC_EOS_LOCALLY_ISOTHERMAL = 12, ORC_EOS_ISOTHERMAL_CONST_CS = 100 };
+
+int orc_num_threads(void);
+void orc_set_num_threads(int n);
+
+/* ---- a1: bounding box (tree.cu:302-437, seeded with 0 by device_rhs.cu:25-32) ---- */
+void orc_bbox(int64_t n, const double *x, const double *y, const double *z, double box[6]);
+
+/* ---- a2: SFC keys (tree.cu:157-240, tree.cu:40-53, tables tree.cuh:52-118) ---- */
+uint64_t orc_lebesgue_key(double x, double y, double z, const double box[6]);
+uint64_t orc_lebesgue2hilbert(uint64_t lebesgue);
+void orc_keys(int64_t n, const double *x, const double *y, const double *z,
+              const double box[6], int curve, uint64_t *keys);
+
+/* ---- a3: stable key sort (helper.cu:150-167 = cub SortPairs) ---- */
+void orc_sort(int64_t n, const uint64_t *keys, int32_t *perm);
+
+/* ---- a5: reference insertion octree (tree.cu:768-1082), serial ---- */
+int orc_tree_build(int64_t n, const double *x, const double *y, const double *z,
+                   const double box[6], int64_t cap_cells,
+                   int32_t *child, int32_t *level, double *nx, double *ny, double *nz,
+                   int64_t *ncells);
+
+/* ---- a6: centres of mass, level by level (tree.cu:1416-1497, miluphpc.cpp:1205-1210) ---- */
+void orc_com(int64_t n, int64_t ncells, const int32_t *child, const int32_t *level,
+             double *nm, double *nx, double *ny, double *nz);
+
+/* traceless quadrupole moments about each cell's COM (build extension for the
+ * mixed-precision far field), q = [xx, xy, xz, yy, yz, zz] per node */
+void orc_quadrupole(int64_t n, int64_t ncells, const int32_t *child, const int32_t *level,
+                    const double *nm, const double *nx, const double *ny, const double *nz,
+                    double *q);
+
+/* ---- a7 / a7': Barnes-Hut ---- */
+/* per-particle MAC, reference computeForces_v1 (gravity.cu:252-396) */
+int orc_bh_v1(int64_t n, int64_t ncells, const int32_t *child, const double *nm,
+              const double *nx, const double *ny, const double *nz, double radius,
+              double theta, double smoothing, double *ax, double *ay, double *az, double *u);
+/* group MAC over `group` consecutive particles of the key order: the semantics the
+ * wavefront-cooperative HIP walk implements. mac: 0 = s^2 < theta^2 (d^2+eps^2) (v1
+ * form), 1 = s^2/theta <= d^2+eps^2 (v2 form, gravity.cu:798-841), d = distance from
+ * the cell's COM to the group's bounding box.  q may be NULL (monopole only). */
+int orc_bh_group(int64_t n, const int32_t *perm, int group, int64_t ncells,
+                 const int32_t *child, const int32_t *level, const double *nm,
+                 const double *nx, const double *ny, const double *nz, const double *q,
+                 const double box[6], double theta, double smoothing, int mac,
+                 double *ax, double *ay, double *az, double *phi,
+                 int64_t *n_cell_int, int64_t *n_part_int);
+/* direct summation on a subset of targets (FP64, softened) */
+void orc_direct(int64_t n, const double *x, const double *y, const double *z,
+                const double *m, double smoothing, int64_t ntarget, const int32_t *targets,
+                double *ax, double *ay, double *az, double *phi);
+
+/* ---- a8: neighbour search (sph.cu:93-281); lists are n x max_nb, row-major ---- */
+int orc_nn_brute(int64_t n, const double *x, const double *y, const double *z,
+                 const double *h, int max_nb, int32_t *nnl, int32_t *noi);
+int orc_nn_grid(int64_t n, const double *x, const double *y, const double *z,
+                const double *h, int max_nb, int32_t *nnl, int32_t *noi);
+int orc_nn_refwalk(int64_t n, const double *x, const double *y, const double *z,
+                   const double *h, const int32_t *child, const double *nx,
+                   const double *ny, const double *nz, double diam, int max_nb,
+                   int32_t *nnl, int32_t *noi);
+
+/* ---- a9: plain h-iteration, Jacobi form of sph.cu:739-961 ---- */
+void orc_h_factor_table(int n_target, int tmax, double *table);
+int orc_h_iterate(int64_t n, const double *x, const double *y, const double *z, double *h,
+                  int n_target, int band_lo, int band_hi, int max_counts, int max_nb,
+                  const double *table, int tmax, int32_t *iters, int32_t *noi);
+
+/* ---- a10-a14: SPH (FP64) ---- */
+void orc_cubic_spline(double r2, double dx, double dy, double dz, double h, double *W,
+                      double dWdx[3], double *dWdr);
+int orc_density(int64_t n, const double *x, const double *y, const double *z,
+                const double *m, const double *h, const int32_t *nnl, const int32_t *noi,
+                int max_nb, int sml_avg, double *rho);
+void orc_soundspeed(int64_t n, int eos, double gamma, double K, const double *rho,
+                    const double *p, double *cs);
+void orc_pressure(int64_t n, int eos, double gamma, double K, const double *rho,
+                  const double *e, const double *cs, double *p);
+void orc_forces(int64_t n, const double *x, const double *y, const double *z,
+                const double *vx, const double *vy, const double *vz, const double *m,
+                const double *h, const double *rho, const double *p, const double *cs,
+                const int32_t *nnl, const int32_t *noi, int max_nb, int sml_avg,
+                double alpha, double beta, double *ax, double *ay, double *az,
+                double *drhodt, double *dedt, double *dsmldt);
+
+/* ---- a17: leapfrog kick-drift (device_leapfrog.cu:4-58) ---- */
+void orc_leapfrog_update_x(int64_t n, double dt, double *x, double *y, double *z,
+                           const double *vx, const double *vy, const double *vz,
+                           const double *gx, const double *gy, const double *gz,
+                           double *gx_old, double *gy_old, double *gz_old);
+void orc_leapfrog_update_v(int64_t n, double dt, double *vx, double *vy, double *vz,
+                           const double *gx, const double *gy, const double *gz,
+                           const double *gx_old, const double *gy_old, const double *gz_old);
+
+/* ---- learned smoothing length: 7 -> H -> H -> 1 MLP, FP64 forward/backward ---- */
+void orc_mlp_forward(int64_t nb, int nin, int nh, const double *X, const double *W1,
+                     const double *b1, const double *W2, const double *b2, const double *W3,
+                     double b3, double *Y, double *A1, double *A2);
+double orc_mlp_backward(int64_t nb, int nin, int nh, const double *X, const double *T,
+                        const double *W1, const double *b1, const double *W2,
+                        const double *b2, const double *W3, double b3,
+                        double *gW1, double *gb1, double *gW2, double *gb2, double *gW3,
+                        double *gb3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,236 @@
+"""GPU parity of bbox, keys, sort, octree, centres of mass and Barnes-Hut against the
+oracle, through the C-ABI.  Bit-exact for keys, order, tree structure, COM (FP64 in the
+same order) and the walk's interaction counts; FP64 forces to 1e-12, FP32+Kahan forces to
+the stated per-particle tolerance."""
+import numpy as np
+import pytest
+
+from miluphpc_amd import ics
+from conftest import print_case, vec_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def make_ctx(lib, s, params, cap=None, cap_cells=0):
+    n = len(s["x"])
+    ctx = lib.Context(cap or n, params, cap_cells=cap_cells)
+    ctx.load(**{k: v for k, v in s.items() if k in lib.FIELD_ID})
+    return ctx
+
+
+def stage_to_tree(ctx):
+    box = ctx.bbox()
+    ctx.keys()
+    ctx.sort()
+    ctx.build_tree()
+    return box
+
+
+@pytest.fixture(scope="module")
+def plummer64k():
+    return ics.plummer(65536, seed=ics.SEED)
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_keys_and_sort_bitexact(gpu_lib, orc, plummer64k, curve):
+    print_case(f"keys+sort plummer 65536 curve={curve}")
+    s = plummer64k
+    p = gpu_lib.default_params(curve=curve)
+    with make_ctx(gpu_lib, s, p) as ctx:
+        box = ctx.bbox()
+        assert np.array_equal(box, orc.bbox(s["x"], s["y"], s["z"]))
+        ctx.keys()
+        ctx.sort()
+        k = ctx.download("key")
+        perm = ctx.download("sorted")
+    ko = orc.keys(s["x"], s["y"], s["z"], box, curve)
+    assert np.array_equal(k, ko)
+    assert np.array_equal(perm, orc.sort(ko))
+
+
+def test_sort_ties_and_ragged_sizes(gpu_lib, orc):
+    """Duplicated keys keep index order; sizes that are not multiples of the tile."""
+    rng = np.random.default_rng(31)
+    for n in (1, 2, 63, 4097, 70001):
+        print_case(f"sort ties n={n}")
+        # coarse lattice of 8^3 points repeated: many equal keys (the sort alone is fine
+        # with duplicates; the tree would reject them)
+        g = rng.integers(0, 8, (n, 3)).astype(np.float64) / 8.0 - 0.4
+        s = {"x": g[:, 0].copy(), "y": g[:, 1].copy(), "z": g[:, 2].copy(), "mass": np.full(n, 1.0 / n)}
+        with make_ctx(gpu_lib, s, gpu_lib.default_params()) as ctx:
+            box = ctx.bbox()
+            ctx.keys()
+            ctx.sort()
+            k = ctx.download("key")
+            perm = ctx.download("sorted")
+        ko = orc.keys(s["x"], s["y"], s["z"], box, 0)
+        assert np.array_equal(k, ko)
+        assert np.array_equal(perm, np.argsort(ko, kind="stable"))
+
+
+def canonical_gpu(tree, n, perm, lkeys):
+    """(level, Lebesgue prefix) -> (cell index) and canonical children of every GPU cell."""
+    lev, start = tree["level"], tree["start"]
+    slk = lkeys[perm]
+    canon = {}
+    for c in range(1, len(lev)):
+        L = int(lev[c])
+        canon[(L, int(slk[start[c]]) >> (3 * (21 - L)))] = c
+    return canon
+
+
+def canonical_oracle(t, lkeys):
+    n = t.n
+    child = t.child.reshape(-1, 8)
+    cells = {}
+    stack = [(n, 0, 0)]
+    while stack:
+        node, L, prefix = stack.pop()
+        if L > 0:
+            cells[(L, prefix)] = node
+        for s_ in range(8):
+            c = child[node, s_]
+            if c >= n:
+                stack.append((c, L + 1, (prefix << 3) | s_))
+    return cells
+
+
+@pytest.mark.parametrize("case", ["plummer", "sedov"])
+def test_tree_and_com_bitexact(gpu_lib, orc, plummer64k, case):
+    print_case(f"tree+com {case}")
+    if case == "plummer":
+        s = plummer64k
+        curve = 1
+    else:
+        s = ics.sedov_lattice(32)
+        curve = 0
+    n = len(s["x"])
+    p = gpu_lib.default_params(curve=curve, quadrupole=1)
+    with make_ctx(gpu_lib, s, p) as ctx:
+        box = stage_to_tree(ctx)
+        ctx.centres_of_mass()
+        tree = ctx.tree()
+        perm = ctx.download("sorted")
+        plevel = ctx.download("level")
+    lk = orc.keys(s["x"], s["y"], s["z"], box, 0)
+    t = orc.Tree(s["x"], s["y"], s["z"], box)
+    assert t.status == 0
+    t.com(s["mass"])
+    q = t.quadrupole().reshape(-1, 6)
+    og = canonical_oracle(t, lk)
+    gg = canonical_gpu(tree, n, perm, lk)
+    assert set(og) == set(gg)
+    assert len(tree["level"]) == t.ncells
+    assert np.array_equal(plevel, t.level[:n])
+    # children, canonically: particle children by original index, cells by (L, prefix)
+    ochild = t.child.reshape(-1, 8)
+    inv = {v: k for k, v in gg.items()}
+    for key, oc in og.items():
+        gc = gg[key]
+        for s_ in range(8):
+            a, b = ochild[oc, s_], tree["child"][gc, s_]
+            if a == -1:
+                assert b == -1
+            elif a < n:
+                assert b < n and perm[b] == a
+            else:
+                assert b >= n and inv[b - n] == (key[0] + 1, (key[1] << 3) | s_)
+        assert tree["mass"][gc] == t.nm[oc]
+        assert tree["x"][gc] == t.nx[oc] and tree["y"][gc] == t.ny[oc] and tree["z"][gc] == t.nz[oc]
+        assert np.array_equal(tree["quad"][:, gc], q[oc])
+    assert tree["mass"][0] == t.nm[n]
+
+
+def run_gravity(gpu_lib, s, params, potential=False):
+    with make_ctx(gpu_lib, s, params) as ctx:
+        st = ctx.rhs(gpu_lib.RHS_GRAVITY | (gpu_lib.RHS_POTENTIAL if potential else 0))
+        out = {k: ctx.download(k) for k in ("g_ax", "g_ay", "g_az", "u", "sorted")}
+        out["box"] = ctx.get_box()
+    out["stats"] = st
+    return out
+
+
+@pytest.mark.parametrize("mac", [0, 1])
+def test_barnes_hut_fp64_parity(gpu_lib, orc, plummer64k, mac):
+    print_case(f"bh fp64 plummer 65536 mac={mac}")
+    s = plummer64k
+    p = gpu_lib.default_params(theta=0.5, smoothing=1e-6, curve=1, mac=mac, precision=0)
+    g = run_gravity(gpu_lib, s, p, potential=True)
+    box = orc.bbox(s["x"], s["y"], s["z"])
+    t = orc.Tree(s["x"], s["y"], s["z"], box)
+    t.com(s["mass"])
+    o = orc.bh_group(t, g["sorted"], box, 0.5, 1e-6, mac=mac, group=64, potential=True)
+    assert g["stats"]["n_cell_interactions"] == o["n_cell"]
+    assert g["stats"]["n_part_interactions"] == o["n_part"]
+    err = vec_rel_err(g["g_ax"], g["g_ay"], g["g_az"], o["ax"], o["ay"], o["az"])
+    assert np.max(err) < 1e-12
+    u_ref = 0.5 * s["mass"] * o["phi"]
+    assert np.max(np.abs(g["u"] - u_ref) / np.abs(u_ref)) < 1e-12
+
+
+def test_barnes_hut_fp32_quadrupole_parity(gpu_lib, orc, plummer64k):
+    """Mixed precision: FP32 pair math + Kahan vs the FP64 oracle of the same walk."""
+    print_case("bh fp32+kahan quadrupole plummer 65536")
+    s = plummer64k
+    p = gpu_lib.default_params(theta=0.6, smoothing=4e-8, curve=1, mac=0, quadrupole=1, precision=1)
+    g = run_gravity(gpu_lib, s, p, potential=True)
+    box = orc.bbox(s["x"], s["y"], s["z"])
+    t = orc.Tree(s["x"], s["y"], s["z"], box)
+    t.com(s["mass"])
+    o = orc.bh_group(t, g["sorted"], box, 0.6, 4e-8, mac=0, group=64, quad=True, potential=True)
+    assert g["stats"]["n_cell_interactions"] == o["n_cell"]
+    assert g["stats"]["n_part_interactions"] == o["n_part"]
+    err = vec_rel_err(g["g_ax"], g["g_ay"], g["g_az"], o["ax"], o["ay"], o["az"])
+    print(f"fp32 quad: rel err p50 {np.median(err):.2e} p99 {np.percentile(err, 99):.2e} max {err.max():.2e}")
+    assert np.percentile(err, 99) < 1e-5
+    # the BH approximation error itself is far larger than the FP32 error
+    sub = np.random.default_rng(5).choice(len(s["x"]), 512, replace=False).astype(np.int32)
+    d = orc.direct(s["x"], s["y"], s["z"], s["mass"], 4e-8, targets=sub)
+    e_bh = vec_rel_err(o["ax"][sub], o["ay"][sub], o["az"][sub], d["ax"], d["ay"], d["az"])
+    assert np.percentile(err, 99) < 0.1 * np.percentile(e_bh, 99)
+    u_ref = 0.5 * s["mass"] * o["phi"]
+    assert np.percentile(np.abs(g["u"] - u_ref) / np.abs(u_ref), 99) < 1e-5
+
+
+def test_tree_errors(gpu_lib):
+    print_case("tree errors: duplicate positions, node capacity")
+    s = ics.plummer(5000, seed=9)
+    s["x"][10], s["y"][10], s["z"][10] = s["x"][20], s["y"][20], s["z"][20]
+    with make_ctx(gpu_lib, s, gpu_lib.default_params()) as ctx:
+        with pytest.raises(gpu_lib.MhpcError) as e:
+            ctx.rhs(gpu_lib.RHS_GRAVITY)
+        assert e.value.status == gpu_lib.E_DEPTH
+    s = ics.plummer(5000, seed=9)
+    with make_ctx(gpu_lib, s, gpu_lib.default_params(), cap_cells=100) as ctx:
+        with pytest.raises(gpu_lib.MhpcError) as e:
+            ctx.rhs(gpu_lib.RHS_GRAVITY)
+        assert e.value.status == gpu_lib.E_CAP_NODES
+
+
+def test_leapfrog_matches_oracle(gpu_lib, orc):
+    print_case("leapfrog 3 steps plummer 8192")
+    s = ics.plummer(8192, seed=10)
+    p = gpu_lib.default_params(theta=0.5, smoothing=1e-6, curve=1, precision=0)
+    dt = 1e-3
+    with make_ctx(gpu_lib, s, p) as ctx:
+        ctx.rhs(gpu_lib.RHS_GRAVITY)
+        for _ in range(3):
+            ctx.leapfrog_update_x(dt)
+            ctx.rhs(gpu_lib.RHS_GRAVITY)
+            ctx.leapfrog_update_v(dt)
+        g = {k: ctx.download(k) for k in ("x", "y", "z", "vx", "vy", "vz", "g_ax", "g_ay", "g_az")}
+    st = {k: s[k].copy() for k in ("x", "y", "z", "vx", "vy", "vz")}
+
+    def acc(st):
+        r = orc.gravity_rhs({**st, "mass": s["mass"]}, 0.5, 1e-6, curve=1)
+        return r["ax"], r["ay"], r["az"]
+
+    st["g_ax"], st["g_ay"], st["g_az"] = acc(st)
+    for k in ("g_ax_old", "g_ay_old", "g_az_old"):
+        st[k] = np.zeros(len(s["x"]))
+    for _ in range(3):
+        orc.leapfrog_update_x(dt, st)
+        st["g_ax"], st["g_ay"], st["g_az"] = acc(st)
+        orc.leapfrog_update_v(dt, st)
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.max(np.abs(g[k] - st[k])) < 1e-12 * max(1.0, np.abs(st[k]).max())
