@@ -147,6 +147,9 @@ int mhpc_synchronize(mhpc_ctx *ctx);
 int mhpc_upload(mhpc_ctx *ctx, int field, const void *host, int64_t n);
 int mhpc_download(mhpc_ctx *ctx, int field, void *host, int64_t n);
 int mhpc_field_ptr(mhpc_ctx *ctx, int field, void **device_ptr);
+/* device-to-device copy of n elements into a field, ordered on the context stream (for
+ * callers whose inputs already live in HBM, e.g. restoring a state between RHS calls) */
+int mhpc_upload_device(mhpc_ctx *ctx, int field, const void *device_src, int64_t n);
 /* neighbour lists in the reference layout nnl[i * max_nb + k] (original indices) */
 int mhpc_download_neighbours(mhpc_ctx *ctx, int32_t *nnl, int32_t *noi, int64_t n, int max_nb);
 /* octree in key-sorted numbering: particles [0,n) (sorted positions), cells n + c */

@@ -97,9 +97,8 @@ static int ensure_nbr(mhpc_ctx *c) {
 
 static void free_all(mhpc_ctx *c) {
     void *ptrs[] = {c->noi, c->level, c->h_iters, c->key, c->lkey, c->skey, c->keyB, c->perm,
-                    c->valB, c->tile_hist, c->tile_off, c->digit_hist, c->slkey, c->sx, c->sy,
-                    c->sz, c->sm, c->sh, c->sh2, c->svx, c->svy, c->svz, c->se, c->srho, c->sp,
-                    c->scs, c->snoi, c->siters, c->sdone, c->lcp, c->tcnt, c->toff,
+                    c->valB, c->tile_hist, c->tile_off, c->digit_hist, c->slkey, c->sxm, c->sv4,
+                    c->sr4, c->sh, c->sh2, c->se, c->srho, c->sp, c->scs, c->snoi, c->siters, c->sdone, c->lcp, c->tcnt, c->toff,
                     c->level_count, c->cell_base, c->cstart, c->cend, c->clevel, c->cchild,
                     c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax,
                     c->nbr, c->htable, c->bbox_partial, c->box, c->err, c->counters};
@@ -142,9 +141,11 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->tile_hist, MHPC_SORT_BUCKETS * c->sort_tiles_cap));
     HIPC(dalloc(&c->tile_off, MHPC_SORT_BUCKETS * c->sort_tiles_cap));
     HIPC(dalloc(&c->digit_hist, MHPC_SORT_PASSES * MHPC_SORT_BUCKETS));
-    double **sorted[] = {&c->sx, &c->sy, &c->sz, &c->sm, &c->sh, &c->sh2, &c->svx,
-                         &c->svy, &c->svz, &c->se, &c->srho, &c->sp, &c->scs};
+    double **sorted[] = {&c->sh, &c->sh2, &c->se, &c->srho, &c->sp, &c->scs};
     for (double **p : sorted) HIPC(dalloc(p, cap));
+    HIPC(dalloc(&c->sxm, cap));
+    HIPC(dalloc(&c->sv4, cap));
+    HIPC(dalloc(&c->sr4, cap));
     HIPC(dalloc(&c->snoi, cap));
     HIPC(dalloc(&c->siters, cap));
     HIPC(hipMemset(c->siters, 0, sizeof(int32_t) * (size_t)cap));
@@ -282,6 +283,16 @@ extern "C" int mhpc_download(mhpc_ctx *c, int field, void *host, int64_t n) {
     if (!d.ptr) return MHPC_E_ARG;
     HIPC(hipMemcpyAsync(host, d.ptr, d.elem * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_upload_device(mhpc_ctx *c, int field, const void *device_src, int64_t n) {
+    if (!c || !device_src || n < 0 || n > c->n) return MHPC_E_ARG;
+    FieldDesc d = field_desc(c, field);
+    if (!d.ptr) return MHPC_E_ARG;
+    HIPC(hipMemcpyAsync(d.ptr, device_src, d.elem * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+    c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
+    c->nn_valid = c->rho_valid = c->eos_valid = false;
     return MHPC_OK;
 }
 

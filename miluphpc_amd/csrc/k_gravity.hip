@@ -144,7 +144,7 @@ __device__ __forceinline__ void eval_cells(const BHWaveLds &L, int cnt, double p
 struct BHArgs {
     int64_t n;
     const int32_t *perm;
-    const double *sx, *sy, *sz, *sm;
+    const double4 *sxm;
     const int32_t *clevel, *cchild;
     const double *cmass, *cx, *cy, *cz;
     const float4 *cquadf;
@@ -166,7 +166,8 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
     const int64_t n = A.n;
     if (group * 64 >= n) return;   // whole wave exits together
     const bool valid = k < n;
-    const double px = valid ? A.sx[k] : 0.0, py = valid ? A.sy[k] : 0.0, pz = valid ? A.sz[k] : 0.0;
+    const double4 own = valid ? A.sxm[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double px = own.x, py = own.y, pz = own.z;
     // group bounding box (exact)
     const double glo_x = wave_min(valid ? px : 1e300), ghi_x = wave_max(valid ? px : -1e300);
     const double glo_y = wave_min(valid ? py : 1e300), ghi_y = wave_max(valid ? py : -1e300);
@@ -222,10 +223,11 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
         const uint64_t mP = wave_ballot(isPart), mA = wave_ballot(accept);
         if (isPart) {
             const int pos = pl_n + popc64(mP & lt);
-            L.px[pos] = A.sx[node];
-            L.py[pos] = A.sy[node];
-            L.pz[pos] = A.sz[node];
-            L.pm[pos] = A.sm[node];
+            const double4 r = A.sxm[node];
+            L.px[pos] = r.x;
+            L.py[pos] = r.y;
+            L.pz[pos] = r.z;
+            L.pm[pos] = r.w;
             L.pid[pos] = node;
         }
         if (accept) {
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
         A.gax[i] = A.G * acc.x.value();
         A.gay[i] = A.G * acc.y.value();
         A.gaz[i] = A.G * acc.z.value();
-        if (POT) A.u[i] = 0.5 * A.sm[k] * A.G * acc.phi.value();
+        if (POT) A.u[i] = 0.5 * own.w * A.G * acc.phi.value();
     }
     if (lane == 0) {
         atomicAdd(&A.counters[CNT_CELL_INT], (unsigned long long)(n_cells * nvalid));
@@ -336,7 +338,7 @@ void launch_gravity(mhpc_ctx *c, bool potential) {
     BHArgs a;
     a.n = c->n;
     a.perm = c->perm;
-    a.sx = c->sx; a.sy = c->sy; a.sz = c->sz; a.sm = c->sm;
+    a.sxm = c->sxm;
     a.clevel = c->clevel;
     a.cchild = c->cchild;
     a.cmass = c->cmass; a.cx = c->cx; a.cy = c->cy; a.cz = c->cz;

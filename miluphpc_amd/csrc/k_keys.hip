@@ -178,13 +178,18 @@ void launch_keys(mhpc_ctx *c) {
 }
 
 // ------------------------------------------------------------------------------------
-// gather the fields a phase reads into key order (coalesced writes, reads follow perm)
+// gather into key order: the packed records {x,y,z,m} (and for SPH {vx,vy,vz,0}) plus
+// the scalar fields a phase reads.  Writes are coalesced; reads follow perm.
 struct GatherArgs {
     const int32_t *perm;
     int64_t n;
+    const double *x, *y, *z, *m;
+    double4 *sxm;
+    const double *vx, *vy, *vz;
+    double4 *sv4;               // null when SPH is off
     int nf;
-    const double *src[12];
-    double *dst[12];
+    const double *src[6];
+    double *dst[6];
     const uint64_t *lkey;
     uint64_t *slkey;
 };
@@ -194,8 +199,11 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     if (k >= a.n) return;
     const int32_t i = a.perm[k];
     if (i < 0 || (int64_t)i >= a.n) return;
-#pragma unroll 4
-    for (int f = 0; f < a.nf; f++) a.dst[f][k] = a.src[f][i];
+    a.sxm[k] = make_double4(a.x[i], a.y[i], a.z[i], a.m[i]);
+    if (a.sv4) a.sv4[k] = make_double4(a.vx[i], a.vy[i], a.vz[i], 0.0);
+#pragma unroll
+    for (int f = 0; f < 6; f++)
+        if (f < a.nf) a.dst[f][k] = a.src[f][i];
     if (a.slkey) a.slkey[k] = a.lkey[i];
 }
 
@@ -204,21 +212,18 @@ void launch_gather(mhpc_ctx *c, bool gravity, bool sph) {
     GatherArgs a = {};
     a.perm = c->perm;
     a.n = c->n;
+    a.x = c->f[MHPC_F_X]; a.y = c->f[MHPC_F_Y]; a.z = c->f[MHPC_F_Z]; a.m = c->f[MHPC_F_MASS];
+    a.sxm = c->sxm;
     int nf = 0;
     auto add = [&](int field, double *dst) {
         a.src[nf] = c->f[field];
         a.dst[nf] = dst;
         nf++;
     };
-    add(MHPC_F_X, c->sx);
-    add(MHPC_F_Y, c->sy);
-    add(MHPC_F_Z, c->sz);
-    add(MHPC_F_MASS, c->sm);
     if (sph) {
+        a.vx = c->f[MHPC_F_VX]; a.vy = c->f[MHPC_F_VY]; a.vz = c->f[MHPC_F_VZ];
+        a.sv4 = c->sv4;
         add(MHPC_F_SML, c->sh);
-        add(MHPC_F_VX, c->svx);
-        add(MHPC_F_VY, c->svy);
-        add(MHPC_F_VZ, c->svz);
         add(MHPC_F_E, c->se);
         add(MHPC_F_P, c->sp);
         add(MHPC_F_CS, c->scs);

@@ -29,7 +29,7 @@ struct NNWaveLds {
 struct NNArgs {
     int64_t n;
     const int32_t *perm;
-    const double *sx, *sy, *sz;
+    const double4 *sxm;
     const double *h;          // h used for the search (h_cur in the h-iteration)
     const int32_t *cstart, *cend, *cchild;
     const double *cbmin, *cbmax;
@@ -72,7 +72,8 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
         if (MODE == 1 && valid) A.h_next[k] = A.h[k];
         return;
     }
-    const double xi = valid ? A.sx[k] : 0.0, yi = valid ? A.sy[k] : 0.0, zi = valid ? A.sz[k] : 0.0;
+    const double4 own = valid ? A.sxm[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double xi = own.x, yi = own.y, zi = own.z;
     const double hi = valid ? A.h[k] : 0.0;
     const double hi2 = hi * hi;
     // search box: bounding box of the active lanes grown by their largest h (plus a
@@ -103,7 +104,8 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             int rs = 0, rc = 0;
             int64_t c = 0;
             if (node >= 0 && node < n) {
-                const double px = A.sx[node], py = A.sy[node], pz = A.sz[node];
+                const double4 r = A.sxm[node];
+                const double px = r.x, py = r.y, pz = r.z;
                 take = px >= lox && px <= hix && py >= loy && py <= hiy && pz >= loz && pz <= hiz;
                 rs = node;
                 rc = 1;
@@ -169,9 +171,10 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
                 }
                 const int j = L.rstart[rlo] + (t - L.rpre[rlo]);
                 const double hj = A.h[j];
-                L.tx[lane] = A.sx[j];
-                L.ty[lane] = A.sy[j];
-                L.tz[lane] = A.sz[j];
+                const double4 r = A.sxm[j];
+                L.tx[lane] = r.x;
+                L.ty[lane] = r.y;
+                L.tz[lane] = r.z;
                 L.th2[lane] = hj * hj;
                 L.tj[lane] = j;
             }
@@ -255,7 +258,7 @@ static NNArgs nn_common(mhpc_ctx *c) {
     NNArgs a = {};
     a.n = c->n;
     a.perm = c->perm;
-    a.sx = c->sx; a.sy = c->sy; a.sz = c->sz;
+    a.sxm = c->sxm;
     a.cstart = c->cstart; a.cend = c->cend; a.cchild = c->cchild;
     a.cbmin = c->cbmin; a.cbmax = c->cbmax;
     a.cap_cells = c->cap_cells;
@@ -319,22 +322,6 @@ void launch_neighbours(mhpc_ctx *c) {
 // ------------------------------------------------------------------------------------
 // a10 cubic spline (kernel.cu:50-83), support radius h.  FP64 form: the reference's
 // expressions in the same order; FP32 form: the same formula in float.
-__device__ __forceinline__ double w_cubic64(double r, double h) {
-    const double q = r / h;
-    const double f = 8. / M_PI * 1. / (h * h * h);
-    if (q > 1) return 0.0;
-    if (q > 0.5) return 2. * f * (1. - q) * (1. - q) * (1 - q);
-    return f * (6. * q * q * q - 6. * q * q + 1.);
-}
-
-__device__ __forceinline__ double dwdr_cubic64(double r, double h) {
-    const double q = r / h;
-    const double f = 8. / M_PI * 1. / (h * h * h);
-    if (q > 1) return 0.0;
-    if (q > 0.5) return -6. * f * 1. / h * (1. - q) * (1. - q);
-    return 6. * f / h * (3 * q * q - 2 * q);
-}
-
 #define MHPC_8_OVER_PI_F 2.546479089470325f
 
 __device__ __forceinline__ void cubic32(float r, float h, float &W, float &dWdr) {
@@ -355,12 +342,13 @@ __device__ __forceinline__ void cubic32(float r, float h, float &W, float &dWdr)
 }
 
 // ------------------------------------------------------------------------------------
-// a11 density (density.cu:8-72): rho_i = m_i W(0, h_i) + sum_j m_j W(r_ij, h_ij)
+// a11 density (density.cu:8-72): rho_i = m_i W(0, h_i) + sum_j m_j W(r_ij, h_ij).
+// Neighbour records are read as one 32-byte {x,y,z,m} plus h; the loop is unrolled so a
+// wave keeps several independent gathers in flight.
+#define K8PI 2.5464790894703255   // 8 / pi
+
 template <bool FP64>
-__global__ __launch_bounds__(256) void k_density(int64_t n, const double *__restrict__ sx,
-                                                 const double *__restrict__ sy,
-                                                 const double *__restrict__ sz,
-                                                 const double *__restrict__ sm,
+__global__ __launch_bounds__(256) void k_density(int64_t n, const double4 *__restrict__ sxm,
                                                  const double *__restrict__ sh,
                                                  const int32_t *__restrict__ snoi,
                                                  const int32_t *__restrict__ nbr, int64_t stride,
@@ -369,31 +357,45 @@ __global__ __launch_bounds__(256) void k_density(int64_t n, const double *__rest
                                                  uint32_t *err) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const double xi = sx[k], yi = sy[k], zi = sz[k], hi = sh[k];
+    const double4 pi = sxm[k];
+    const double hi = sh[k];
     int cnt = snoi[k];
     cnt = cnt > max_nb ? max_nb : cnt;
     double r;
     if constexpr (FP64) {
-        r = sm[k] * w_cubic64(0.0, hi);
+        double acc = 0.0;
+#pragma unroll 4
         for (int s = 0; s < cnt; s++) {
             const int32_t j = nbr[(int64_t)s * stride + k];
-            const double sml = sml_avg ? 0.5 * (hi + sh[j]) : hi;
-            const double dx = xi - sx[j], dy = yi - sy[j], dz = zi - sz[j];
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            r += sm[j] * w_cubic64(sqrt(r2), sml);
+            const double4 pj = sxm[j];
+            const double hj = sh[j];
+            const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            const double ih = 1.0 / hb;
+            const double q = rr * ih;
+            const double f = K8PI * ih * ih * ih;
+            double W = 0.0;
+            if (q <= 0.5) W = f * (6.0 * q * q * (q - 1.0) + 1.0);
+            else if (q <= 1.0) { const double t = 1.0 - q; W = 2.0 * f * t * t * t; }
+            acc += pj.w * W;
         }
+        r = pi.w * (K8PI / (hi * hi * hi)) + acc;
     } else {
         Kahan acc;
+#pragma unroll 4
         for (int s = 0; s < cnt; s++) {
             const int32_t j = nbr[(int64_t)s * stride + k];
-            const float sml = (float)(sml_avg ? 0.5 * (hi + sh[j]) : hi);
-            const float dx = (float)(xi - sx[j]), dy = (float)(yi - sy[j]), dz = (float)(zi - sz[j]);
+            const double4 pj = sxm[j];
+            const double hj = sh[j];
+            const float hb = (float)(sml_avg ? 0.5 * (hi + hj) : hi);
+            const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y), dz = (float)(pi.z - pj.z);
             const float rr = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
             float W, dW;
-            cubic32(rr, sml, W, dW);
-            acc.add((float)sm[j] * W);
+            cubic32(rr, hb, W, dW);
+            acc.add((float)pj.w * W);
         }
-        r = sm[k] * w_cubic64(0.0, hi) + acc.value();
+        r = pi.w * (K8PI / (hi * hi * hi)) + acc.value();
     }
     srho[k] = r;
     rho[perm[k]] = r;
@@ -404,17 +406,19 @@ void launch_density(mhpc_ctx *c) {
     if (c->n <= 0) return;
     const unsigned g = (unsigned)ceil_div(c->n, 256);
     if (c->p.precision == MHPC_PREC_FP64)
-        hipLaunchKernelGGL(k_density<true>, dim3(g), dim3(256), 0, c->stream, c->n, c->sx, c->sy, c->sz,
-                           c->sm, c->sh, c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg,
-                           c->perm, c->srho, c->f[MHPC_F_RHO], c->err);
+        hipLaunchKernelGGL(k_density<true>, dim3(g), dim3(256), 0, c->stream, c->n, c->sxm, c->sh, c->snoi,
+                           c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
+                           c->f[MHPC_F_RHO], c->err);
     else
-        hipLaunchKernelGGL(k_density<false>, dim3(g), dim3(256), 0, c->stream, c->n, c->sx, c->sy, c->sz,
-                           c->sm, c->sh, c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg,
-                           c->perm, c->srho, c->f[MHPC_F_RHO], c->err);
+        hipLaunchKernelGGL(k_density<false>, dim3(g), dim3(256), 0, c->stream, c->n, c->sxm, c->sh, c->snoi,
+                           c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
+                           c->f[MHPC_F_RHO], c->err);
 }
 
 // ------------------------------------------------------------------------------------
-// a12 / a13 sound speed (soundspeed.cu:38-97) and pressure (pressure.cu:6-87), fused.
+// a12 / a13 sound speed (soundspeed.cu:38-97) and pressure (pressure.cu:6-87), fused;
+// also writes the records the force loop gathers: cs into {vx,vy,vz,cs} and
+// {rho, p/rho^2, m/rho, p}.
 __device__ __forceinline__ double eos_cs(int eos, double gamma, double K, double rho, double p,
                                          double cs) {
     switch (eos) {
@@ -440,7 +444,9 @@ __device__ __forceinline__ double eos_p(int eos, double gamma, double K, double 
 __global__ __launch_bounds__(256) void k_eos(int64_t n, int eos, double gamma, double K,
                                              int cs_before_p, const double *__restrict__ srho,
                                              const double *__restrict__ se, double *__restrict__ sp,
-                                             double *__restrict__ scs, const int32_t *__restrict__ perm,
+                                             double *__restrict__ scs, const double4 *__restrict__ sxm,
+                                             double4 *__restrict__ sv4, double4 *__restrict__ sr4,
+                                             const int32_t *__restrict__ perm,
                                              double *__restrict__ p_out, double *__restrict__ cs_out) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -455,6 +461,8 @@ __global__ __launch_bounds__(256) void k_eos(int64_t n, int eos, double gamma, d
     }
     sp[k] = p;
     scs[k] = cs;
+    sv4[k].w = cs;
+    sr4[k] = make_double4(rho, p / (rho * rho), sxm[k].w / rho, p);
     const int32_t i = perm[k];
     p_out[i] = p;
     cs_out[i] = cs;
@@ -464,15 +472,20 @@ void launch_eos(mhpc_ctx *c) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_eos, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->n,
                        c->p.eos, c->p.gamma, c->p.K, c->p.cs_before_p, c->srho, c->se, c->sp, c->scs,
-                       c->perm, c->f[MHPC_F_P], c->f[MHPC_F_CS]);
+                       c->sxm, c->sv4, c->sr4, c->perm, c->f[MHPC_F_P], c->f[MHPC_F_CS]);
 }
 
 // ------------------------------------------------------------------------------------
 // a14 internal forces (internal_forces.cu:4-668): SPH_EQU_VERSION 1 pressure term,
 // Monaghan viscosity, drho/dt, de/dt (INTEGRATE_ENERGY), dh/dt (INTEGRATE_SML).
+// Per pair: grad W = dW/dr (r_ij / r); pressure factor pf = -m_j (p_i/rho_i^2 +
+// p_j/rho_j^2); a_i += (pf - m_j Pi_ij) grad W; drho_i += rho_i (m_j/rho_j) v_ij.gradW;
+// dh_i -= h_i/3 (m_j/rho_j) v_ij.gradW; de_i += 1/2 (m_j Pi_ij - pf) v_ij.gradW (the
+// reference's 1/2 m_j Pi v.gradW + 1/2 a^p_ij . (-v_ij), with a^p_ij = pf gradW).
 struct ForceArgs {
     int64_t n;
-    const double *sx, *sy, *sz, *svx, *svy, *svz, *sm, *sh, *srho, *sp, *scs;
+    const double4 *sxm, *sv4, *sr4;
+    const double *sh;
     const int32_t *snoi, *nbr;
     int64_t stride;
     int max_nb, sml_avg;
@@ -485,98 +498,98 @@ template <bool FP64>
 __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= A.n) return;
-    const double xi = A.sx[k], yi = A.sy[k], zi = A.sz[k];
-    const double vxi = A.svx[k], vyi = A.svy[k], vzi = A.svz[k];
-    const double hi = A.sh[k], rhoi = A.srho[k], pi = A.sp[k], csi = A.scs[k];
+    const double4 pi = A.sxm[k], vi = A.sv4[k], ri = A.sr4[k];
+    const double hi = A.sh[k];
     int cnt = A.snoi[k];
     cnt = cnt > A.max_nb ? A.max_nb : cnt;
     double out[6];
     if constexpr (FP64) {
-        double acc0 = 0, acc1 = 0, acc2 = 0, drho = 0, de = 0, dsml = 0;
-        const double pri = pi / (rhoi * rhoi);
+        double a0 = 0, a1 = 0, a2 = 0, drho = 0, de = 0, dsml = 0;
+#pragma unroll 2
         for (int s = 0; s < cnt; s++) {
             const int32_t j = A.nbr[(int64_t)s * A.stride + k];
-            const double hj = A.sh[j], rhoj = A.srho[j], mj = A.sm[j];
-            const double sml = A.sml_avg ? 0.5 * (hi + hj) : hi;
-            const double dr0 = xi - A.sx[j], dr1 = yi - A.sy[j], dr2 = zi - A.sz[j];
-            double r2 = 0.0;
-            r2 += dr0 * dr0;
-            r2 += dr1 * dr1;
-            r2 += dr2 * dr2;
+            const double4 pj = A.sxm[j], vj = A.sv4[j], rj = A.sr4[j];
+            const double hj = A.sh[j];
+            const double hb = 0.5 * (hi + hj);
+            const double sml = A.sml_avg ? hb : hi;
+            const double d0 = pi.x - pj.x, d1 = pi.y - pj.y, d2 = pi.z - pj.z;
+            const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
             const double r = sqrt(r2);
-            const double dWdr = dwdr_cubic64(r, sml);
-            const double g0 = dWdr / r * dr0, g1 = dWdr / r * dr1, g2 = dWdr / r * dr2;
-            const double dv0 = vxi - A.svx[j], dv1 = vyi - A.svy[j], dv2 = vzi - A.svz[j];
-            double vvnablaW = dv0 * g0;
-            vvnablaW += dv1 * g1;
-            vvnablaW += dv2 * g2;
-            double rr = 0.0, vr = 0.0;
-            rr += dr0 * dr0; vr += dv0 * dr0;
-            rr += dr1 * dr1; vr += dv1 * dr1;
-            rr += dr2 * dr2; vr += dv2 * dr2;
-            double pij = 0.0;
+            const double ih = 1.0 / sml;
+            const double q = r * ih;
+            const double fih = K8PI * ih * ih * ih * ih;
+            double dWdr = 0.0;
+            if (q <= 0.5) dWdr = 6.0 * fih * q * (3.0 * q - 2.0);
+            else if (q <= 1.0) { const double t = 1.0 - q; dWdr = -6.0 * fih * t * t; }
+            const double gs = dWdr / r;
+            const double g0 = gs * d0, g1 = gs * d1, g2 = gs * d2;
+            const double v0 = vi.x - vj.x, v1 = vi.y - vj.y, v2 = vi.z - vj.z;
+            const double vv = v0 * g0 + v1 * g1 + v2 * g2;
+            const double vr = v0 * d0 + v1 * d1 + v2 * d2;
+            double mpij = 0.0;
             if (vr < 0) {
-                const double csbar = 0.5 * (csi + A.scs[j]);
-                const double smooth = 0.5 * (hi + hj);
-                const double mu = smooth * vr / (rr + smooth * smooth * 1e-2);
-                const double rhobar = 0.5 * (rhoi + rhoj);
-                pij = (A.beta * mu - A.alpha * csbar) * mu / rhobar;
+                const double csbar = 0.5 * (vi.w + vj.w);
+                const double mu = hb * vr / (r2 + hb * hb * 1e-2);
+                const double rhobar = 0.5 * (ri.x + rj.x);
+                mpij = pj.w * (A.beta * mu - A.alpha * csbar) * mu / rhobar;
             }
-            const double pf = -mj * (pri + A.sp[j] / (rhoj * rhoj));
-            const double as0 = pf * g0, as1 = pf * g1, as2 = pf * g2;
-            acc0 += as0; acc1 += as1; acc2 += as2;
-            acc0 += mj * (-pij) * g0;
-            acc1 += mj * (-pij) * g1;
-            acc2 += mj * (-pij) * g2;
-            drho += rhoi / rhoj * mj * vvnablaW;
-            dsml -= 1. / 3 * hi * mj / rhoj * vvnablaW;
-            de += 0.5 * mj * pij * vvnablaW;
-            de += 0.5 * as0 * -dv0;
-            de += 0.5 * as1 * -dv1;
-            de += 0.5 * as2 * -dv2;
+            const double pf = -pj.w * (ri.y + rj.y);
+            const double fa = pf - mpij;
+            a0 += fa * g0;
+            a1 += fa * g1;
+            a2 += fa * g2;
+            const double mr_vv = rj.z * vv;
+            drho += mr_vv;
+            dsml += mr_vv;
+            de += (mpij - pf) * vv;
         }
-        out[0] = acc0; out[1] = acc1; out[2] = acc2; out[3] = drho; out[4] = de; out[5] = dsml;
+        out[0] = a0; out[1] = a1; out[2] = a2;
+        out[3] = ri.x * drho;
+        out[4] = 0.5 * de;
+        out[5] = -(1.0 / 3.0) * hi * dsml;
     } else {
         Kahan a0, a1, a2, drho, de, dsml;
-        const float hif = (float)hi, rhoif = (float)rhoi, csif = (float)csi;
-        const float pri = (float)(pi / (rhoi * rhoi));
+        const float hif = (float)hi, rhoif = (float)ri.x, csif = (float)vi.w, prif = (float)ri.y;
         const float alpha = (float)A.alpha, beta = (float)A.beta;
+#pragma unroll 2
         for (int s = 0; s < cnt; s++) {
             const int32_t j = A.nbr[(int64_t)s * A.stride + k];
-            const float hj = (float)A.sh[j], rhoj = (float)A.srho[j], mj = (float)A.sm[j];
-            const float sml = A.sml_avg ? (float)(0.5 * (hi + A.sh[j])) : hif;
-            const float dr0 = (float)(xi - A.sx[j]), dr1 = (float)(yi - A.sy[j]), dr2 = (float)(zi - A.sz[j]);
-            const float dv0 = (float)(vxi - A.svx[j]), dv1 = (float)(vyi - A.svy[j]),
-                        dv2 = (float)(vzi - A.svz[j]);
-            const float rr = __builtin_fmaf(dr0, dr0, __builtin_fmaf(dr1, dr1, dr2 * dr2));
-            const float r = sqrtf(rr);
+            const double4 pj = A.sxm[j], vj = A.sv4[j], rj = A.sr4[j];
+            const float hj = (float)A.sh[j];
+            const float hb = 0.5f * (hif + hj);
+            const float sml = A.sml_avg ? hb : hif;
+            const float d0 = (float)(pi.x - pj.x), d1 = (float)(pi.y - pj.y), d2 = (float)(pi.z - pj.z);
+            const float v0 = (float)(vi.x - vj.x), v1 = (float)(vi.y - vj.y), v2 = (float)(vi.z - vj.z);
+            const float r2 = __builtin_fmaf(d0, d0, __builtin_fmaf(d1, d1, d2 * d2));
+            const float r = sqrtf(r2);
             float W, dWdr;
             cubic32(r, sml, W, dWdr);
             const float gs = dWdr / r;
-            const float g0 = gs * dr0, g1 = gs * dr1, g2 = gs * dr2;
-            const float vvnablaW = __builtin_fmaf(dv0, g0, __builtin_fmaf(dv1, g1, dv2 * g2));
-            const float vr = __builtin_fmaf(dv0, dr0, __builtin_fmaf(dv1, dr1, dv2 * dr2));
-            float pij = 0.f;
+            const float g0 = gs * d0, g1 = gs * d1, g2 = gs * d2;
+            const float vv = __builtin_fmaf(v0, g0, __builtin_fmaf(v1, g1, v2 * g2));
+            const float vr = __builtin_fmaf(v0, d0, __builtin_fmaf(v1, d1, v2 * d2));
+            const float mj = (float)pj.w;
+            float mpij = 0.f;
             if (vr < 0.f) {
-                const float csbar = 0.5f * (csif + (float)A.scs[j]);
-                const float smooth = 0.5f * (hif + hj);
-                const float mu = smooth * vr / __builtin_fmaf(smooth * smooth, 1e-2f, rr);
-                const float rhobar = 0.5f * (rhoif + rhoj);
-                pij = (beta * mu - alpha * csbar) * mu / rhobar;
+                const float csbar = 0.5f * (csif + (float)vj.w);
+                const float mu = hb * vr / __builtin_fmaf(hb * hb, 1e-2f, r2);
+                const float rhobar = 0.5f * (rhoif + (float)rj.x);
+                mpij = mj * (beta * mu - alpha * csbar) * mu / rhobar;
             }
-            const float pf = -mj * (pri + (float)(A.sp[j] / (A.srho[j] * A.srho[j])));
-            const float as0 = pf * g0, as1 = pf * g1, as2 = pf * g2;
-            const float mp = -mj * pij;
-            a0.add(as0 + mp * g0);
-            a1.add(as1 + mp * g1);
-            a2.add(as2 + mp * g2);
-            drho.add(rhoif / rhoj * mj * vvnablaW);
-            dsml.add(-(1.f / 3.f) * hif * mj / rhoj * vvnablaW);
-            de.add(0.5f * mj * pij * vvnablaW -
-                   0.5f * __builtin_fmaf(as0, dv0, __builtin_fmaf(as1, dv1, as2 * dv2)));
+            const float pf = -mj * (prif + (float)rj.y);
+            const float fa = pf - mpij;
+            a0.add(fa * g0);
+            a1.add(fa * g1);
+            a2.add(fa * g2);
+            const float mr_vv = (float)rj.z * vv;
+            drho.add(mr_vv);
+            dsml.add(mr_vv);
+            de.add((mpij - pf) * vv);
         }
         out[0] = a0.value(); out[1] = a1.value(); out[2] = a2.value();
-        out[3] = drho.value(); out[4] = de.value(); out[5] = dsml.value();
+        out[3] = ri.x * drho.value();
+        out[4] = 0.5 * de.value();
+        out[5] = -(1.0 / 3.0) * hi * dsml.value();
     }
     const int32_t i = A.perm[k];
     A.ax[i] = out[0];
@@ -591,9 +604,7 @@ void launch_forces(mhpc_ctx *c) {
     if (c->n <= 0) return;
     ForceArgs a;
     a.n = c->n;
-    a.sx = c->sx; a.sy = c->sy; a.sz = c->sz;
-    a.svx = c->svx; a.svy = c->svy; a.svz = c->svz;
-    a.sm = c->sm; a.sh = c->sh; a.srho = c->srho; a.sp = c->sp; a.scs = c->scs;
+    a.sxm = c->sxm; a.sv4 = c->sv4; a.sr4 = c->sr4; a.sh = c->sh;
     a.snoi = c->snoi; a.nbr = c->nbr; a.stride = c->cap_pad;
     a.max_nb = c->p.max_nb; a.sml_avg = c->p.sml_avg;
     a.alpha = c->p.alpha; a.beta = c->p.beta;

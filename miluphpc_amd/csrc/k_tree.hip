@@ -217,8 +217,7 @@ void launch_tree(mhpc_ctx *c) {
 // Each launch reads its level's range from cell_base on the device.
 __global__ __launch_bounds__(256) void k_bottom_up(
     int L, int64_t n, const int32_t *__restrict__ cell_base, const int32_t *__restrict__ cchild,
-    const double *__restrict__ sx, const double *__restrict__ sy, const double *__restrict__ sz,
-    const double *__restrict__ sm, double *__restrict__ cmass, double *__restrict__ cx,
+    const double4 *__restrict__ sxm, double *__restrict__ cmass, double *__restrict__ cx,
     double *__restrict__ cy, double *__restrict__ cz, double *__restrict__ cquad,
     float4 *__restrict__ cquadf, double *__restrict__ cbmin, double *__restrict__ cbmax,
     int64_t cap_cells, int gravity, int quad) {
@@ -239,7 +238,8 @@ __global__ __launch_bounds__(256) void k_bottom_up(
             if (k == -1) continue;
             double px, py, pz, pm, lo[3], hi[3];
             if (k < n) {
-                px = sx[k]; py = sy[k]; pz = sz[k]; pm = sm[k];
+                const double4 r = sxm[k];
+                px = r.x; py = r.y; pz = r.z; pm = r.w;
                 lo[0] = hi[0] = px; lo[1] = hi[1] = py; lo[2] = hi[2] = pz;
             } else {
                 const int64_t cc = k - n;
@@ -276,7 +276,8 @@ __global__ __launch_bounds__(256) void k_bottom_up(
             double px, py, pz, pm;
             double q[6] = {0, 0, 0, 0, 0, 0};
             if (k < n) {
-                px = sx[k]; py = sy[k]; pz = sz[k]; pm = sm[k];
+                const double4 r = sxm[k];
+                px = r.x; py = r.y; pz = r.z; pm = r.w;
             } else {
                 const int64_t cc = k - n;
                 px = cx[cc]; py = cy[cc]; pz = cz[cc]; pm = cmass[cc];
@@ -306,7 +307,7 @@ void launch_bottom_up(mhpc_ctx *c, bool gravity) {
     if (grid > 2048) grid = 2048;
     for (int L = MHPC_LEVELS; L >= 0; L--) {
         hipLaunchKernelGGL(k_bottom_up, dim3(L <= 3 ? 1 : grid), dim3(256), 0, c->stream, L, c->n,
-                           c->cell_base, c->cchild, c->sx, c->sy, c->sz, c->sm, c->cmass, c->cx, c->cy,
+                           c->cell_base, c->cchild, c->sxm, c->cmass, c->cx, c->cy,
                            c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax, c->cap_cells,
                            gravity ? 1 : 0, (gravity && c->p.quadrupole) ? 1 : 0);
     }

@@ -71,11 +71,12 @@ struct mhpc_ctx {
     int64_t sort_tiles_cap = 0;
     uint64_t *slkey = nullptr;                   // Lebesgue keys in sorted order
 
-    // sorted working copies
-    double *sx = nullptr, *sy = nullptr, *sz = nullptr, *sm = nullptr;
+    // sorted working copies; 32-byte records so a neighbour costs two 16-byte loads
+    double4 *sxm = nullptr;                      // {x, y, z, m}
+    double4 *sv4 = nullptr;                      // {vx, vy, vz, cs}
+    double4 *sr4 = nullptr;                      // {rho, p / rho^2, m / rho, p}
     double *sh = nullptr, *sh2 = nullptr;
-    double *svx = nullptr, *svy = nullptr, *svz = nullptr, *se = nullptr;
-    double *srho = nullptr, *sp = nullptr, *scs = nullptr;
+    double *se = nullptr, *srho = nullptr, *sp = nullptr, *scs = nullptr;
     int32_t *snoi = nullptr, *siters = nullptr;
     uint8_t *sdone = nullptr;
 

@@ -129,6 +129,7 @@ def _load() -> C.CDLL:
         "mhpc_upload": (i32, [vp, i32, vp, i64]),
         "mhpc_download": (i32, [vp, i32, vp, i64]),
         "mhpc_field_ptr": (i32, [vp, i32, C.POINTER(vp)]),
+        "mhpc_upload_device": (i32, [vp, i32, vp, i64]),
         "mhpc_download_neighbours": (i32, [vp, vp, vp, i64, i32]),
         "mhpc_tree_size": (i32, [vp, C.POINTER(i64)]),
         "mhpc_download_tree": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -245,6 +246,12 @@ class Context:
         self.set_num_particles(n)
         for k, v in fields.items():
             self.upload(k, v)
+
+    def upload_device(self, field: str, device_ptr: int, n: int | None = None) -> None:
+        """Device-to-device copy into a field (e.g. from a torch tensor's data_ptr())."""
+        n = self.n if n is None else int(n)
+        self._check(LIB.mhpc_upload_device(self._h, FIELD_ID[field], C.c_void_p(int(device_ptr)), n),
+                    f"upload_device {field}")
 
     def field_ptr(self, field: str) -> int:
         p = C.c_void_p()
