@@ -98,7 +98,10 @@ def main() -> int:
     ap.add_argument("--nside", type=int, default=128)
     ap.add_argument("--plummer-n", type=int, default=16777216)
     ap.add_argument("--no-plummer", action="store_true")
+    ap.add_argument("--no-sedov", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (profiler byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
@@ -111,6 +114,35 @@ def main() -> int:
     torch.cuda.set_device(0)
     from miluphpc_amd import ics, lib
 
+    if args.copy_calibration_mb > 0:
+        nbytes = args.copy_calibration_mb << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written")
+        del src, dst
+
+    result = {
+        "metric": "RHS particles/s: Sedov 2M SPH, Plummer 16M BH; learned-h vs plain, same GPU",
+        "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+    }
+    if args.no_sedov:
+        s = None
+    else:
+        s = run_sedov(args, lib, ics, torch, result)
+    if not args.no_plummer:
+        run_plummer(args, lib, ics, result)
+    if not args.no_cpu and s is not None:
+        log("cpu baseline (oracle) ...")
+        result["cpu_baseline"] = cpu_baseline_sedov(s)
+        log(f"cpu baseline {result['cpu_baseline']['value']:.4e} particles/s")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def run_sedov(args, lib, ics, torch, result):
     # ---- Sedov 128^3, SPH, plain h-iteration, FP64 ------------------------------------
     s = ics.sedov_lattice(args.nside)
     n = len(s["x"])
@@ -130,19 +162,10 @@ def main() -> int:
     units = n if dom != "ms_hiter" else max(1, med["n_h_counts"])
     traffic_bytes = bpp[dom] * units
     achieved = traffic_bytes / (med[dom] * 1e-3) / 1e9
-    result = {
-        "metric": "RHS particles/s: Sedov 2M SPH, Plummer 16M BH; learned-h vs plain, same GPU",
+    result.update({
         "value": n / wall,
-        "unit": "particles/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": wall * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic",
         "config": {"workload": f"Sedov-Taylor {args.nside}^3 lattice SPH RHS, plain h-iteration "
                                f"(target 64, band 50-80), cubic spline, ideal gas, FP64",
                    "n_particles": n, "path": "plain"},
@@ -152,11 +175,14 @@ def main() -> int:
         "phases_ms": {k[3:]: round(v, 4) for k, v in med.items() if k.startswith("ms_")},
         "mean_neighbours": nnb,
         "mean_h_counts": med["n_h_counts"] / n,
-    }
+    })
     log(f"sedov: {n / wall:.4e} particles/s, dominant {dom} {med[dom]:.3f} ms, {achieved:.1f} GB/s")
+    return s
 
+
+def run_plummer(args, lib, ics, result):
     # ---- Plummer 16M, Barnes-Hut, theta 0.6, quadrupole, FP32 + Kahan ----------------
-    if not args.no_plummer:
+    if True:
         pl = ics.plummer(args.plummer_n)
         log(f"plummer {args.plummer_n} generated")
         pp = ics.params_plummer_large()
@@ -178,13 +204,6 @@ def main() -> int:
         }
         log(f"plummer: {args.plummer_n / pw:.4e} particles/s, gravity {pm['ms_gravity']:.2f} ms, {tflops:.2f} TFLOP/s")
         del pl
-
-    if not args.no_cpu:
-        log("cpu baseline (oracle) ...")
-        result["cpu_baseline"] = cpu_baseline_sedov(s)
-        log(f"cpu baseline {result['cpu_baseline']['value']:.4e} particles/s")
-    print(json.dumps(result), flush=True)
-    return 0
 
 
 if __name__ == "__main__":

@@ -90,7 +90,7 @@ static int ensure_nbr(mhpc_ctx *c) {
     if (c->nbr && c->nbr_rows >= c->p.max_nb) return MHPC_OK;
     if (c->nbr) (void)hipFree(c->nbr);
     c->nbr = nullptr;
-    HIPC(dalloc(&c->nbr, (int64_t)c->p.max_nb * c->cap_pad));
+    HIPC(dalloc(&c->nbr, ceil_div(c->p.max_nb, 4) * 4 * c->cap_pad));
     c->nbr_rows = c->p.max_nb;
     return MHPC_OK;
 }

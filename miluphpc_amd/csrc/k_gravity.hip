@@ -161,7 +161,7 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
     __shared__ BHWaveLds lds_all[BH_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
     BHWaveLds &L = lds_all[w];
-    const int64_t group = (int64_t)blockIdx.x * BH_WAVES + w;
+    const int64_t group = xcd_block(blockIdx.x, gridDim.x) * BH_WAVES + w;
     const int64_t k = group * 64 + lane;
     const int64_t n = A.n;
     if (group * 64 >= n) return;   // whole wave exits together

@@ -34,8 +34,8 @@ struct NNArgs {
     const int32_t *cstart, *cend, *cchild;
     const double *cbmin, *cbmax;
     int64_t cap_cells;
-    // list mode
-    int32_t *nbr;
+    // list mode: entry s of particle k is nbr4[(s / 4) * stride + k].{x,y,z,w}[s % 4]
+    int4 *nbr4;
     int64_t stride;
     int max_nb;
     int32_t *snoi;
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
     NNWaveLds &L = lds_all[w];
-    const int64_t group = (int64_t)blockIdx.x * NN_WAVES + w;
+    const int64_t group = xcd_block(blockIdx.x, gridDim.x) * NN_WAVES + w;
     const int64_t k = group * 64 + lane;
     const int64_t n = A.n;
     if (group * 64 >= n) return;
@@ -85,6 +85,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     const double loz = wave_min(active ? zi : 1e300) - grow, hiz = wave_max(active ? zi : -1e300) + grow;
 
     int count = 0;
+    int b0 = 0, b1 = 0, b2 = 0;   // pending list entries, flushed four at a time as one int4
     int top = 1, nr = 0;
     if (lane == 0) L.stack[0] = (int)n;
     wave_sync();
@@ -188,7 +189,19 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
                     const double d = dx * dx + dy * dy + dz * dz;
                     const int j = L.tj[e];
                     if (d < hi2 && d < L.th2[e] && j != (int)k) {
-                        if (MODE == 0 && count < A.max_nb) A.nbr[(int64_t)count * A.stride + k] = j;
+                        if (MODE == 0) {
+                            const int slot = count & 3;
+                            if (slot == 3) {
+                                if (count < A.max_nb)
+                                    A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, j);
+                            } else if (slot == 0) {
+                                b0 = j;
+                            } else if (slot == 1) {
+                                b1 = j;
+                            } else {
+                                b2 = j;
+                            }
+                        }
                         count++;
                     }
                 }
@@ -209,6 +222,8 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
         return;
     }
     if (MODE == 0) {
+        if (valid && (count & 3) != 0 && count < A.max_nb)
+            A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, 0);
         if (valid) {
             A.snoi[k] = count;
             const int32_t i = A.perm[k];
@@ -307,7 +322,7 @@ void launch_neighbours(mhpc_ctx *c) {
     if (c->n <= 0) return;
     NNArgs a = nn_common(c);
     a.h = c->sh;
-    a.nbr = c->nbr;
+    a.nbr4 = reinterpret_cast<int4 *>(c->nbr);
     a.stride = c->cap_pad;
     a.snoi = c->snoi;
     a.noi_out = c->noi;
@@ -355,37 +370,44 @@ __global__ __launch_bounds__(256) void k_density(int64_t n, const double4 *__res
                                                  int max_nb, int sml_avg, const int32_t *__restrict__ perm,
                                                  double *__restrict__ srho, double *__restrict__ rho,
                                                  uint32_t *err) {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double4 pi = sxm[k];
     const double hi = sh[k];
     int cnt = snoi[k];
     cnt = cnt > max_nb ? max_nb : cnt;
     double r;
+    const int4 *nbr4 = reinterpret_cast<const int4 *>(nbr);
     if constexpr (FP64) {
         double acc = 0.0;
-#pragma unroll 4
-        for (int s = 0; s < cnt; s++) {
-            const int32_t j = nbr[(int64_t)s * stride + k];
-            const double4 pj = sxm[j];
-            const double hj = sh[j];
-            const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
-            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const double rr = sqrt(dx * dx + dy * dy + dz * dz);
-            const double ih = 1.0 / hb;
-            const double q = rr * ih;
-            const double f = K8PI * ih * ih * ih;
-            double W = 0.0;
-            if (q <= 0.5) W = f * (6.0 * q * q * (q - 1.0) + 1.0);
-            else if (q <= 1.0) { const double t = 1.0 - q; W = 2.0 * f * t * t * t; }
-            acc += pj.w * W;
+        for (int c4 = 0; c4 < cnt; c4 += 4) {
+            const int4 q4 = nbr4[(int64_t)(c4 >> 2) * stride + k];
+            const int jj[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (c4 + u >= cnt) break;
+                const int32_t j = jj[u];
+                const double4 pj = sxm[j];
+                const double hj = sh[j];
+                const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
+                const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                const double rr = sqrt(dx * dx + dy * dy + dz * dz);
+                const double ih = 1.0 / hb;
+                const double q = rr * ih;
+                const double f = K8PI * ih * ih * ih;
+                double W = 0.0;
+                if (q <= 0.5) W = f * (6.0 * q * q * (q - 1.0) + 1.0);
+                else if (q <= 1.0) { const double t = 1.0 - q; W = 2.0 * f * t * t * t; }
+                acc += pj.w * W;
+            }
         }
         r = pi.w * (K8PI / (hi * hi * hi)) + acc;
     } else {
         Kahan acc;
-#pragma unroll 4
         for (int s = 0; s < cnt; s++) {
-            const int32_t j = nbr[(int64_t)s * stride + k];
+            const int4 q4 = nbr4[(int64_t)(s >> 2) * stride + k];
+            const int u = s & 3;
+            const int32_t j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
             const double4 pj = sxm[j];
             const double hj = sh[j];
             const float hb = (float)(sml_avg ? 0.5 * (hi + hj) : hi);
@@ -496,7 +518,7 @@ struct ForceArgs {
 
 template <bool FP64>
 __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= A.n) return;
     const double4 pi = A.sxm[k], vi = A.sv4[k], ri = A.sr4[k];
     const double hi = A.sh[k];
@@ -505,9 +527,12 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
     double out[6];
     if constexpr (FP64) {
         double a0 = 0, a1 = 0, a2 = 0, drho = 0, de = 0, dsml = 0;
-#pragma unroll 2
+        const int4 *nbr4 = reinterpret_cast<const int4 *>(A.nbr);
+        int4 q4 = make_int4(0, 0, 0, 0);
         for (int s = 0; s < cnt; s++) {
-            const int32_t j = A.nbr[(int64_t)s * A.stride + k];
+            if ((s & 3) == 0) q4 = nbr4[(int64_t)(s >> 2) * A.stride + k];
+            const int u = s & 3;
+            const int32_t j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
             const double4 pj = A.sxm[j], vj = A.sv4[j], rj = A.sr4[j];
             const double hj = A.sh[j];
             const double hb = 0.5 * (hi + hj);
@@ -551,9 +576,12 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
         Kahan a0, a1, a2, drho, de, dsml;
         const float hif = (float)hi, rhoif = (float)ri.x, csif = (float)vi.w, prif = (float)ri.y;
         const float alpha = (float)A.alpha, beta = (float)A.beta;
-#pragma unroll 2
+        const int4 *nbr4 = reinterpret_cast<const int4 *>(A.nbr);
+        int4 q4 = make_int4(0, 0, 0, 0);
         for (int s = 0; s < cnt; s++) {
-            const int32_t j = A.nbr[(int64_t)s * A.stride + k];
+            if ((s & 3) == 0) q4 = nbr4[(int64_t)(s >> 2) * A.stride + k];
+            const int u = s & 3;
+            const int32_t j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
             const double4 pj = A.sxm[j], vj = A.sv4[j], rj = A.sr4[j];
             const float hj = (float)A.sh[j];
             const float hb = 0.5f * (hif + hj);
@@ -632,7 +660,10 @@ __global__ __launch_bounds__(256) void k_nnl_to_caller(int64_t n, const int32_t 
     noi[i] = cnt;
     cnt = cnt > cap_nb ? cap_nb : cnt;
     cnt = cnt > max_nb ? max_nb : cnt;
-    for (int s = 0; s < cnt; s++) nnl[(int64_t)i * max_nb + s] = perm[nbr[(int64_t)s * stride + k]];
+    for (int s = 0; s < cnt; s++) {
+        const int32_t j = nbr[((int64_t)(s >> 2) * stride + k) * 4 + (s & 3)];
+        nnl[(int64_t)i * max_nb + s] = perm[j];
+    }
 }
 
 void launch_neighbours_to_caller(mhpc_ctx *c, int32_t *dst_nnl, int32_t *dst_noi, int max_nb) {

@@ -8,8 +8,9 @@
 //    cell is a contiguous range [cstart, cend] of sorted positions;
 //  * the octree in level-major cell order (cell 0 = root), child slots in Lebesgue
 //    octant order (tree.cu:768-1082), node ids: particles [0,n), cells n + c;
-//  * neighbour lists transposed, nbr[k * nbr_stride + i] (sorted indices), so a wave
-//    reading its k-th neighbours touches one contiguous 256 B row.
+//  * neighbour lists in chunks of four, entry s of sorted particle i at
+//    ((s/4) * cap_pad + i) * 4 + s%4: a wave reading its next four neighbours per lane
+//    touches one contiguous 1 KiB row, and the search writes whole 16-byte chunks.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -200,6 +201,17 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 }
 
 __device__ __forceinline__ void set_err(uint32_t *err, uint32_t f) { atomicOr(err, f); }
+
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each with its own
+// L2.  Kernels whose neighbouring workgroups share data (tree walks, neighbour gathers over
+// key-sorted particles) remap the hardware block id so every XCD works on one contiguous
+// chunk of the key order and the shared data stays in one L2.  A bijection on [0, nb).
+#define MHPC_NUM_XCD 8
+__device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nb) {
+    const int64_t x = bid % MHPC_NUM_XCD, i = bid / MHPC_NUM_XCD;
+    const int64_t q = nb / MHPC_NUM_XCD, r = nb % MHPC_NUM_XCD;
+    return x * q + (x < r ? x : r) + i;
+}
 
 // block-wide exclusive scan of one uint32 per thread (blockDim.x = 256); returns the total.
 // Contains two __syncthreads: every thread of the block must call it.
