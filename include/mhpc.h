@@ -189,6 +189,22 @@ int mhpc_eos(mhpc_ctx *ctx, float *ms);
 /* SPH::Kernel::Launch::internalForces (internal_forces.cu:671-676) */
 int mhpc_internal_forces(mhpc_ctx *ctx, float *ms);
 
+/* -------- global sums: Miluphpc::energy / angularMomentum (miluphpc.cpp:577-677),
+ * Physics::Kernel::kineticEnergy and calculateAngularMomentumBlockwise
+ * (subdomain.cu:2428-2619); FP64, compensated, deterministic order -------- */
+typedef struct {
+    double kinetic;                /* sum 1/2 m v^2 */
+    double potential;              /* sum u (u = 1/2 m phi from mhpc_rhs with MHPC_RHS_POTENTIAL) */
+    double internal;               /* sum m e */
+    double total;                  /* kinetic + potential + internal */
+    double momentum[3];            /* sum m v */
+    double angular_momentum[3];    /* sum m x cross v */
+    double angular_momentum_norm;  /* |L| (the reference's totalAngularMomentum) */
+    double mass;
+    double com[3];                 /* centre of mass */
+} mhpc_diagnostics;
+int mhpc_compute_diagnostics(mhpc_ctx *ctx, mhpc_diagnostics *out);
+
 /* -------- integrator kernels (LeapfrogNS::Kernel::Launch::updateX / updateV,
  * device_leapfrog.cu:4-58) -------- */
 int mhpc_leapfrog_update_x(mhpc_ctx *ctx, double dt);

@@ -101,7 +101,8 @@ static void free_all(mhpc_ctx *c) {
                     c->sr4, c->sh, c->sh2, c->se, c->srho, c->sp, c->scs, c->snoi, c->siters, c->sdone, c->lcp, c->tcnt, c->toff,
                     c->level_count, c->cell_base, c->cstart, c->cend, c->clevel, c->cchild,
                     c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax,
-                    c->nbr, c->htable, c->bbox_partial, c->box, c->err, c->counters};
+                    c->nbr, c->htable, c->bbox_partial, c->box, c->err, c->counters,
+                    c->diag_partial, c->diag_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
@@ -173,6 +174,8 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->htable, MHPC_HTABLE_MAX + 1));
     upload_htable(c);
     HIPC(dalloc(&c->bbox_partial, 2 * 1024));
+    HIPC(dalloc(&c->diag_partial, diag_partial_size()));
+    HIPC(dalloc(&c->diag_out, diag_num_quantities()));
     HIPC(dalloc(&c->box, 6));
     HIPC(hipMemset(c->box, 0, sizeof(double) * 6));
     HIPC(dalloc(&c->err, 1));
@@ -489,6 +492,29 @@ extern "C" int mhpc_leapfrog_update_v(mhpc_ctx *c, double dt) {
     launch_leapfrog_v(c, dt);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(c->stream));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_compute_diagnostics(mhpc_ctx *c, mhpc_diagnostics *out) {
+    if (!c || !out) return MHPC_E_ARG;
+    double q[32];
+    launch_diagnostics(c, c->diag_partial, c->diag_out);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(q, c->diag_out, sizeof(double) * diag_num_quantities(), hipMemcpyDeviceToHost,
+                        c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    memset(out, 0, sizeof(*out));
+    out->kinetic = q[0];
+    out->potential = q[1];
+    out->internal = q[2];
+    out->total = q[0] + q[1] + q[2];
+    for (int a = 0; a < 3; a++) {
+        out->momentum[a] = q[3 + a];
+        out->angular_momentum[a] = q[6 + a];
+        out->com[a] = q[9] > 0.0 ? q[10 + a] / q[9] : 0.0;
+    }
+    out->angular_momentum_norm = sqrt(q[6] * q[6] + q[7] * q[7] + q[8] * q[8]);
+    out->mass = q[9];
     return MHPC_OK;
 }
 

@@ -102,6 +102,7 @@ struct mhpc_ctx {
 
     // misc
     double *bbox_partial = nullptr;              // [2][bbox blocks]
+    double *diag_partial = nullptr, *diag_out = nullptr;
     double *box = nullptr;                       // [6]
     uint32_t *err = nullptr;
     unsigned long long *counters = nullptr;
@@ -134,6 +135,9 @@ void launch_scatter_h(mhpc_ctx *c);
 void launch_leapfrog_x(mhpc_ctx *c, double dt);
 void launch_leapfrog_v(mhpc_ctx *c, double dt);
 void launch_neighbours_to_caller(mhpc_ctx *c, int32_t *dst_nnl, int32_t *dst_noi, int max_nb);
+int diag_num_quantities();
+int diag_partial_size();
+void launch_diagnostics(mhpc_ctx *c, double *partial, double *out);
 
 // ------------------------------------------------------------------------------------
 // device helpers

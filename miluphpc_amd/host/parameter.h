@@ -1,0 +1,78 @@
+// parameter.h -- host-side types of the drop-in facade.  Names follow the reference's
+// include/parameter.h (real/integer/keyType typedefs :12-19, SimulationParameters
+// :122-159, struct To :161-171) so calling code compiles against either.
+#pragma once
+
+#include <string>
+
+typedef double real;
+typedef int integer;
+typedef unsigned long keyType;
+typedef int idInteger;
+
+#define MAX_LEVEL 21
+#define DIM 3
+#define POW_DIM 8
+
+struct To {
+    enum Target { host, device };
+    Target t_;
+    To(Target t) : t_(t) {}
+    operator Target() const { return t_; }
+};
+
+namespace Curve {
+enum Type { lebesgue = 0, hilbert = 1 };
+}
+
+// Runtime parameters.  Fields of the reference that configure the multi-process, I/O or
+// load-balancing machinery (outputRank, loadBalancing*, particlesSent2H5, domainListSize,
+// removeParticles*) are kept so existing set-up code compiles; they have no effect in a
+// single-subdomain build.
+struct SimulationParameters {
+    std::string directory;
+    std::string logDirectory;
+    int verbosity = 0;
+    bool timeKernels = true;
+    int numOutputFiles = 1;
+    real timeStep = 1e-3;
+    real maxTimeStep = -1.0;
+    real timeEnd = 1e-3;
+    bool loadBalancing = false;
+    int loadBalancingInterval = 0;
+    int loadBalancingBins = 0;
+    std::string inputFile;
+    std::string materialConfigFile;
+    int outputRank = 0;
+    bool performanceLog = false;
+    bool particlesSent2H5 = false;
+    int sfcSelection = Curve::lebesgue;
+    int integratorSelection = 0;
+    // gravity
+    real theta = 0.5;
+    real smoothing = 0.0;          // eps^2
+    // acceptance form of the reference's kernel versions (miluphpc.cpp:2150-2280): 0 and 1
+    // (computeForces_v2, _v2_1) test s^2/theta <= d^2+eps^2; 2, 3, 4 (computeForces_v1,
+    // _v1_1, _v1_2) test (d^2+eps^2) theta^2 > s^2.  Both are evaluated per group of 64
+    // key-sorted particles against the group's bounding box.
+    int gravityForceVersion = 0;
+    // SPH
+    int smoothingKernelSelection = 1;   // cubic spline
+    int sphFixedRadiusNNVersion = 0;
+    bool removeParticles = false;
+    int removeParticlesCriterion = 0;
+    real removeParticlesDimension = 0.0;
+    int bins = 0;
+    bool calculateAngularMomentum = false;
+    bool calculateEnergy = false;
+    bool calculateCenterOfMass = false;
+    real particleMemoryContingent = 1.0;
+    int domainListSize = 0;
+    // compile-time switches of parameter.h, runtime here
+    bool gravitySim = true;        // GRAVITY_SIM
+    bool sphSim = false;           // SPH_SIM
+    bool variableSML = false;      // VARIABLE_SML (plain h-iteration)
+    // build extensions
+    bool quadrupole = false;       // monopole + quadrupole far field
+    int precision = 0;             // 0 FP64 pair sums, 1 FP32 + Kahan
+};

@@ -76,6 +76,27 @@ class Params(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Diagnostics(C.Structure):
+    _fields_ = [
+        ("kinetic", C.c_double),
+        ("potential", C.c_double),
+        ("internal", C.c_double),
+        ("total", C.c_double),
+        ("momentum", C.c_double * 3),
+        ("angular_momentum", C.c_double * 3),
+        ("angular_momentum_norm", C.c_double),
+        ("mass", C.c_double),
+        ("com", C.c_double * 3),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if not isinstance(v, float) else v
+        return out
+
+
 class RhsStats(C.Structure):
     _fields_ = [
         ("ms_total", C.c_float),
@@ -146,6 +167,7 @@ def _load() -> C.CDLL:
         "mhpc_density": (i32, [vp, fp]),
         "mhpc_eos": (i32, [vp, fp]),
         "mhpc_internal_forces": (i32, [vp, fp]),
+        "mhpc_compute_diagnostics": (i32, [vp, C.POINTER(Diagnostics)]),
         "mhpc_leapfrog_update_x": (i32, [vp, C.c_double]),
         "mhpc_leapfrog_update_v": (i32, [vp, C.c_double]),
     }
@@ -310,6 +332,11 @@ class Context:
 
     def leapfrog_update_v(self, dt: float) -> None:
         self._check(LIB.mhpc_leapfrog_update_v(self._h, float(dt)), "mhpc_leapfrog_update_v")
+
+    def diagnostics(self) -> dict:
+        d = Diagnostics()
+        self._check(LIB.mhpc_compute_diagnostics(self._h, C.byref(d)), "mhpc_compute_diagnostics")
+        return d.as_dict()
 
     def synchronize(self) -> None:
         self._check(LIB.mhpc_synchronize(self._h), "mhpc_synchronize")
