@@ -31,14 +31,37 @@ def log(msg: str) -> None:
 
 
 def sph_bytes_per_particle(nnb: float) -> dict:
-    """Algorithmic (compulsory) HBM bytes per particle of each SPH kernel."""
+    """Algorithmic (compulsory) HBM bytes per particle of each SPH kernel: each particle's
+    own fields once, the neighbour list (4 B per entry + 4 B count) once, neighbour data
+    assumed reused through LDS / L2."""
     return {
         "ms_nn": 32 + 4 + 4 * nnb,           # read x,y,z,h; write list + count
         "ms_density": 40 + 4 + 4 * nnb + 8,  # read x,y,z,h,m + list; write rho
         "ms_eos": 24 + 16,                   # read rho,e,p; write p,cs
         "ms_forces": 88 + 4 + 4 * nnb + 48,  # read 11 fields + list; write 6 fields
-        "ms_hiter": 32 + 8,                  # per active particle and pass: read x,y,z,h; write h
     }
+
+
+# the kernel the roofline line is quoted for: the largest total time per RHS by kernel
+# name in the rocprofv3 kernel statistics of this workload (one launch per RHS)
+ROOFLINE_KERNEL = "ms_forces"
+ROOFLINE_KERNEL_NAME = "k_forces<true, false>"
+
+
+def measured_traffic(kernel_name: str):
+    """HBM bytes per launch of `kernel_name` from the committed PMC summary (FETCH_SIZE +
+    WRITE_SIZE, calibrated against a plain copy), or None if there is none."""
+    path = REPO / "profiles" / "traffic_sedov128.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split("\t")
+        if len(parts) >= 2 and parts[0].strip() == kernel_name:
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
 
 
 def time_rhs(ctx, lib, flags, steps, warmup, label, restore=None):
@@ -100,6 +123,10 @@ def main() -> int:
     ap.add_argument("--no-plummer", action="store_true")
     ap.add_argument("--no-sedov", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-nside", type=int, default=128)
+    ap.add_argument("--learned-steps", type=int, default=10)
+    ap.add_argument("--learned-train-iters", type=int, default=100)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -118,9 +145,9 @@ def main() -> int:
         nbytes = args.copy_calibration_mb << 20
         src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.neg(src, out=dst)   # one elementwise kernel: reads and writes nbytes each
         torch.cuda.synchronize()
-        log(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written")
+        log(f"calibration copy (neg kernel): {nbytes} bytes read, {nbytes} bytes written")
         del src, dst
 
     result = {
@@ -134,6 +161,8 @@ def main() -> int:
         s = run_sedov(args, lib, ics, torch, result)
     if not args.no_plummer:
         run_plummer(args, lib, ics, result)
+    if not args.no_learned:
+        run_learned(args, lib, ics, result)
     if not args.no_cpu and s is not None:
         log("cpu baseline (oracle) ...")
         result["cpu_baseline"] = cpu_baseline_sedov(s)
@@ -157,11 +186,13 @@ def run_sedov(args, lib, ics, torch, result):
     med = median_stats(stats)
     nnb = med["n_neighbours"] / n
     bpp = sph_bytes_per_particle(nnb)
-    kern = {k: med[k] for k in bpp}
-    dom = max(kern, key=kern.get)
-    units = n if dom != "ms_hiter" else max(1, med["n_h_counts"])
-    traffic_bytes = bpp[dom] * units
-    achieved = traffic_bytes / (med[dom] * 1e-3) / 1e9
+    dom = ROOFLINE_KERNEL
+    alg_bytes = bpp[dom] * n
+    achieved = alg_bytes / (med[dom] * 1e-3) / 1e9
+    kernels = {k[3:]: {"ms": round(med[k], 4), "alg_bytes_per_particle": round(bpp[k], 1),
+                       "GB/s": round(bpp[k] * n / (med[k] * 1e-3) / 1e9, 1),
+                       "frac": round(bpp[k] * n / (med[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+               for k in bpp}
     result.update({
         "value": n / wall,
         "ms_per_step": wall * 1e3,
@@ -169,15 +200,79 @@ def run_sedov(args, lib, ics, torch, result):
         "config": {"workload": f"Sedov-Taylor {args.nside}^3 lattice SPH RHS, plain h-iteration "
                                f"(target 64, band 50-80), cubic spline, ideal gas, FP64",
                    "n_particles": n, "path": "plain"},
-        "roofline": {"bound": "hbm", "kernel": dom[3:], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": ROOFLINE_KERNEL_NAME, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(ROOFLINE_KERNEL_NAME),
+                     "algorithmic_bytes_per_launch": alg_bytes,
                      "algorithmic_bytes_per_particle": bpp[dom], "kernel_ms": med[dom]},
+        "sph_kernels": kernels,
         "phases_ms": {k[3:]: round(v, 4) for k, v in med.items() if k.startswith("ms_")},
         "mean_neighbours": nnb,
         "mean_h_counts": med["n_h_counts"] / n,
     })
     log(f"sedov: {n / wall:.4e} particles/s, dominant {dom} {med[dom]:.3f} ms, {achieved:.1f} GB/s")
     return s
+
+
+STATE_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "mass", "sml", "e", "rho", "p", "cs", "ax", "ay", "az",
+                "g_ax", "g_ay", "g_az", "drhodt", "dedt", "dsmldt", "muijmax", "curlv", "gradrho", "noi")
+
+
+def run_learned(args, lib, ics, result):
+    """Learned-h vs plain on an evolving Sedov run (same GPU, same bands): 5 plain
+    predictor-corrector steps produce the training pairs (the network is trained online
+    on the GPU after each RHS), then the same state is advanced K steps three ways --
+    plain (reference structure: count passes, then a full list walk), speculative lists
+    with the previous h as the guess (no network), and learned (network prediction +
+    speculative lists) -- and the RHS time and h-iteration counts are compared."""
+    import torch
+
+    nside = args.learned_nside
+    s = ics.sedov_lattice(nside)
+    n = len(s["x"])
+    for k in ("ax", "ay", "az", "dedt", "g_ax", "g_ay", "g_az", "muijmax", "drhodt", "dsmldt"):
+        s.setdefault(k, np.zeros(n))
+    p = ics.params_sedov()
+    base = lib.RHS_SPH | lib.RHS_VARIABLE_SML
+    train = dict(nsamples=65536, iters=args.learned_train_iters, lr=2e-3, seed=5)
+    out = {}
+    with lib.Context(n, p) as ctx:
+        ctx.load(**{k: v for k, v in s.items() if k in lib.FIELD_ID})
+        ctx.mlp_init(20261202)
+        t0 = time.perf_counter()
+        losses = []
+        for step in range(5):
+            _, s0, s1 = ctx.pc_euler_step(base | lib.RHS_FEATURES, courant_fact=0.3, train=train)
+            losses += [s0["mlp_loss"], s1["mlp_loss"]]
+        torch.cuda.synchronize()
+        log(f"learned: 5 plain training steps ({n} particles), loss {losses[0]:.3g} -> {losses[-1]:.3g}, "
+            f"{time.perf_counter() - t0:.2f} s")
+        snap = {k: ctx.download(k) for k in STATE_FIELDS}
+        for name, flags in (("plain", base), ("speculative", base | lib.RHS_SPECULATIVE_NN),
+                            ("learned", base | lib.RHS_LEARNED_H)):
+            ctx.load(**snap)
+            rhs_ms, hcounts, hiter_ms, nn_ms, mlp_ms = [], [], [], [], []
+            for step in range(args.learned_steps):
+                _, s0, s1 = ctx.pc_euler_step(flags, courant_fact=0.3)
+                for st in (s0, s1):
+                    rhs_ms.append(st["ms_total"])
+                    hcounts.append(st["n_h_counts"] / n)
+                    hiter_ms.append(st["ms_hiter"])
+                    nn_ms.append(st["ms_nn"])
+                    mlp_ms.append(st["ms_mlp"])
+            out[name] = {"rhs_ms": float(np.median(rhs_ms)), "mean_h_iterations": float(np.mean(hcounts)),
+                         "hiter_ms": float(np.median(hiter_ms)), "nn_ms": float(np.median(nn_ms)),
+                         "mlp_ms": float(np.median(mlp_ms))}
+            log(f"learned-vs-plain: {name}: {out[name]['rhs_ms']:.3f} ms/RHS, "
+                f"{out[name]['mean_h_iterations']:.4f} h-iterations")
+    result["learned_h"] = {
+        "workload": f"Sedov {nside}^3 evolving (PC-Euler, CFL 0.3): 5 plain training steps, then "
+                    f"{args.learned_steps} steps each way from the same state; band 50-80, FP64",
+        "n_particles": n, "train_loss_first_last": [losses[0], losses[-1]],
+        **out,
+        "speedup_learned_vs_plain": out["plain"]["rhs_ms"] / out["learned"]["rhs_ms"],
+        "speedup_speculative_vs_plain": out["plain"]["rhs_ms"] / out["speculative"]["rhs_ms"],
+    }
 
 
 def run_plummer(args, lib, ics, result):

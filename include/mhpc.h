@@ -98,6 +98,9 @@ typedef enum {
     MHPC_F_G_AX_OLD, MHPC_F_G_AY_OLD, MHPC_F_G_AZ_OLD,
     MHPC_F_DRHODT, MHPC_F_DEDT, MHPC_F_DSMLDT,
     MHPC_F_U,                                   /* gravitational potential energy 1/2 m phi */
+    MHPC_F_MUIJMAX,                             /* max mu_ij of the artificial viscosity */
+    MHPC_F_CURLV,                               /* |curl v| (learned-h feature) */
+    MHPC_F_GRADRHO,                             /* |grad rho| (learned-h feature) */
     MHPC_F_NOI,                                 /* int32: number of interactions */
     MHPC_F_KEY,                                 /* uint64: SFC key of each particle */
     MHPC_F_SORTED,                              /* int32: key-sorted order -> particle index */
@@ -113,7 +116,13 @@ enum {
     MHPC_RHS_VARIABLE_SML = 1u << 2,   /* plain h-iteration before the neighbour search */
     MHPC_RHS_POTENTIAL = 1u << 3,      /* also accumulate the gravitational potential */
     MHPC_RHS_TIMING = 1u << 4,         /* per-phase hipEvent timing (adds no sync) */
-    MHPC_RHS_LEARNED_H = 1u << 5       /* MLP predicts h; the plain iteration corrects */
+    MHPC_RHS_LEARNED_H = 1u << 5,      /* MLP predicts h; the plain iteration corrects */
+    MHPC_RHS_SPECULATIVE_NN = 1u << 6, /* pass 0 of the h-iteration writes the neighbour
+                                          lists; only particles whose list can change are
+                                          re-listed (same sets; implied by LEARNED_H) */
+    MHPC_RHS_FEATURES = 1u << 7        /* learned-h features only (curl v, grad rho, MLP
+                                          inputs), for plain steps that produce training
+                                          pairs (implied by LEARNED_H) */
 };
 
 typedef struct {
@@ -204,6 +213,45 @@ typedef struct {
     double com[3];                 /* centre of mass */
 } mhpc_diagnostics;
 int mhpc_compute_diagnostics(mhpc_ctx *ctx, mhpc_diagnostics *out);
+
+/* -------- predictor-corrector Euler (PredictorCorrectorEulerNS, predictor_corrector_euler.cpp,
+ * device_predictor_corrector_euler.cu:130-540) -------- */
+/* CFL time step (Kernel::Launch::setTimeStep): dt = min(courant_fact * min h/cs,
+ * forces_fact * min sqrt(h/|a+g|), courant_fact * min h/(cs + 1.2(alpha cs + beta mu_max)),
+ * max h / max |v|, dt_max if > 0).  parts (may be NULL) receives the five terms. */
+int mhpc_time_step(mhpc_ctx *ctx, double courant_fact, double forces_fact, double dt_max,
+                   double *dt, double parts[5]);
+/* Kernel::Launch::predictor: saves the state and writes the predicted one into the fields */
+int mhpc_pc_predictor(mhpc_ctx *ctx, double dt);
+/* Kernel::Launch::corrector: combines the saved state with the predicted derivatives */
+int mhpc_pc_corrector(mhpc_ctx *ctx, double dt);
+
+/* -------- learned smoothing length (build extension, no reference counterpart): a
+ * 7 -> 64 -> 64 -> 1 tanh MLP, BF16 weights on MFMA with FP32 accumulation, predicting
+ * log(h_converged / h_previous); the plain h-iteration starts from the prediction and
+ * remains the judge (MHPC_RHS_LEARNED_H).  Parameters: flat FP32 vector of
+ * MHPC_MLP_NPARAM = W1[64][8] (input 8 is padding), b1[64], W2[64][64], b2[64], W3[64], b3.
+ * norm[18]: feature means [0..6], feature stds [8..14], target mean [16], target std [17]. */
+#define MHPC_MLP_NPARAM 4801
+int mhpc_mlp_init(mhpc_ctx *ctx, uint64_t seed);
+int mhpc_mlp_set_params(mhpc_ctx *ctx, const float *params, int nparam);
+int mhpc_mlp_get_params(mhpc_ctx *ctx, float *params, int nparam);
+int mhpc_mlp_set_norm(mhpc_ctx *ctx, const float norm[18]);
+int mhpc_mlp_get_norm(mhpc_ctx *ctx, float norm[18]);
+/* features of every particle from the current fields (the previous RHS's outputs) and
+ * the time step dt; also saves h as h_previous.  Raw (unnormalised) features if
+ * norm has not been set. */
+int mhpc_mlp_features(mhpc_ctx *ctx, double dt);
+int mhpc_mlp_download_features(mhpc_ctx *ctx, float *X, int64_t n);
+/* sml <- h_previous * exp(prediction) (clamped to a factor of 4) */
+int mhpc_mlp_predict(mhpc_ctx *ctx, float *ms);
+/* `iters` Adam steps on random subsets of `nsamples` pairs (features, log(sml /
+ * h_previous)) -- the pairs the plain h-iteration just produced */
+int mhpc_mlp_train(mhpc_ctx *ctx, int nsamples, int iters, float lr, uint64_t seed, float *loss);
+/* test hooks: forward / loss gradient on caller-given normalised inputs and targets */
+int mhpc_mlp_forward(mhpc_ctx *ctx, const float *X, int64_t n, float *Y);
+int mhpc_mlp_gradient(mhpc_ctx *ctx, const float *X, const float *T, int64_t n, float *grad,
+                      float *loss);
 
 /* -------- integrator kernels (LeapfrogNS::Kernel::Launch::updateX / updateV,
  * device_leapfrog.cu:4-58) -------- */

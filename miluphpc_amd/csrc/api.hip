@@ -102,7 +102,11 @@ static void free_all(mhpc_ctx *c) {
                     c->level_count, c->cell_base, c->cstart, c->cend, c->clevel, c->cchild,
                     c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax,
                     c->nbr, c->htable, c->bbox_partial, c->box, c->err, c->counters,
-                    c->diag_partial, c->diag_out};
+                    c->diag_partial, c->diag_out, c->pc_backup, c->mlp_params, c->mlp_m1, c->mlp_m2,
+                    c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
+                    c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
+                    c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->hslot, c->sdirty,
+                    c->hmax_bits};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
@@ -151,6 +155,13 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->siters, cap));
     HIPC(hipMemset(c->siters, 0, sizeof(int32_t) * (size_t)cap));
     HIPC(dalloc(&c->sdone, cap));
+    HIPC(dalloc(&c->hlist[0], cap));
+    HIPC(dalloc(&c->hlist[1], cap));
+    HIPC(dalloc(&c->hlist[2], cap));
+    HIPC(dalloc(&c->sdirty, cap));
+    HIPC(dalloc(&c->hmax_bits, 1));
+    HIPC(dalloc(&c->hlist_count, MHPC_MAX_PASSES + 1));
+    HIPC(dalloc(&c->hslot, cap));
     HIPC(dalloc(&c->lcp, cap));
     c->tree_tiles_cap = ceil_div(cap, MHPC_TREE_TILE);
     HIPC(dalloc(&c->tcnt, (MHPC_LEVELS + 1) * c->tree_tiles_cap));
@@ -425,7 +436,7 @@ extern "C" int mhpc_h_iterate(mhpc_ctx *c, float *ms) {
     if (!c) return MHPC_E_ARG;
     if (!c->tree_valid) return MHPC_E_STATE;
     int st = run_stage(c, ms, [&] {
-        launch_h_iterate(c, c->p.max_counts);
+        launch_h_iterate(c, c->p.max_counts, false);
         launch_scatter_h(c);
         return (int)MHPC_OK;
     });
@@ -472,9 +483,282 @@ extern "C" int mhpc_internal_forces(mhpc_ctx *c, float *ms) {
     if (!c) return MHPC_E_ARG;
     if (!c->eos_valid) return MHPC_E_STATE;
     return run_stage(c, ms, [&] {
-        launch_forces(c);
+        launch_forces(c, false);
         return (int)MHPC_OK;
     });
+}
+
+extern "C" int mhpc_time_step(mhpc_ctx *c, double courant_fact, double forces_fact, double dt_max,
+                              double *dt, double parts[5]) {
+    if (!c || !dt) return MHPC_E_ARG;
+    double q[5];
+    launch_time_step(c, c->diag_partial, c->diag_out);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(q, c->diag_out, sizeof(q), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    double t = fmin(courant_fact * q[1], forces_fact * q[0]);
+    t = fmin(t, courant_fact * q[2]);
+    if (q[3] > 0.0 && q[4] > 0.0) t = fmin(t, q[4] / q[3]);
+    if (dt_max > 0.0) t = fmin(t, dt_max);
+    *dt = t;
+    c->last_dt = t;
+    if (parts)
+        for (int k = 0; k < 5; k++) parts[k] = q[k];
+    return MHPC_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// learned smoothing length (no reference counterpart)
+#define MLP_MAX_SAMPLES 262144
+
+static int ensure_mlp(mhpc_ctx *c) {
+    if (c->mlp_params) return MHPC_OK;
+    const int np = mlp_num_params();
+    const int64_t max_waves = ceil_div(MLP_MAX_SAMPLES, 64);
+    HIPC(dalloc(&c->mlp_params, np));
+    HIPC(dalloc(&c->mlp_m1, np));
+    HIPC(dalloc(&c->mlp_m2, np));
+    HIPC(dalloc(&c->mlp_grad, np));
+    HIPC(dalloc(&c->mlp_norm, 18));
+    HIPC(dalloc(&c->mlp_x, c->cap * 8));
+    HIPC(dalloc(&c->mlp_t, c->cap));
+    HIPC(dalloc(&c->mlp_hprev, c->cap));
+    HIPC(dalloc(&c->mlp_idx, MLP_MAX_SAMPLES));
+    HIPC(dalloc(&c->mlp_partial, max_waves * np));
+    HIPC(dalloc(&c->mlp_loss_partial, max_waves));
+    HIPC(dalloc(&c->mlp_loss, 1));
+    float norm[18];
+    for (int k = 0; k < 18; k++) norm[k] = (k >= 8 && k < 15) || k == 17 ? 1.f : 0.f;
+    HIPC(hipMemcpy(c->mlp_norm, norm, sizeof(norm), hipMemcpyHostToDevice));
+    c->mlp_norm_set = false;
+    c->mlp_x_valid = false;
+    return MHPC_OK;
+}
+
+static uint64_t splitmix64(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static int upload_mlp_params(mhpc_ctx *c, const float *params) {
+    const size_t bytes = sizeof(float) * (size_t)mlp_num_params();
+    HIPC(hipMemcpy(c->mlp_params, params, bytes, hipMemcpyHostToDevice));
+    HIPC(hipMemset(c->mlp_m1, 0, bytes));
+    HIPC(hipMemset(c->mlp_m2, 0, bytes));
+    c->mlp_adam_step = 0;
+    return MHPC_OK;
+}
+
+// uniform(-a, a) weights with a = sqrt(6 / (fan_in + fan_out)) (Glorot), zero biases
+extern "C" int mhpc_mlp_init(mhpc_ctx *c, uint64_t seed) {
+    if (!c) return MHPC_E_ARG;
+    int st = ensure_mlp(c);
+    if (st != MHPC_OK) return st;
+    const int H = 64, I = 8;
+    std::vector<float> p((size_t)mlp_num_params(), 0.f);
+    uint64_t s = seed;
+    auto uni = [&](float a) { return (float)(((double)(splitmix64(s) >> 11) * 0x1.0p-53 * 2.0 - 1.0) * a); };
+    const float a1 = sqrtf(6.f / (7 + H)), a2 = sqrtf(6.f / (H + H)), a3 = sqrtf(6.f / (H + 1));
+    for (int o = 0; o < H; o++)
+        for (int k = 0; k < I; k++) p[(size_t)(o * I + k)] = k < 7 ? uni(a1) : 0.f;
+    float *w2 = p.data() + H * I + H;
+    for (int t = 0; t < H * H; t++) w2[t] = uni(a2);
+    float *w3 = w2 + H * H + H;
+    for (int t = 0; t < H; t++) w3[t] = uni(a3);
+    return upload_mlp_params(c, p.data());
+}
+
+extern "C" int mhpc_mlp_set_params(mhpc_ctx *c, const float *params, int nparam) {
+    if (!c || !params || nparam != mlp_num_params()) return MHPC_E_ARG;
+    int st = ensure_mlp(c);
+    if (st != MHPC_OK) return st;
+    return upload_mlp_params(c, params);
+}
+
+extern "C" int mhpc_mlp_get_params(mhpc_ctx *c, float *params, int nparam) {
+    if (!c || !params || nparam != mlp_num_params()) return MHPC_E_ARG;
+    if (!c->mlp_params) return MHPC_E_STATE;
+    HIPC(hipMemcpy(params, c->mlp_params, sizeof(float) * (size_t)nparam, hipMemcpyDeviceToHost));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_mlp_set_norm(mhpc_ctx *c, const float norm[18]) {
+    if (!c || !norm) return MHPC_E_ARG;
+    for (int k = 0; k < 7; k++)
+        if (!(norm[8 + k] > 0.f)) return MHPC_E_ARG;
+    if (!(norm[17] > 0.f)) return MHPC_E_ARG;
+    int st = ensure_mlp(c);
+    if (st != MHPC_OK) return st;
+    HIPC(hipMemcpy(c->mlp_norm, norm, sizeof(float) * 18, hipMemcpyHostToDevice));
+    c->mlp_norm_set = true;
+    c->mlp_x_valid = false;   // features must be recomputed with the new statistics
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_mlp_get_norm(mhpc_ctx *c, float norm[18]) {
+    if (!c || !norm) return MHPC_E_ARG;
+    if (!c->mlp_norm) return MHPC_E_STATE;
+    HIPC(hipMemcpy(norm, c->mlp_norm, sizeof(float) * 18, hipMemcpyDeviceToHost));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_mlp_features(mhpc_ctx *c, double dt) {
+    if (!c || !(dt >= 0.0)) return MHPC_E_ARG;
+    if (!c->mlp_params) return MHPC_E_STATE;
+    launch_mlp_features(c, dt);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->stream));
+    c->mlp_x_valid = true;
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_mlp_download_features(mhpc_ctx *c, float *X, int64_t n) {
+    if (!c || !X || n != c->n) return MHPC_E_ARG;
+    if (!c->mlp_x_valid) return MHPC_E_STATE;
+    HIPC(hipMemcpy(X, c->mlp_x, sizeof(float) * 8 * (size_t)n, hipMemcpyDeviceToHost));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_mlp_predict(mhpc_ctx *c, float *ms) {
+    if (!c) return MHPC_E_ARG;
+    if (!c->mlp_x_valid || !c->mlp_norm_set) return MHPC_E_STATE;
+    return run_stage(c, ms, [&] {
+        launch_mlp_predict(c);
+        return (int)MHPC_OK;
+    });
+}
+
+// feature / target statistics from the current raw features and the converged h
+static int mlp_take_norm(mhpc_ctx *c) {
+    const int64_t n = c->n;
+    std::vector<float> X((size_t)n * 8);
+    std::vector<double> h((size_t)n), hp((size_t)n);
+    HIPC(hipMemcpy(X.data(), c->mlp_x, sizeof(float) * X.size(), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(h.data(), c->f[MHPC_F_SML], sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hp.data(), c->mlp_hprev, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+    float norm[18] = {};
+    for (int k = 0; k < 8; k++) {
+        double m = 0.0, q = 0.0;
+        for (int64_t i = 0; i < n; i++) m += X[(size_t)(i * 8 + k)];
+        m /= (double)n;
+        for (int64_t i = 0; i < n; i++) {
+            const double d = X[(size_t)(i * 8 + k)] - m;
+            q += d * d;
+        }
+        const double sd = sqrt(q / (double)n);
+        norm[k] = k < 7 ? (float)m : 0.f;
+        norm[8 + k] = k < 7 && sd > 1e-6 * (fabs(m) + 1e-30) && sd > 0.0 ? (float)sd : 1.f;
+    }
+    double m = 0.0, q = 0.0;
+    for (int64_t i = 0; i < n; i++) m += log(h[(size_t)i] / hp[(size_t)i]);
+    m /= (double)n;
+    for (int64_t i = 0; i < n; i++) {
+        const double d = log(h[(size_t)i] / hp[(size_t)i]) - m;
+        q += d * d;
+    }
+    const double sd = sqrt(q / (double)n);
+    norm[16] = (float)m;
+    norm[17] = sd > 1e-4 ? (float)sd : 1e-4f;
+    HIPC(hipMemcpy(c->mlp_norm, norm, sizeof(norm), hipMemcpyHostToDevice));
+    launch_mlp_renorm(c);
+    HIPC(hipGetLastError());
+    c->mlp_norm_set = true;
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_mlp_train(mhpc_ctx *c, int nsamples, int iters, float lr, uint64_t seed, float *loss) {
+    if (!c || nsamples < 1 || nsamples > MLP_MAX_SAMPLES || iters < 0 || !(lr > 0.f)) return MHPC_E_ARG;
+    if (!c->mlp_x_valid || c->n < 1) return MHPC_E_STATE;
+    if (!c->mlp_norm_set) {
+        int st = mlp_take_norm(c);
+        if (st != MHPC_OK) return st;
+    }
+    launch_mlp_targets(c);
+    for (int it = 0; it < iters; it++) {
+        launch_mlp_sample(c, nsamples, seed + (uint64_t)it);
+        launch_mlp_gradient(c, nsamples, c->mlp_x, c->mlp_t, c->mlp_idx, c->mlp_grad, c->mlp_loss);
+        launch_mlp_adam(c, lr, ++c->mlp_adam_step);
+    }
+    HIPC(hipGetLastError());
+    float l = 0.f;
+    if (iters > 0) HIPC(hipMemcpyAsync(&l, c->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
+    return MHPC_OK;
+}
+
+static int ensure_mlp_io(mhpc_ctx *c, int64_t n) {
+    if (c->mlp_io_cap >= n) return MHPC_OK;
+    if (c->mlp_io_x) (void)hipFree(c->mlp_io_x);
+    if (c->mlp_io_y) (void)hipFree(c->mlp_io_y);
+    c->mlp_io_x = c->mlp_io_y = nullptr;
+    c->mlp_io_cap = 0;
+    HIPC(dalloc(&c->mlp_io_x, n * 8));
+    HIPC(dalloc(&c->mlp_io_y, n));
+    c->mlp_io_cap = n;
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_mlp_forward(mhpc_ctx *c, const float *X, int64_t n, float *Y) {
+    if (!c || !X || !Y || n < 1) return MHPC_E_ARG;
+    if (!c->mlp_params) return MHPC_E_STATE;
+    int st = ensure_mlp_io(c, n);
+    if (st != MHPC_OK) return st;
+    HIPC(hipMemcpyAsync(c->mlp_io_x, X, sizeof(float) * 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    launch_mlp_forward_raw(c, n, c->mlp_io_x, c->mlp_io_y);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(Y, c->mlp_io_y, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_mlp_gradient(mhpc_ctx *c, const float *X, const float *T, int64_t n, float *grad,
+                                 float *loss) {
+    if (!c || !X || !T || !grad || n < 1 || n > MLP_MAX_SAMPLES) return MHPC_E_ARG;
+    if (!c->mlp_params) return MHPC_E_STATE;
+    int st = ensure_mlp_io(c, n);
+    if (st != MHPC_OK) return st;
+    HIPC(hipMemcpyAsync(c->mlp_io_x, X, sizeof(float) * 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->mlp_io_y, T, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    launch_mlp_gradient(c, n, c->mlp_io_x, c->mlp_io_y, nullptr, c->mlp_grad, c->mlp_loss);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(grad, c->mlp_grad, sizeof(float) * (size_t)mlp_num_params(), hipMemcpyDeviceToHost,
+                        c->stream));
+    float l = 0.f;
+    HIPC(hipMemcpyAsync(&l, c->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
+    return MHPC_OK;
+}
+
+static int ensure_pc_backup(mhpc_ctx *c) {
+    if (c->pc_backup) return MHPC_OK;
+    HIPC(dalloc(&c->pc_backup, (int64_t)pc_backup_fields() * c->cap));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_pc_predictor(mhpc_ctx *c, double dt) {
+    if (!c) return MHPC_E_ARG;
+    int st = ensure_pc_backup(c);
+    if (st != MHPC_OK) return st;
+    launch_pc_predictor(c, dt);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->stream));
+    c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
+    c->nn_valid = c->rho_valid = c->eos_valid = false;
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_pc_corrector(mhpc_ctx *c, double dt) {
+    if (!c || !c->pc_backup) return c ? MHPC_E_STATE : MHPC_E_ARG;
+    launch_pc_corrector(c, dt);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->stream));
+    c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
+    c->nn_valid = c->rho_valid = c->eos_valid = false;
+    return MHPC_OK;
 }
 
 extern "C" int mhpc_leapfrog_update_x(mhpc_ctx *c, double dt) {
@@ -580,13 +864,32 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     const bool sph = flags & MHPC_RHS_SPH;
     const bool hiter = sph && (flags & MHPC_RHS_VARIABLE_SML);
     const bool pot = grav && (flags & MHPC_RHS_POTENTIAL);
+    const bool learned = hiter && (flags & MHPC_RHS_LEARNED_H);
+    // learned-h features (MLP inputs and curl v / grad rho) without the prediction:
+    // the plain steps that produce training pairs
+    const bool feats = hiter && (flags & (MHPC_RHS_LEARNED_H | MHPC_RHS_FEATURES));
+    // speculative lists: pass 0 of the h-iteration writes the lists, and only particles
+    // whose list can differ from the final one are re-listed (same neighbour sets)
+    const bool spec = hiter && (learned || (flags & MHPC_RHS_SPECULATIVE_NN));
     if (sph) {
         int st = ensure_nbr(c);
         if (st != MHPC_OK) return st;
     }
+    if (feats) {
+        int st = ensure_mlp(c);
+        if (st != MHPC_OK) return st;
+    }
+    // the learned start applies once the model has feature statistics; before that the
+    // features are still taken (raw) so that a following mhpc_mlp_train can use them
+    const bool predict = learned && c->mlp_norm_set;
     int e = 0;
     auto mark = [&](int idx) { (void)hipEventRecord(c->ev[idx], c->stream); };
     reset_flags(c);
+    mark(13);
+    if (feats) {
+        launch_mlp_features(c, c->last_dt);
+        if (predict) launch_mlp_predict(c);
+    }
     mark(0);
     launch_bbox(c);
     mark(1);
@@ -603,23 +906,25 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     if (grav) launch_gravity(c, pot);
     mark(7);
     if (hiter) {
-        launch_h_iterate(c, c->p.max_counts);
+        launch_h_iterate(c, c->p.max_counts, spec);
         launch_scatter_h(c);
     }
     mark(8);
-    if (sph) launch_neighbours(c);
+    if (spec) launch_neighbours_fixup(c);
+    else if (sph) launch_neighbours(c);
     mark(9);
     if (sph) launch_density(c);
     mark(10);
     if (sph) launch_eos(c);
     mark(11);
-    if (sph) launch_forces(c);
+    if (sph) launch_forces(c, feats);
     mark(12);
     (void)e;
     int st = finish(c);
     c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = (st == MHPC_OK);
     c->com_valid = c->com_valid && grav;
     c->nn_valid = c->rho_valid = c->eos_valid = (st == MHPC_OK) && sph;
+    if (feats) c->mlp_x_valid = st == MHPC_OK;
     if (stats) {
         memset(stats, 0, sizeof(*stats));
         stats->ms_bbox = elapsed(c, 0, 1);
@@ -634,7 +939,8 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
         stats->ms_density = elapsed(c, 9, 10);
         stats->ms_eos = elapsed(c, 10, 11);
         stats->ms_forces = elapsed(c, 11, 12);
-        stats->ms_total = elapsed(c, 0, 12);
+        stats->ms_mlp = elapsed(c, 13, 0);
+        stats->ms_total = elapsed(c, 13, 12);
         int32_t cb[MHPC_LEVELS + 3];
         if (hipMemcpy(cb, c->cell_base, sizeof(cb), hipMemcpyDeviceToHost) == hipSuccess)
             stats->num_cells = cb[MHPC_LEVELS + 1];

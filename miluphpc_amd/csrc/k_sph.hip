@@ -9,6 +9,8 @@
 // LDS in tiles of 64 with coalesced SoA loads; every lane tests every candidate of the
 // tile against its own particle with the reference criterion d^2 < h_i^2 and
 // d^2 < h_j^2 (FP64, exact), so the neighbour sets equal a brute-force search.
+#include <algorithm>
+
 #include "mhpc_internal.h"
 
 #define NN_WAVES 2
@@ -50,47 +52,33 @@ struct NNArgs {
     const double *table;
     int tmax;
     int pass, max_counts, band_lo, band_hi;
+    // compacted lists of the particles still iterating: pass p reads list_in / count_in
+    // (per-particle mode) and appends the particles that stay unconverged to list_out
+    const int32_t *list_in;
+    const uint32_t *count_in;
+    int32_t *list_out;
+    uint32_t *count_out;
+    double *h_slot;           // per-particle mode: new h of list slot t
+    const uint8_t *only;      // list mode: when set, only particles with only[k] != 0
+    uint8_t *dirty;           // dirty marking: particles whose list must be rebuilt
+    const double *hmax_p;     // dirty marking: largest h of all particles
     uint32_t *err;
     unsigned long long *counters;
 };
 
-// MODE 0: build lists; MODE 1: one Jacobi pass of the h-iteration (count only)
-template <int MODE>
-__global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
-    __shared__ NNWaveLds lds_all[NN_WAVES];
-    const int w = threadIdx.x / 64, lane = wave_lane();
-    NNWaveLds &L = lds_all[w];
-    const int64_t group = xcd_block(blockIdx.x, gridDim.x) * NN_WAVES + w;
-    const int64_t k = group * 64 + lane;
+// Walk the tree for the axis-aligned box [lo, hi] and hand the candidate particles to
+// `tile` in LDS tiles of up to 64 (L.tx/ty/tz/th2/tj, count m).  Leaves of at most
+// NN_LEAF particles and single particles are taken as ranges of sorted positions; larger
+// cells are opened.  Returns false on stack overflow.  Wave-uniform control flow.
+template <typename TileFn>
+__device__ __forceinline__ bool nn_walk_box(NNWaveLds &L, const NNArgs &A, double lox, double hix, double loy,
+                                            double hiy, double loz, double hiz, TileFn &&tile) {
+    const int lane = wave_lane();
     const int64_t n = A.n;
-    if (group * 64 >= n) return;
-    const bool valid = k < n;
-    bool active = valid;
-    if (MODE == 1 && valid) active = A.done[k] == 0;
-    const uint64_t mActive = wave_ballot(active);
-    if (mActive == 0) {
-        if (MODE == 1 && valid) A.h_next[k] = A.h[k];
-        return;
-    }
-    const double4 own = valid ? A.sxm[k] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const double xi = own.x, yi = own.y, zi = own.z;
-    const double hi = valid ? A.h[k] : 0.0;
-    const double hi2 = hi * hi;
-    // search box: bounding box of the active lanes grown by their largest h (plus a
-    // relative margin far above FP64 rounding, so pruning never drops a true pair)
-    const double hmax = wave_max(active ? hi : 0.0);
-    const double grow = hmax * (1.0 + 1e-9);
-    const double lox = wave_min(active ? xi : 1e300) - grow, hix = wave_max(active ? xi : -1e300) + grow;
-    const double loy = wave_min(active ? yi : 1e300) - grow, hiy = wave_max(active ? yi : -1e300) + grow;
-    const double loz = wave_min(active ? zi : 1e300) - grow, hiz = wave_max(active ? zi : -1e300) + grow;
-
-    int count = 0;
-    int b0 = 0, b1 = 0, b2 = 0;   // pending list entries, flushed four at a time as one int4
     int top = 1, nr = 0;
     if (lane == 0) L.stack[0] = (int)n;
     wave_sync();
     const uint64_t lt = lanemask_lt();
-    bool overflow = false;
     for (;;) {
         // walk until 64 ranges are pending or the stack is empty
         while (top > 0 && nr < 64) {
@@ -141,10 +129,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             }
             const int incl = wave_incl_scan(nc);
             const int tot = __shfl(incl, 63);
-            if (top + tot > NN_STACK_PHYS) {
-                overflow = true;
-                break;
-            }
+            if (top + tot > NN_STACK_PHYS) return false;
             if (open) {
                 int pos = top + incl - nc;
 #pragma unroll
@@ -154,7 +139,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             top += tot;
             wave_sync();
         }
-        if (overflow || (nr == 0 && top == 0)) break;
+        if (nr == 0 && top == 0) return true;
         // stream the candidates of up to 64 ranges through LDS
         const int np = nr < 64 ? nr : 64;
         const int v = lane < np ? L.rcnt[lane] : 0;
@@ -180,32 +165,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
                 L.tj[lane] = j;
             }
             wave_sync();
-            const int m = (total - base) < 64 ? (total - base) : 64;
-            if (active) {
-                for (int e = 0; e < m; e++) {
-                    const double dx = xi - L.tx[e];
-                    const double dy = yi - L.ty[e];
-                    const double dz = zi - L.tz[e];
-                    const double d = dx * dx + dy * dy + dz * dz;
-                    const int j = L.tj[e];
-                    if (d < hi2 && d < L.th2[e] && j != (int)k) {
-                        if (MODE == 0) {
-                            const int slot = count & 3;
-                            if (slot == 3) {
-                                if (count < A.max_nb)
-                                    A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, j);
-                            } else if (slot == 0) {
-                                b0 = j;
-                            } else if (slot == 1) {
-                                b1 = j;
-                            } else {
-                                b2 = j;
-                            }
-                        }
-                        count++;
-                    }
-                }
-            }
+            tile((total - base) < 64 ? (total - base) : 64);
             wave_sync();
         }
         const int rest = nr - np;
@@ -217,11 +177,109 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
         nr = rest;
         wave_sync();
     }
-    if (overflow) {
+}
+
+// the plain h-iteration's per-particle update after a count N (sph.cu:909-935 in the
+// Jacobi form): finish when N is in the band (or at the last pass) and below max_nb,
+// else scale h by 1/2 (1 + (n_target / N)^(1/3)) from the table.  Returns fin.
+__device__ __forceinline__ bool h_update(const NNArgs &A, int N, double hi, double &hn) {
+    const bool inband = N >= A.band_lo && N <= A.band_hi;
+    const bool last = A.pass == A.max_counts - 1;
+    bool fin = false;
+    hn = hi;
+    if ((inband || last) && N < A.max_nb) {
+        fin = true;
+    } else {
+        int Nc = N < 1 ? 1 : N;
+        Nc = Nc > A.tmax ? A.tmax : Nc;
+        hn = hi * A.table[Nc];
+        if (last) fin = true;
+    }
+    if (fin && (N >= A.max_nb || N == 0)) set_err(A.err, ERRF_CAP_NEIGHBOURS);
+    return fin;
+}
+
+// Group mode: one wavefront owns 64 consecutive sorted particles and walks once with the
+// bounding box of its active particles grown by their largest h.
+// MODE 0: build lists (only the particles flagged in `only` when it is set);
+// MODE 1: one Jacobi pass of the h-iteration (count only; with list_out set, the
+// unconverged particles are appended to it for the next pass);
+// MODE 2: MODE 1 that also writes every particle's list with the h of this pass
+// (speculative: kept for the particles whose list cannot change afterwards)
+template <int MODE>
+__global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
+    __shared__ NNWaveLds lds_all[NN_WAVES];
+    const int w = threadIdx.x / 64, lane = wave_lane();
+    NNWaveLds &L = lds_all[w];
+    const int64_t group = xcd_block(blockIdx.x, gridDim.x) * NN_WAVES + w;
+    const int64_t k = group * 64 + lane;
+    const int64_t n = A.n;
+    if (group * 64 >= n) return;
+    const bool valid = k < n;
+    bool active = valid;
+    if (MODE >= 1 && valid) active = A.done[k] == 0;
+    if (MODE == 0 && valid && A.only) active = A.only[k] != 0;
+    const uint64_t mActive = wave_ballot(active);
+    if (mActive == 0) {
+        if (MODE >= 1 && valid) A.h_next[k] = A.h[k];
+        return;
+    }
+    const double4 own = valid ? A.sxm[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double xi = own.x, yi = own.y, zi = own.z;
+    const double hi = valid ? A.h[k] : 0.0;
+    const double hi2 = hi * hi;
+    // search box: bounding box of the active lanes grown by their largest h (plus a
+    // relative margin far above FP64 rounding, so pruning never drops a true pair)
+    const double hmax = wave_max(active ? hi : 0.0);
+    const double grow = hmax * (1.0 + 1e-9);
+    const double lox = wave_min(active ? xi : 1e300) - grow, hix = wave_max(active ? xi : -1e300) + grow;
+    const double loy = wave_min(active ? yi : 1e300) - grow, hiy = wave_max(active ? yi : -1e300) + grow;
+    const double loz = wave_min(active ? zi : 1e300) - grow, hiz = wave_max(active ? zi : -1e300) + grow;
+
+    int count = 0;
+    int b0 = 0, b1 = 0, b2 = 0;   // pending list entries, flushed four at a time as one int4
+    const bool ok = nn_walk_box(L, A, lox, hix, loy, hiy, loz, hiz, [&](int m) {
+        if (!active) return;
+        for (int e = 0; e < m; e++) {
+            const double dx = xi - L.tx[e];
+            const double dy = yi - L.ty[e];
+            const double dz = zi - L.tz[e];
+            const double d = dx * dx + dy * dy + dz * dz;
+            const int j = L.tj[e];
+            if (d < hi2 && d < L.th2[e] && j != (int)k) {
+                if (MODE != 1) {
+                    const int slot = count & 3;
+                    if (slot == 3) {
+                        if (count < A.max_nb)
+                            A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, j);
+                    } else if (slot == 0) {
+                        b0 = j;
+                    } else if (slot == 1) {
+                        b1 = j;
+                    } else {
+                        b2 = j;
+                    }
+                }
+                count++;
+            }
+        }
+    });
+    if (!ok) {
         if (lane == 0) set_err(A.err, ERRF_STACK);
         return;
     }
-    if (MODE == 0) {
+    if (MODE == 2 && active && (count & 3) != 0 && count < A.max_nb)
+        A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, 0);
+    if (MODE == 0 && A.only) {
+        // re-list of flagged particles: counts and caller-order outputs are published for
+        // all particles by k_publish_noi afterwards
+        if (active) {
+            if ((count & 3) != 0 && count < A.max_nb)
+                A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, 0);
+            A.snoi[k] = count;
+            if (count > A.max_nb) set_err(A.err, ERRF_CAP_NEIGHBOURS);
+        }
+    } else if (MODE == 0) {
         if (valid && (count & 3) != 0 && count < A.max_nb)
             A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, 0);
         if (valid) {
@@ -239,34 +297,153 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             atomicMax(&A.counters[CNT_MAX_NOI], (unsigned long long)mx);
         }
     } else {
+        bool stay = false;
         if (valid) {
             if (active) {
-                const int N = count;
                 A.iters[k] = A.pass + 1;
-                A.snoi[k] = N;
-                const bool inband = N >= A.band_lo && N <= A.band_hi;
-                const bool last = A.pass == A.max_counts - 1;
-                bool fin = false;
-                double hn = hi;
-                if ((inband || last) && N < A.max_nb) {
-                    fin = true;
-                } else {
-                    int Nc = N < 1 ? 1 : N;
-                    Nc = Nc > A.tmax ? A.tmax : Nc;
-                    hn = hi * A.table[Nc];
-                    if (last) fin = true;
-                }
+                A.snoi[k] = count;
+                double hn;
+                const bool fin = h_update(A, count, hi, hn);
                 A.h_next[k] = hn;
-                if (fin) {
-                    A.done[k] = 1;
-                    if (N >= A.max_nb || N == 0) set_err(A.err, ERRF_CAP_NEIGHBOURS);
-                }
+                if (fin) A.done[k] = 1;
+                stay = !fin;
             } else {
                 A.h_next[k] = hi;
             }
         }
+        if (A.list_out) {
+            // append the unconverged particles (wave-aggregated; list order is free, the
+            // next pass's results do not depend on it)
+            const uint64_t mS = wave_ballot(stay);
+            uint32_t b = 0;
+            if (lane == 0 && mS) b = atomicAdd(A.count_out, (uint32_t)popc64(mS));
+            b = __shfl(b, 0);
+            if (stay) A.list_out[b + popc64(mS & lanemask_lt())] = (int32_t)k;
+        }
         if (lane == 0) atomicAdd(&A.counters[CNT_H_COUNTS], (unsigned long long)popc64(mActive));
     }
+}
+
+// Per-particle mode of an h-iteration pass (passes after the first): one wavefront per
+// listed particle, searching the particle's own box x +- h with the 64 lanes testing 64
+// candidates at a time.  Same criterion and update as the group mode, so the counts and
+// the new h are identical; only the unconverged particles are visited.  The new h goes
+// to h_slot[t] and k_h_apply writes it after the pass (Jacobi: every count of pass p
+// sees the h of pass p).
+#define NN1_WAVES 4
+__global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one_pass(NNArgs A) {
+    __shared__ NNWaveLds lds_all[NN1_WAVES];
+    const int w = threadIdx.x / 64, lane = wave_lane();
+    NNWaveLds &L = lds_all[w];
+    const uint32_t cnt = *A.count_in;
+    const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
+    for (int64_t t = (int64_t)blockIdx.x * NN1_WAVES + w; t < cnt; t += nwaves) {
+        const int64_t k = A.list_in[t];
+        const double4 own = A.sxm[k];
+        const double hi = A.h[k];
+        const double hi2 = hi * hi;
+        const double grow = hi * (1.0 + 1e-9);
+        int count = 0;
+        const bool ok = nn_walk_box(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
+                                    own.z + grow, [&](int m) {
+            bool hit = false;
+            if (lane < m) {
+                const double dx = own.x - L.tx[lane];
+                const double dy = own.y - L.ty[lane];
+                const double dz = own.z - L.tz[lane];
+                const double d = dx * dx + dy * dy + dz * dz;
+                hit = d < hi2 && d < L.th2[lane] && L.tj[lane] != (int)k;
+            }
+            count += popc64(wave_ballot(hit));
+        });
+        if (!ok) {
+            if (lane == 0) set_err(A.err, ERRF_STACK);
+            return;
+        }
+        double hn;
+        const bool fin = h_update(A, count, hi, hn);
+        if (lane == 0) {
+            A.iters[k] = A.pass + 1;
+            A.snoi[k] = count;
+            A.h_slot[t] = hn;
+            if (fin) A.done[k] = 1;
+            else A.list_out[atomicAdd(A.count_out, 1u)] = (int32_t)k;
+            atomicAdd(&A.counters[CNT_H_COUNTS], 1ull);
+        }
+        wave_sync();
+    }
+}
+
+// Dirty marking after a speculative pass 0: for every particle j whose h changed after
+// pass 0 (list_in), flag each particle i with d_ij < h_i (the only pairs whose membership
+// d < h_i and d < h_j can differ from pass 0), and j itself.  One wavefront per changed
+// particle, searching x_j +- max h; lanes are candidates.
+__global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
+    __shared__ NNWaveLds lds_all[NN1_WAVES];
+    const int w = threadIdx.x / 64, lane = wave_lane();
+    NNWaveLds &L = lds_all[w];
+    const uint32_t cnt = *A.count_in;
+    const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
+    const double H = *A.hmax_p;
+    const double grow = H * (1.0 + 1e-9);
+    for (int64_t t = (int64_t)blockIdx.x * NN1_WAVES + w; t < cnt; t += nwaves) {
+        const int64_t jc = A.list_in[t];
+        const double4 own = A.sxm[jc];
+        if (lane == 0) A.dirty[jc] = 1;
+        const bool ok = nn_walk_box(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
+                                    own.z + grow, [&](int m) {
+            if (lane < m) {
+                const double dx = own.x - L.tx[lane];
+                const double dy = own.y - L.ty[lane];
+                const double dz = own.z - L.tz[lane];
+                const double d = dx * dx + dy * dy + dz * dz;
+                if (d < L.th2[lane]) A.dirty[L.tj[lane]] = 1;
+            }
+        });
+        if (!ok) {
+            if (lane == 0) set_err(A.err, ERRF_STACK);
+            return;
+        }
+        wave_sync();
+    }
+}
+
+// largest h (positive doubles order like their bit patterns)
+__global__ __launch_bounds__(256) void k_hmax(int64_t n, const double *__restrict__ h,
+                                              unsigned long long *__restrict__ out) {
+    double m = 0.0;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, h[k]);
+    m = wave_max(m);
+    if (wave_lane() == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// caller-order counts and the neighbour statistics from the sorted counts
+__global__ __launch_bounds__(256) void k_publish_noi(int64_t n, const int32_t *__restrict__ perm,
+                                                     const int32_t *__restrict__ snoi, int32_t *__restrict__ noi,
+                                                     unsigned long long *__restrict__ counters) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int v = k < n ? snoi[k] : 0;
+    if (k < n) noi[perm[k]] = v;
+    const unsigned long long s = (unsigned long long)wave_sum(v);
+    const int mx = wave_max(v);
+    if (wave_lane() == 0) {
+        atomicAdd(&counters[CNT_NEIGHBOURS], s);
+        atomicMax(&counters[CNT_MAX_NOI], (unsigned long long)mx);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, int64_t n) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+
+// h[list[t]] = h_slot[t] after a per-particle pass
+__global__ __launch_bounds__(256) void k_h_apply(const int32_t *__restrict__ list, const uint32_t *__restrict__ count,
+                                                 const double *__restrict__ h_slot, double *__restrict__ h) {
+    const uint32_t cnt = *count;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x)
+        h[list[t]] = h_slot[t];
 }
 
 static NNArgs nn_common(mhpc_ctx *c) {
@@ -285,19 +462,23 @@ static NNArgs nn_common(mhpc_ctx *c) {
 
 static unsigned nn_grid(mhpc_ctx *c) { return (unsigned)ceil_div(ceil_div(c->n, 64), NN_WAVES); }
 
-__global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters) {
+// done flags and iteration counts to zero; the per-pass list counters to zero
+__global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *list_counts, int nlc) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k < n) {
         done[k] = 0;
         iters[k] = 0;
     }
+    if (k < nlc) list_counts[k] = 0;
 }
 
-// a9: the plain h-iteration as Jacobi passes; sh holds the final h afterwards
-void launch_h_iterate(mhpc_ctx *c, int passes) {
+// a9: the plain h-iteration as Jacobi passes; sh holds the final h afterwards.  Pass 0
+// runs in group mode over all particles and lists the unconverged ones; later passes run
+// in per-particle mode over that list only.
+void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     if (c->n <= 0) return;
-    hipLaunchKernelGGL(k_h_init, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->n,
-                       c->sdone, c->siters);
+    hipLaunchKernelGGL(k_h_init, dim3((unsigned)ceil_div(c->n > 64 ? c->n : 64, 256)), dim3(256), 0, c->stream,
+                       c->n, c->sdone, c->siters, c->hlist_count, MHPC_MAX_PASSES + 1);
     NNArgs a = nn_common(c);
     a.done = c->sdone;
     a.iters = c->siters;
@@ -307,15 +488,63 @@ void launch_h_iterate(mhpc_ctx *c, int passes) {
     a.max_counts = passes;
     a.band_lo = c->p.band_lo;
     a.band_hi = c->p.band_hi;
-    for (int pass = 0; pass < passes; pass++) {
+    // pass 0, group mode; its list of unconverged particles (hlist[2]) is also the list
+    // of particles whose h changed after pass 0
+    a.pass = 0;
+    a.h = c->sh;
+    a.h_next = c->sh2;
+    a.list_out = c->hlist[2];
+    a.count_out = c->hlist_count + 0;
+    if (speculative) {
+        a.nbr4 = reinterpret_cast<int4 *>(c->nbr);
+        a.stride = c->cap_pad;
+        hipLaunchKernelGGL(k_nn_walk<2>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
+        a.nbr4 = nullptr;
+    } else {
+        hipLaunchKernelGGL(k_nn_walk<1>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
+    }
+    std::swap(c->sh, c->sh2);
+    // passes 1.., per-particle mode over the list of the previous pass
+    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), 2048);
+    for (int pass = 1; pass < passes; pass++) {
         a.pass = pass;
         a.h = c->sh;
-        a.h_next = c->sh2;
-        hipLaunchKernelGGL(k_nn_walk<1>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
-        double *t = c->sh;
-        c->sh = c->sh2;
-        c->sh2 = t;
+        a.list_in = pass == 1 ? c->hlist[2] : c->hlist[(pass - 1) & 1];
+        a.count_in = c->hlist_count + (pass - 1);
+        a.list_out = c->hlist[pass & 1];
+        a.count_out = c->hlist_count + pass;
+        a.h_slot = c->hslot;
+        hipLaunchKernelGGL(k_nn_one_pass, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
+        hipLaunchKernelGGL(k_h_apply, dim3(256), dim3(256), 0, c->stream, a.list_in, a.count_in, c->hslot, c->sh);
     }
+}
+
+// After a speculative h-iteration: re-list exactly the particles whose pass-0 list may
+// differ from the list with the final h (the changed particles and every particle within
+// its own h of one), then publish counts.  The lists then equal those of a full list walk
+// with the final h as sets (order may differ).
+void launch_neighbours_fixup(mhpc_ctx *c) {
+    if (c->n <= 0) return;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->sdirty, c->n);
+    launch_fill_u64(c, c->hmax_bits, 0ull, 1);
+    hipLaunchKernelGGL(k_hmax, dim3(512), dim3(256), 0, c->stream, c->n, c->sh, c->hmax_bits);
+    NNArgs a = nn_common(c);
+    a.h = c->sh;
+    a.list_in = c->hlist[2];
+    a.count_in = c->hlist_count + 0;
+    a.dirty = c->sdirty;
+    a.hmax_p = reinterpret_cast<const double *>(c->hmax_bits);
+    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), 2048);
+    hipLaunchKernelGGL(k_nn_mark_dirty, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
+    NNArgs b = nn_common(c);
+    b.h = c->sh;
+    b.nbr4 = reinterpret_cast<int4 *>(c->nbr);
+    b.stride = c->cap_pad;
+    b.snoi = c->snoi;
+    b.only = c->sdirty;
+    hipLaunchKernelGGL(k_nn_walk<0>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, b);
+    hipLaunchKernelGGL(k_publish_noi, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->n, c->perm,
+                       c->snoi, c->noi, c->counters);
 }
 
 void launch_neighbours(mhpc_ctx *c) {
@@ -331,8 +560,6 @@ void launch_neighbours(mhpc_ctx *c) {
     a.iters_out = nullptr;
     hipLaunchKernelGGL(k_nn_walk<0>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
 }
-
-
 
 // ------------------------------------------------------------------------------------
 // a10 cubic spline (kernel.cu:50-83), support radius h.  FP64 form: the reference's
@@ -513,10 +740,13 @@ struct ForceArgs {
     int max_nb, sml_avg;
     double alpha, beta;
     const int32_t *perm;
-    double *ax, *ay, *az, *drhodt, *dedt, *dsmldt;
+    double *ax, *ay, *az, *drhodt, *dedt, *dsmldt, *muijmax;
+    double *curlv, *gradrho;   // learned-h features (FEAT)
 };
 
-template <bool FP64>
+// FEAT also accumulates curl v = sum (m_j/rho_j) grad W x (v_j - v_i) and
+// grad rho = sum m_j grad W, the learned-h features (plain sums in the pair precision)
+template <bool FP64, bool FEAT>
 __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
     const int64_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= A.n) return;
@@ -525,6 +755,8 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
     int cnt = A.snoi[k];
     cnt = cnt > A.max_nb ? A.max_nb : cnt;
     double out[6];
+    double mumax = 0.0;
+    double c0 = 0, c1 = 0, c2 = 0, gr0 = 0, gr1 = 0, gr2 = 0;
     if constexpr (FP64) {
         double a0 = 0, a1 = 0, a2 = 0, drho = 0, de = 0, dsml = 0;
         const int4 *nbr4 = reinterpret_cast<const int4 *>(A.nbr);
@@ -555,6 +787,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
             if (vr < 0) {
                 const double csbar = 0.5 * (vi.w + vj.w);
                 const double mu = hb * vr / (r2 + hb * hb * 1e-2);
+                mumax = fmax(mumax, mu);
                 const double rhobar = 0.5 * (ri.x + rj.x);
                 mpij = pj.w * (A.beta * mu - A.alpha * csbar) * mu / rhobar;
             }
@@ -567,6 +800,14 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
             drho += mr_vv;
             dsml += mr_vv;
             de += (mpij - pf) * vv;
+            if (FEAT) {
+                c0 += rj.z * (v1 * g2 - v2 * g1);
+                c1 += rj.z * (v2 * g0 - v0 * g2);
+                c2 += rj.z * (v0 * g1 - v1 * g0);
+                gr0 += pj.w * g0;
+                gr1 += pj.w * g1;
+                gr2 += pj.w * g2;
+            }
         }
         out[0] = a0; out[1] = a1; out[2] = a2;
         out[3] = ri.x * drho;
@@ -601,6 +842,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
             if (vr < 0.f) {
                 const float csbar = 0.5f * (csif + (float)vj.w);
                 const float mu = hb * vr / __builtin_fmaf(hb * hb, 1e-2f, r2);
+                mumax = fmax(mumax, (double)mu);
                 const float rhobar = 0.5f * (rhoif + (float)rj.x);
                 mpij = mj * (beta * mu - alpha * csbar) * mu / rhobar;
             }
@@ -613,6 +855,15 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
             drho.add(mr_vv);
             dsml.add(mr_vv);
             de.add((mpij - pf) * vv);
+            if (FEAT) {
+                const float vj = (float)rj.z;
+                c0 += vj * (v1 * g2 - v2 * g1);
+                c1 += vj * (v2 * g0 - v0 * g2);
+                c2 += vj * (v0 * g1 - v1 * g0);
+                gr0 += mj * g0;
+                gr1 += mj * g1;
+                gr2 += mj * g2;
+            }
         }
         out[0] = a0.value(); out[1] = a1.value(); out[2] = a2.value();
         out[3] = ri.x * drho.value();
@@ -626,9 +877,14 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
     A.drhodt[i] = out[3];
     A.dedt[i] = out[4];
     A.dsmldt[i] = out[5];
+    A.muijmax[i] = mumax;
+    if (FEAT) {
+        A.curlv[i] = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+        A.gradrho[i] = sqrt(gr0 * gr0 + gr1 * gr1 + gr2 * gr2);
+    }
 }
 
-void launch_forces(mhpc_ctx *c) {
+void launch_forces(mhpc_ctx *c, bool features) {
     if (c->n <= 0) return;
     ForceArgs a;
     a.n = c->n;
@@ -639,11 +895,17 @@ void launch_forces(mhpc_ctx *c) {
     a.perm = c->perm;
     a.ax = c->f[MHPC_F_AX]; a.ay = c->f[MHPC_F_AY]; a.az = c->f[MHPC_F_AZ];
     a.drhodt = c->f[MHPC_F_DRHODT]; a.dedt = c->f[MHPC_F_DEDT]; a.dsmldt = c->f[MHPC_F_DSMLDT];
+    a.muijmax = c->f[MHPC_F_MUIJMAX];
+    a.curlv = c->f[MHPC_F_CURLV];
+    a.gradrho = c->f[MHPC_F_GRADRHO];
     const unsigned g = (unsigned)ceil_div(c->n, 256);
-    if (c->p.precision == MHPC_PREC_FP64)
-        hipLaunchKernelGGL(k_forces<true>, dim3(g), dim3(256), 0, c->stream, a);
-    else
-        hipLaunchKernelGGL(k_forces<false>, dim3(g), dim3(256), 0, c->stream, a);
+    if (c->p.precision == MHPC_PREC_FP64) {
+        if (features) hipLaunchKernelGGL((k_forces<true, true>), dim3(g), dim3(256), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_forces<true, false>), dim3(g), dim3(256), 0, c->stream, a);
+    } else {
+        if (features) hipLaunchKernelGGL((k_forces<false, true>), dim3(g), dim3(256), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_forces<false, false>), dim3(g), dim3(256), 0, c->stream, a);
+    }
 }
 
 // ------------------------------------------------------------------------------------

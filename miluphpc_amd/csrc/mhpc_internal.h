@@ -31,6 +31,7 @@
 #define MHPC_SORT_TILE 4096     // keys per scatter block (256 threads x 16)
 #define MHPC_TREE_TILE 1024     // positions per tree block (256 threads x 4)
 #define MHPC_HTABLE_MAX 4096    // h-iteration factor table size
+#define MHPC_MAX_PASSES 64       // upper bound of max_counts
 #define MHPC_NUM_EVENTS 24
 
 // device error flags (sticky word, mapped onto mhpc_status by the host)
@@ -103,6 +104,26 @@ struct mhpc_ctx {
     // misc
     double *bbox_partial = nullptr;              // [2][bbox blocks]
     double *diag_partial = nullptr, *diag_out = nullptr;
+    double *pc_backup = nullptr;                 // predictor-corrector saved state, lazily
+    double last_dt = 0.0;                        // last CFL time step (learned-h feature)
+    int32_t *hlist[3] = {nullptr, nullptr, nullptr};   // h-iteration: unconverged particles per pass
+    uint8_t *sdirty = nullptr;                   // speculative lists: particles to re-list
+    unsigned long long *hmax_bits = nullptr;     // largest h (bit pattern)
+    uint32_t *hlist_count = nullptr;             // [MHPC_MAX_PASSES + 1]
+    double *hslot = nullptr;                     // new h per list slot
+
+    // learned smoothing length
+    float *mlp_params = nullptr, *mlp_m1 = nullptr, *mlp_m2 = nullptr, *mlp_grad = nullptr;
+    float *mlp_norm = nullptr;                   // [18]
+    float *mlp_x = nullptr;                      // [cap][8]
+    float *mlp_t = nullptr;                      // [cap] targets
+    double *mlp_hprev = nullptr;                 // [cap]
+    int32_t *mlp_idx = nullptr;                  // [max samples]
+    float *mlp_partial = nullptr, *mlp_loss_partial = nullptr, *mlp_loss = nullptr;
+    float *mlp_io_x = nullptr, *mlp_io_y = nullptr;   // test-hook staging
+    int64_t mlp_io_cap = 0;
+    int mlp_adam_step = 0;
+    bool mlp_x_valid = false, mlp_norm_set = false;
     double *box = nullptr;                       // [6]
     uint32_t *err = nullptr;
     unsigned long long *counters = nullptr;
@@ -126,15 +147,30 @@ void launch_gather(mhpc_ctx *c, bool gravity, bool sph);
 void launch_tree(mhpc_ctx *c);
 void launch_bottom_up(mhpc_ctx *c, bool gravity);
 void launch_gravity(mhpc_ctx *c, bool potential);
-void launch_h_iterate(mhpc_ctx *c, int passes);
+void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative);
 void launch_neighbours(mhpc_ctx *c);
+void launch_neighbours_fixup(mhpc_ctx *c);
 void launch_density(mhpc_ctx *c);
 void launch_eos(mhpc_ctx *c);
-void launch_forces(mhpc_ctx *c);
+void launch_forces(mhpc_ctx *c, bool features);
 void launch_scatter_h(mhpc_ctx *c);
 void launch_leapfrog_x(mhpc_ctx *c, double dt);
 void launch_leapfrog_v(mhpc_ctx *c, double dt);
 void launch_neighbours_to_caller(mhpc_ctx *c, int32_t *dst_nnl, int32_t *dst_noi, int max_nb);
+int mlp_num_params();
+void launch_mlp_features(mhpc_ctx *c, double dt);
+void launch_mlp_predict(mhpc_ctx *c);
+void launch_mlp_forward_raw(mhpc_ctx *c, int64_t n, const float *X, float *Y);
+void launch_mlp_gradient(mhpc_ctx *c, int64_t nb, const float *X, const float *T, const int32_t *idx,
+                         float *grad, float *loss);
+void launch_mlp_adam(mhpc_ctx *c, float lr, int step);
+void launch_mlp_targets(mhpc_ctx *c);
+void launch_mlp_sample(mhpc_ctx *c, int64_t nsamples, uint64_t seed);
+void launch_mlp_renorm(mhpc_ctx *c);
+int pc_backup_fields();
+void launch_pc_predictor(mhpc_ctx *c, double dt);
+void launch_pc_corrector(mhpc_ctx *c, double dt);
+void launch_time_step(mhpc_ctx *c, double *partial, double *out);
 int diag_num_quantities();
 int diag_partial_size();
 void launch_diagnostics(mhpc_ctx *c, double *partial, double *out);

@@ -72,6 +72,9 @@ struct SimulationParameters {
     bool gravitySim = true;        // GRAVITY_SIM
     bool sphSim = false;           // SPH_SIM
     bool variableSML = false;      // VARIABLE_SML (plain h-iteration)
+    // COURANT_FACT / FORCES_FACT of parameter.h:111-113 (compile-time there)
+    real courantFactor = 0.4;
+    real forcesFactor = 0.2;
     // build extensions
     bool quadrupole = false;       // monopole + quadrupole far field
     int precision = 0;             // 0 FP64 pair sums, 1 FP32 + Kahan

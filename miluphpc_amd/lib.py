@@ -36,11 +36,18 @@ RHS_VARIABLE_SML = 1 << 2
 RHS_POTENTIAL = 1 << 3
 RHS_TIMING = 1 << 4
 RHS_LEARNED_H = 1 << 5
+RHS_SPECULATIVE_NN = 1 << 6
+RHS_FEATURES = 1 << 7
+
+MLP_NPARAM = 4801   # W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3
+MLP_HIDDEN = 64
+MLP_IN = 8          # 7 features + 1 zero pad
 
 FIELDS = [
     "x", "y", "z", "vx", "vy", "vz", "mass", "sml", "e", "rho", "p", "cs",
     "ax", "ay", "az", "g_ax", "g_ay", "g_az", "g_ax_old", "g_ay_old", "g_az_old",
-    "drhodt", "dedt", "dsmldt", "u", "noi", "key", "sorted", "level", "h_iters",
+    "drhodt", "dedt", "dsmldt", "u", "muijmax", "curlv", "gradrho", "noi", "key", "sorted", "level",
+    "h_iters",
 ]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 FIELD_DTYPE = {name: np.float64 for name in FIELDS[: FIELD_ID["noi"]]}
@@ -168,8 +175,22 @@ def _load() -> C.CDLL:
         "mhpc_eos": (i32, [vp, fp]),
         "mhpc_internal_forces": (i32, [vp, fp]),
         "mhpc_compute_diagnostics": (i32, [vp, C.POINTER(Diagnostics)]),
+        "mhpc_time_step": (i32, [vp, C.c_double, C.c_double, C.c_double, dp, dp]),
+        "mhpc_pc_predictor": (i32, [vp, C.c_double]),
+        "mhpc_pc_corrector": (i32, [vp, C.c_double]),
         "mhpc_leapfrog_update_x": (i32, [vp, C.c_double]),
         "mhpc_leapfrog_update_v": (i32, [vp, C.c_double]),
+        "mhpc_mlp_init": (i32, [vp, C.c_uint64]),
+        "mhpc_mlp_set_params": (i32, [vp, vp, i32]),
+        "mhpc_mlp_get_params": (i32, [vp, vp, i32]),
+        "mhpc_mlp_set_norm": (i32, [vp, vp]),
+        "mhpc_mlp_get_norm": (i32, [vp, vp]),
+        "mhpc_mlp_features": (i32, [vp, C.c_double]),
+        "mhpc_mlp_download_features": (i32, [vp, vp, i64]),
+        "mhpc_mlp_predict": (i32, [vp, fp]),
+        "mhpc_mlp_train": (i32, [vp, i32, i32, C.c_float, C.c_uint64, fp]),
+        "mhpc_mlp_forward": (i32, [vp, vp, i64, vp]),
+        "mhpc_mlp_gradient": (i32, [vp, vp, vp, i64, vp, fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -326,6 +347,103 @@ class Context:
 
     def internal_forces(self) -> float:
         return self._stage("mhpc_internal_forces")
+
+    def time_step(self, courant_fact: float = 0.4, forces_fact: float = 0.2, dt_max: float = -1.0):
+        dt = C.c_double(0.0)
+        parts = np.zeros(5)
+        self._check(LIB.mhpc_time_step(self._h, float(courant_fact), float(forces_fact), float(dt_max),
+                                       C.byref(dt), parts.ctypes.data_as(C.POINTER(C.c_double))),
+                    "mhpc_time_step")
+        return float(dt.value), parts
+
+    def pc_euler_step(self, flags: int, courant_fact: float = 0.4, forces_fact: float = 0.2,
+                      dt_max: float = -1.0, t_left: float | None = None,
+                      train: dict | None = None) -> tuple[float, dict, dict]:
+        """One predictor-corrector Euler step (predictor_corrector_euler.cpp): rhs with
+        `flags`, CFL dt, predictor, rhs without self-gravity, corrector.  Returns
+        (dt, stats of the first rhs, stats of the predictor rhs).  With RHS_LEARNED_H in
+        `flags` and `train` given (mlp_train keyword arguments), the network is trained on
+        the pairs each rhs's h-iteration emits, right after that rhs."""
+        s0 = self.rhs(flags)
+        if train is not None:
+            s0["mlp_loss"] = self.mlp_train(**train)
+        dt, _ = self.time_step(courant_fact, forces_fact, dt_max)
+        if t_left is not None:
+            dt = min(dt, t_left)
+        self.pc_predictor(dt)
+        s1 = self.rhs(flags & ~RHS_GRAVITY)
+        if train is not None:
+            s1["mlp_loss"] = self.mlp_train(**train)
+        self.pc_corrector(dt)
+        return dt, s0, s1
+
+    # ---- learned smoothing length
+    def mlp_init(self, seed: int = 1) -> None:
+        self._check(LIB.mhpc_mlp_init(self._h, C.c_uint64(seed)), "mhpc_mlp_init")
+
+    def mlp_set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != MLP_NPARAM:
+            raise ValueError(f"expected {MLP_NPARAM} parameters, got {p.size}")
+        self._check(LIB.mhpc_mlp_set_params(self._h, _ptr(p), MLP_NPARAM), "mhpc_mlp_set_params")
+
+    def mlp_get_params(self) -> np.ndarray:
+        p = np.zeros(MLP_NPARAM, np.float32)
+        self._check(LIB.mhpc_mlp_get_params(self._h, _ptr(p), MLP_NPARAM), "mhpc_mlp_get_params")
+        return p
+
+    def mlp_set_norm(self, norm) -> None:
+        a = np.ascontiguousarray(norm, dtype=np.float32)
+        if a.size != 18:
+            raise ValueError("norm has 18 entries")
+        self._check(LIB.mhpc_mlp_set_norm(self._h, _ptr(a)), "mhpc_mlp_set_norm")
+
+    def mlp_get_norm(self) -> np.ndarray:
+        a = np.zeros(18, np.float32)
+        self._check(LIB.mhpc_mlp_get_norm(self._h, _ptr(a)), "mhpc_mlp_get_norm")
+        return a
+
+    def mlp_features(self, dt: float) -> None:
+        self._check(LIB.mhpc_mlp_features(self._h, float(dt)), "mhpc_mlp_features")
+
+    def mlp_download_features(self) -> np.ndarray:
+        X = np.zeros((self.n, MLP_IN), np.float32)
+        self._check(LIB.mhpc_mlp_download_features(self._h, _ptr(X), self.n), "mhpc_mlp_download_features")
+        return X
+
+    def mlp_predict(self) -> float:
+        return self._stage("mhpc_mlp_predict")
+
+    def mlp_train(self, nsamples: int = 65536, iters: int = 50, lr: float = 1e-3, seed: int = 1) -> float:
+        loss = C.c_float(0.0)
+        self._check(LIB.mhpc_mlp_train(self._h, int(nsamples), int(iters), C.c_float(lr), C.c_uint64(seed),
+                                       C.byref(loss)), "mhpc_mlp_train")
+        return float(loss.value)
+
+    def mlp_forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n = X.shape[0]
+        assert X.shape == (n, MLP_IN)
+        Y = np.zeros(n, np.float32)
+        self._check(LIB.mhpc_mlp_forward(self._h, _ptr(X), n, _ptr(Y)), "mhpc_mlp_forward")
+        return Y
+
+    def mlp_gradient(self, X, T) -> tuple[np.ndarray, float]:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        n = X.shape[0]
+        assert X.shape == (n, MLP_IN) and T.shape == (n,)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_float(0.0)
+        self._check(LIB.mhpc_mlp_gradient(self._h, _ptr(X), _ptr(T), n, _ptr(g), C.byref(loss)),
+                    "mhpc_mlp_gradient")
+        return g, float(loss.value)
+
+    def pc_predictor(self, dt: float) -> None:
+        self._check(LIB.mhpc_pc_predictor(self._h, float(dt)), "mhpc_pc_predictor")
+
+    def pc_corrector(self, dt: float) -> None:
+        self._check(LIB.mhpc_pc_corrector(self._h, float(dt)), "mhpc_pc_corrector")
 
     def leapfrog_update_x(self, dt: float) -> None:
         self._check(LIB.mhpc_leapfrog_update_x(self._h, float(dt)), "mhpc_leapfrog_update_x")

@@ -888,11 +888,11 @@ void orc_forces(int64_t n, const double *x, const double *y, const double *z,
                 const double *h, const double *rho, const double *p, const double *cs,
                 const int32_t *nnl, const int32_t *noi, int max_nb, int sml_avg,
                 double alpha, double beta, double *ax, double *ay, double *az,
-                double *drhodt, double *dedt, double *dsmldt) {
+                double *drhodt, double *dedt, double *dsmldt, double *muijmax) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         double acc[3] = {0, 0, 0};
-        double drho = 0.0, de = 0.0, dsml = 0.0;
+        double drho = 0.0, de = 0.0, dsml = 0.0, mumax = 0.0;
         for (int k = 0; k < noi[i]; k++) {
             int32_t j = nnl[i * max_nb + k];
             double sml = sml_avg ? 0.5 * (h[i] + h[j]) : h[i];
@@ -916,6 +916,7 @@ void orc_forces(int64_t n, const double *x, const double *y, const double *z,
                 double smooth = 0.5 * (h[i] + h[j]);
                 const double eps_artvisc = 1e-2;
                 double mu = smooth * vr / (rr + smooth * smooth * eps_artvisc);
+                if (mu > mumax) mumax = mu;
                 double rhobar = 0.5 * (rho[i] + rho[j]);
                 pij = (beta * mu - alpha * csbar) * mu / rhobar;
             }
@@ -938,6 +939,7 @@ void orc_forces(int64_t n, const double *x, const double *y, const double *z,
         drhodt[i] = drho;
         dedt[i] = de;
         dsmldt[i] = dsml;
+        if (muijmax) muijmax[i] = mumax;
     }
 }
 
@@ -971,11 +973,101 @@ void orc_leapfrog_update_v(int64_t n, double dt, double *vx, double *vy, double 
 }
 
 /* ------------------------------------------------------------------------------------
+ * Predictor-corrector Euler, PredictorCorrectorEulerNS::Kernel::predictor / corrector
+ * (device_predictor_corrector_euler.cu:130-291), hydro fields, INTEGRATE_ENERGY,
+ * h from the predicted state (INTEGRATE_SML 0).  s[] = x,y,z,vx,vy,vz,ax,ay,az,gx,gy,gz,
+ * e,dedt; b[] = saved x,y,z,vx,vy,vz,ax,ay,az,e,dedt.
+ * ---------------------------------------------------------------------------------- */
+void orc_pc_predictor(int64_t n, double dt, double **s, double **b) {
+    for (int64_t i = 0; i < n; i++) {
+        for (int k = 0; k < 9; k++) b[k][i] = s[k][i];
+        b[9][i] = s[12][i];
+        b[10][i] = s[13][i];
+        for (int d = 0; d < 3; d++) {
+            s[d][i] = b[d][i] + dt * b[3 + d][i];
+            s[3 + d][i] = b[3 + d][i] + dt * (b[6 + d][i] + s[9 + d][i]);
+        }
+        double e = b[9][i] + dt * b[10][i];
+        s[12][i] = e < 1e-6 ? 1e-6 : e;
+    }
+}
+
+void orc_pc_corrector(int64_t n, double dt, double **s, double **b) {
+    for (int64_t i = 0; i < n; i++) {
+        for (int d = 0; d < 3; d++) {
+            const double vp = s[3 + d][i], ap = s[6 + d][i], g = s[9 + d][i];
+            s[d][i] = b[d][i] + dt / 2 * (vp + b[3 + d][i]);
+            s[3 + d][i] = b[3 + d][i] + dt / 2 * (ap + b[6 + d][i] + 2 * g);
+            s[6 + d][i] = 0.5 * (ap + b[6 + d][i]) + g;
+        }
+        double e = b[9][i] + dt / 2 * (s[13][i] + b[10][i]);
+        s[12][i] = e < 1e-6 ? 1e-6 : e;
+        s[13][i] = 0.5 * (s[13][i] + b[10][i]);
+    }
+}
+
+/* setTimeStep's per-particle terms (device_predictor_corrector_euler.cu:315-455) */
+void orc_time_step_terms(int64_t n, const double *ax, const double *ay, const double *az,
+                         const double *gx, const double *gy, const double *gz, const double *h,
+                         const double *cs, const double *mu, const double *vx, const double *vy,
+                         const double *vz, double alpha, double beta, double terms[5]) {
+    double forces = 1.7976931348623157e308, courant = forces, artvisc = forces, vmax = 0, hmax = 0;
+    for (int64_t i = 0; i < n; i++) {
+        double a0 = ax[i] + gx[i], a1 = ay[i] + gy[i], a2 = az[i] + gz[i];
+        double t = a0 * a0;
+        t += a1 * a1;
+        t += a2 * a2;
+        double v = sqrt(h[i] / sqrt(t));
+        if (v < forces) forces = v;
+        v = h[i] / cs[i];
+        if (v < courant) courant = v;
+        v = h[i] / (cs[i] + 1.2 * (alpha * cs[i] + beta * mu[i]));
+        if (v < artvisc) artvisc = v;
+        v = sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+        if (v > vmax) vmax = v;
+        if (h[i] > hmax) hmax = h[i];
+    }
+    terms[0] = forces;
+    terms[1] = courant;
+    terms[2] = artvisc;
+    terms[3] = vmax;
+    terms[4] = hmax;
+}
+
+/* ------------------------------------------------------------------------------------
  * Learned smoothing length MLP (build extension, no reference counterpart):
  * A1 = tanh(W1 X + b1), A2 = tanh(W2 A1 + b2), Y = W3 . A2 + b3; loss = mean (Y - T)^2 / 2.
  * W1 is [nh][nin], W2 [nh][nh], W3 [nh]; X is [nb][nin].  The backward returns the loss
  * and writes d loss / d parameter.
  * ---------------------------------------------------------------------------------- */
+void orc_sph_features(int64_t n, const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz, const double *m,
+                      const double *h, const double *rho, const int32_t *nnl, const int32_t *noi,
+                      int max_nb, int sml_avg, double *curl, double *gradrho) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        double c[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+        for (int k = 0; k < noi[i]; k++) {
+            const int32_t j = nnl[i * max_nb + k];
+            const double sml = sml_avg ? 0.5 * (h[i] + h[j]) : h[i];
+            const double dr[3] = {x[i] - x[j], y[i] - y[j], z[i] - z[j]};
+            const double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+            double W, gW[3], dWdr;
+            orc_cubic_spline(r2, dr[0], dr[1], dr[2], sml, &W, gW, &dWdr);
+            const double V = m[j] / rho[j];
+            const double dv[3] = {vx[j] - vx[i], vy[j] - vy[i], vz[j] - vz[i]};
+            c[0] += V * (gW[1] * dv[2] - gW[2] * dv[1]);
+            c[1] += V * (gW[2] * dv[0] - gW[0] * dv[2]);
+            c[2] += V * (gW[0] * dv[1] - gW[1] * dv[0]);
+            for (int e = 0; e < 3; e++) g[e] += m[j] * gW[e];
+        }
+        for (int e = 0; e < 3; e++) {
+            curl[3 * i + e] = c[e];
+            gradrho[3 * i + e] = g[e];
+        }
+    }
+}
+
 void orc_mlp_forward(int64_t nb, int nin, int nh, const double *X, const double *W1,
                      const double *b1, const double *W2, const double *b2, const double *W3,
                      double b3, double *Y, double *A1, double *A2) {

@@ -122,7 +122,7 @@ void orc_forces(int64_t n, const double *x, const double *y, const double *z,
                 const double *h, const double *rho, const double *p, const double *cs,
                 const int32_t *nnl, const int32_t *noi, int max_nb, int sml_avg,
                 double alpha, double beta, double *ax, double *ay, double *az,
-                double *drhodt, double *dedt, double *dsmldt);
+                double *drhodt, double *dedt, double *dsmldt, double *muijmax);
 
 /* ---- a17: leapfrog kick-drift (device_leapfrog.cu:4-58) ---- */
 void orc_leapfrog_update_x(int64_t n, double dt, double *x, double *y, double *z,
@@ -133,7 +133,26 @@ void orc_leapfrog_update_v(int64_t n, double dt, double *vx, double *vy, double 
                            const double *gx, const double *gy, const double *gz,
                            const double *gx_old, const double *gy_old, const double *gz_old);
 
-/* ---- learned smoothing length: 7 -> H -> H -> 1 MLP, FP64 forward/backward ---- */
+/* ---- predictor-corrector Euler (device_predictor_corrector_euler.cu:130-540) ----
+ * s: x,y,z,vx,vy,vz,ax,ay,az,g_ax,g_ay,g_az,e,dedt (14 arrays); b: 9 saved arrays
+ * x,y,z,vx,vy,vz,ax,ay,az + e,dedt (11) */
+void orc_pc_predictor(int64_t n, double dt, double **s, double **b);
+void orc_pc_corrector(int64_t n, double dt, double **s, double **b);
+/* terms[5]: min sqrt(h/|a+g|), min h/cs, min h/(cs+1.2(alpha cs+beta mu)), max|v|, max h */
+void orc_time_step_terms(int64_t n, const double *ax, const double *ay, const double *az,
+                         const double *gx, const double *gy, const double *gz, const double *h,
+                         const double *cs, const double *mu, const double *vx, const double *vy,
+                         const double *vz, double alpha, double beta, double terms[5]);
+
+/* ---- learned smoothing length ----
+ * local-state features over the neighbour lists (no reference counterpart):
+ * curl[3i..] = sum_j (m_j/rho_j) grad_i W_ij x (v_j - v_i),
+ * gradrho[3i..] = sum_j m_j grad_i W_ij, W with the same (averaged) h as the forces */
+void orc_sph_features(int64_t n, const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz, const double *m,
+                      const double *h, const double *rho, const int32_t *nnl, const int32_t *noi,
+                      int max_nb, int sml_avg, double *curl, double *gradrho);
+/* 7 -> H -> H -> 1 tanh MLP, FP64 forward/backward */
 void orc_mlp_forward(int64_t nb, int nin, int nh, const double *X, const double *W1,
                      const double *b1, const double *W2, const double *b2, const double *W3,
                      double b3, double *Y, double *A1, double *A2);

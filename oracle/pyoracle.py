@@ -52,9 +52,13 @@ def _load() -> C.CDLL:
         "orc_soundspeed": (None, [i64, i32, d, d, vp, vp, vp]),
         "orc_pressure": (None, [i64, i32, d, d, vp, vp, vp, vp]),
         "orc_forces": (None, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32,
-                              d, d, vp, vp, vp, vp, vp, vp]),
+                              d, d, vp, vp, vp, vp, vp, vp, vp]),
+        "orc_pc_predictor": (None, [i64, d, vp, vp]),
+        "orc_pc_corrector": (None, [i64, d, vp, vp]),
+        "orc_time_step_terms": (None, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, d, vp]),
         "orc_leapfrog_update_x": (None, [i64, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "orc_leapfrog_update_v": (None, [i64, d, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "orc_sph_features": (None, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, vp]),
         "orc_mlp_forward": (None, [i64, i32, i32, vp, vp, vp, vp, vp, vp, d, vp, vp, vp]),
         "orc_mlp_backward": (d, [i64, i32, i32, vp, vp, vp, vp, vp, vp, vp, d, vp, vp, vp, vp,
                                  vp, vp]),
@@ -268,10 +272,78 @@ def forces(x, y, z, vx, vy, vz, m, h, rho, p, cs, nnl, noi, alpha=1.0, beta=2.0,
     nnl = np.ascontiguousarray(nnl, np.int32)
     noi = np.ascontiguousarray(noi, np.int32)
     n = len(arrs[0])
-    out = [np.empty(n) for _ in range(6)]
+    out = [np.empty(n) for _ in range(7)]
     LIB.orc_forces(n, *[P(a) for a in arrs], P(nnl), P(noi), nnl.shape[1], int(sml_avg), float(alpha),
                    float(beta), *[P(o) for o in out])
-    return dict(zip(["ax", "ay", "az", "drhodt", "dedt", "dsmldt"], out))
+    return dict(zip(["ax", "ay", "az", "drhodt", "dedt", "dsmldt", "muijmax"], out))
+
+
+def sph_features(x, y, z, vx, vy, vz, m, h, rho, nnl, noi, sml_avg=1):
+    """(curl v, grad rho) as (n, 3) arrays over the neighbour lists."""
+    arrs = [f64(a) for a in (x, y, z, vx, vy, vz, m, h, rho)]
+    nnl = np.ascontiguousarray(nnl, np.int32)
+    noi = np.ascontiguousarray(noi, np.int32)
+    n = len(arrs[0])
+    curl = np.empty((n, 3))
+    grad = np.empty((n, 3))
+    LIB.orc_sph_features(n, *[P(a) for a in arrs], P(nnl), P(noi), nnl.shape[1], int(sml_avg), P(curl), P(grad))
+    return curl, grad
+
+
+def mlp_forward(X, W1, b1, W2, b2, W3, b3):
+    """FP64 forward of the 8 -> H -> H -> 1 tanh MLP (rows of X are samples)."""
+    X = f64(X)
+    nb, nin = X.shape
+    nh = W1.shape[0]
+    Y = np.empty(nb)
+    A1 = np.empty((nb, nh))
+    A2 = np.empty((nb, nh))
+    LIB.orc_mlp_forward(nb, nin, nh, P(X), P(f64(W1)), P(f64(b1)), P(f64(W2)), P(f64(b2)), P(f64(W3)),
+                        float(b3), P(Y), P(A1), P(A2))
+    return Y
+
+
+def mlp_backward(X, T, W1, b1, W2, b2, W3, b3):
+    """(loss, dict of gradients) of mean 0.5 (y - t)^2, FP64."""
+    X, T = f64(X), f64(T)
+    nb, nin = X.shape
+    nh = W1.shape[0]
+    g = {"W1": np.zeros((nh, nin)), "b1": np.zeros(nh), "W2": np.zeros((nh, nh)), "b2": np.zeros(nh),
+         "W3": np.zeros(nh), "b3": np.zeros(1)}
+    L = LIB.orc_mlp_backward(nb, nin, nh, P(X), P(T), P(f64(W1)), P(f64(b1)), P(f64(W2)), P(f64(b2)),
+                             P(f64(W3)), float(b3), P(g["W1"]), P(g["b1"]), P(g["W2"]), P(g["b2"]),
+                             P(g["W3"]), P(g["b3"]))
+    return L, g
+
+
+PC_STATE = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "g_ax", "g_ay", "g_az", "e", "dedt")
+
+
+def _ptr_array(arrs):
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def pc_predictor(dt, s: dict, backup: list) -> None:
+    """s: dict of the 14 PC_STATE arrays (float64, modified in place); backup: 11 arrays."""
+    LIB.orc_pc_predictor(len(s["x"]), float(dt), C.cast(_ptr_array([s[k] for k in PC_STATE]), C.c_void_p),
+                         C.cast(_ptr_array(backup), C.c_void_p))
+
+
+def pc_corrector(dt, s: dict, backup: list) -> None:
+    LIB.orc_pc_corrector(len(s["x"]), float(dt), C.cast(_ptr_array([s[k] for k in PC_STATE]), C.c_void_p),
+                         C.cast(_ptr_array(backup), C.c_void_p))
+
+
+def time_step(s: dict, alpha=1.0, beta=2.0, courant_fact=0.4, forces_fact=0.2, dt_max=-1.0):
+    arrs = [f64(s[k]) for k in ("ax", "ay", "az", "g_ax", "g_ay", "g_az", "sml", "cs", "muijmax", "vx", "vy", "vz")]
+    t = np.zeros(5)
+    LIB.orc_time_step_terms(len(arrs[0]), *[P(a) for a in arrs], float(alpha), float(beta), P(t))
+    dt = min(courant_fact * t[1], forces_fact * t[0], courant_fact * t[2])
+    if t[3] > 0 and t[4] > 0:
+        dt = min(dt, t[4] / t[3])
+    if dt_max > 0:
+        dt = min(dt, dt_max)
+    return dt, t
 
 
 def leapfrog_update_x(dt, s: dict) -> None:

@@ -1,0 +1,149 @@
+"""Learned smoothing length on the GPU: MFMA forward and backward kernels against the FP64
+oracle MLP, the SPH feature sums against the oracle, online training, and the learned RHS
+keeping the density consistent with its own converged h."""
+import numpy as np
+import pytest
+
+from miluphpc_amd import ics
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+H, IN = 64, 8
+
+
+def split_params(p):
+    p = np.asarray(p, np.float64)
+    o = 0
+    W1 = p[o:o + H * IN].reshape(H, IN); o += H * IN
+    b1 = p[o:o + H]; o += H
+    W2 = p[o:o + H * H].reshape(H, H); o += H * H
+    b2 = p[o:o + H]; o += H
+    W3 = p[o:o + H]; o += H
+    b3 = p[o]
+    return W1, b1, W2, b2, W3, b3
+
+
+def bf16(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def random_params(rng, scale=0.3):
+    p = (rng.uniform(-1, 1, 4801) * scale).astype(np.float32)
+    return p
+
+
+def random_inputs(rng, n):
+    X = rng.normal(size=(n, IN)).astype(np.float32)
+    X[:, 7] = 0.0
+    return X
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000, 4096])
+def test_mlp_forward_matches_oracle(gpu_lib, orc, n):
+    print_case(f"mlp forward n={n}")
+    rng = np.random.default_rng(n)
+    p = random_params(rng)
+    X = random_inputs(rng, n)
+    with gpu_lib.Context(64) as ctx:
+        ctx.mlp_set_params(p)
+        Y = ctx.mlp_forward(X)
+    # oracle on the BF16-rounded weights and inputs the kernel uses; the remaining
+    # difference is the BF16 rounding of the two hidden activations
+    W1, b1, W2, b2, W3, b3 = split_params(p)
+    Yo = orc.mlp_forward(bf16(X), bf16(W1), b1, bf16(W2), b2, W3, b3)
+    assert np.max(np.abs(Y - Yo)) <= 2e-2 * max(1.0, np.abs(Yo).max())
+    # and against the full-precision network: BF16 storage costs at most a few 1e-2
+    Yf = orc.mlp_forward(X, W1, b1, W2, b2, W3, b3)
+    assert np.max(np.abs(Y - Yf)) <= 5e-2 * max(1.0, np.abs(Yf).max())
+
+
+@pytest.mark.parametrize("n", [100, 1000, 4096])
+def test_mlp_gradient_matches_oracle(gpu_lib, orc, n):
+    print_case(f"mlp gradient n={n}")
+    rng = np.random.default_rng(100 + n)
+    p = random_params(rng)
+    X = random_inputs(rng, n)
+    T = rng.normal(size=n).astype(np.float32)
+    with gpu_lib.Context(64) as ctx:
+        ctx.mlp_set_params(p)
+        g, loss = ctx.mlp_gradient(X, T)
+    W1, b1, W2, b2, W3, b3 = split_params(p)
+    Lo, go = orc.mlp_backward(bf16(X), T, bf16(W1), b1, bf16(W2), b2, W3, b3)
+    assert loss == pytest.approx(Lo, rel=2e-2)
+    gw1, gb1, gw2, gb2, gw3, gb3 = split_params(g)
+    pairs = [(gw1, go["W1"]), (gb1, go["b1"]), (gw2, go["W2"]), (gb2, go["b2"]), (gw3, go["W3"]),
+             (np.array([gb3]), go["b3"])]
+    for name, (a, b) in zip(["W1", "b1", "W2", "b2", "W3", "b3"], pairs):
+        scale = np.abs(b).max()
+        err = np.abs(a - b).max()
+        assert err <= 3e-2 * scale, (name, err, scale)
+        # direction: cosine similarity of the whole block
+        cos = float(np.dot(a.ravel(), b.ravel()) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300))
+        assert cos > 0.999, (name, cos)
+
+
+def sedov_ctx(gpu_lib, nside):
+    s = ics.sedov_lattice(nside)
+    n = len(s["x"])
+    for k in ("ax", "ay", "az", "dedt", "g_ax", "g_ay", "g_az", "muijmax", "drhodt", "dsmldt"):
+        s.setdefault(k, np.zeros(n))
+    ctx = gpu_lib.Context(n, ics.params_sedov())
+    ctx.load(**{k: v for k, v in s.items() if k in gpu_lib.FIELD_ID})
+    return ctx, s
+
+
+def test_features_curl_gradrho_match_oracle(gpu_lib, orc):
+    print_case("learned-h features sedov 16^3 after 2 pc steps")
+    ctx, s = sedov_ctx(gpu_lib, 16)
+    with ctx:
+        flags = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML
+        ctx.mlp_init(3)
+        for _ in range(2):
+            ctx.pc_euler_step(flags, courant_fact=0.3)
+        ctx.rhs(flags | gpu_lib.RHS_LEARNED_H)   # emits curl v and grad rho
+        f = {k: ctx.download(k) for k in ("x", "y", "z", "vx", "vy", "vz", "mass", "sml", "rho", "curlv",
+                                           "gradrho", "noi")}
+        nnl, noi = ctx.neighbour_lists(180)
+    curl, grad = orc.sph_features(f["x"], f["y"], f["z"], f["vx"], f["vy"], f["vz"], f["mass"], f["sml"],
+                                  f["rho"], nnl, noi)
+    cm = np.linalg.norm(curl, axis=1)
+    gm = np.linalg.norm(grad, axis=1)
+    assert np.abs(f["gradrho"] - gm).max() <= 1e-9 * gm.max()
+    assert np.abs(f["curlv"] - cm).max() <= 1e-9 * max(cm.max(), 1e-300)
+    assert gm.max() > 0.0
+
+
+def test_learned_training_reduces_loss_and_keeps_density(gpu_lib, orc):
+    """Online training on the pairs the plain h-iteration emits lowers the loss; with the
+    network in front, every converged h still lies in the neighbour band and the density
+    is the FP64 SPH density of those h."""
+    print_case("learned-h online training sedov 16^3")
+    ctx, s = sedov_ctx(gpu_lib, 16)
+    p = ics.params_sedov()
+    with ctx:
+        flags = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML | gpu_lib.RHS_LEARNED_H
+        ctx.mlp_init(7)
+        train = dict(nsamples=2048, iters=60, lr=3e-3, seed=11)
+        losses = []
+        for step in range(5):
+            _, s0, s1 = ctx.pc_euler_step(flags, courant_fact=0.3, train=train)
+            losses += [s0["mlp_loss"], s1["mlp_loss"]]
+        norm = ctx.mlp_get_norm()
+        assert np.all(norm[8:15] > 0) and norm[17] > 0
+        assert np.isfinite(losses).all()
+        assert min(losses[-4:]) < losses[0]
+        st = ctx.rhs(flags)
+        f = {k: ctx.download(k) for k in ("x", "y", "z", "mass", "sml", "rho", "noi", "h_iters")}
+        nnl, noi = ctx.neighbour_lists(p.max_nb)
+    assert st["ms_mlp"] >= 0.0
+    # converged h: the final counts lie in the band (up to the Jacobi iteration's
+    # neighbour-h changes after a particle finished, as on the plain path)
+    inband = (f["noi"] >= p.band_lo) & (f["noi"] <= p.band_hi)
+    assert inband.mean() >= 0.95
+    st_o, rho_o = orc.density(f["x"], f["y"], f["z"], f["mass"], f["sml"], nnl, noi)
+    assert st_o == 0
+    assert np.max(np.abs(f["rho"] - rho_o) / rho_o) <= 1e-12

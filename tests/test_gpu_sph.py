@@ -150,3 +150,38 @@ def test_gravity_plus_sph_boss_bodenheimer(gpu_lib, orc):
     num = np.sqrt((g["g_ax"] - og["ax"]) ** 2 + (g["g_ay"] - og["ay"]) ** 2 + (g["g_az"] - og["az"]) ** 2)
     den = np.sqrt(og["ax"] ** 2 + og["ay"] ** 2 + og["az"] ** 2)
     assert np.percentile(num / den, 99) < 1e-5
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_speculative_lists_equal_plain(gpu_lib, orc, sedov24, precision):
+    """Speculative lists (pass 0 of the h-iteration writes them, changed neighbourhoods are
+    re-listed) give the plain RHS's h, counts and neighbour sets exactly, and its SPH sums
+    to summation-order rounding; checked again against the oracle's sets."""
+    print_case(f"speculative lists vs plain, sedov 24^3 h*1.25, precision={precision}")
+    s = {**sedov24, "sml": sedov24["sml"] * 1.25}
+    p = gpu_lib.default_params(curve=1, precision=precision)
+    base = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML
+    out = {}
+    for name, flags in (("plain", base), ("spec", base | gpu_lib.RHS_SPECULATIVE_NN)):
+        with make_ctx(gpu_lib, s, p) as ctx:
+            st = ctx.rhs(flags)
+            out[name] = {k: ctx.download(k) for k in ("sml", "noi", "h_iters", "rho", "ax", "ay", "az", "dedt")}
+            out[name]["lists"] = ctx.neighbour_lists()
+            out[name]["stats"] = st
+    a, b = out["plain"], out["spec"]
+    assert np.array_equal(a["sml"], b["sml"])
+    assert np.array_equal(a["h_iters"], b["h_iters"])
+    assert np.array_equal(a["noi"], b["noi"])
+    assert sets(*a["lists"]) == sets(*b["lists"])
+    assert a["stats"]["n_neighbours"] == b["stats"]["n_neighbours"]
+    assert a["stats"]["max_noi"] == b["stats"]["max_noi"]
+    assert (a["h_iters"] > 1).sum() > 100   # the case exercises the re-listing
+    tol = 1e-12 if precision == 0 else 1e-5
+    for k in ("rho", "dedt"):
+        assert np.max(np.abs(a[k] - b[k])) <= tol * np.abs(a[k]).max(), k
+    amag = np.sqrt(a["ax"] ** 2 + a["ay"] ** 2 + a["az"] ** 2).max()
+    for k in ("ax", "ay", "az"):
+        assert np.max(np.abs(a[k] - b[k])) <= tol * amag, k
+    st, onnl, onoi = orc.nn_grid(s["x"], s["y"], s["z"], b["sml"], 180)
+    assert st == 0
+    assert sets(*b["lists"]) == sets(onnl, onoi)

@@ -256,3 +256,74 @@ def test_mlp_gradient_finite_differences(orc):
     lp = loss(W1, b1, W2, b2, W3, b3 + eps)
     lm = loss(W1, b1, W2, b2, W3, b3 - eps)
     assert gb3[0] == pytest.approx((lp - lm) / (2 * eps), rel=1e-5)
+
+
+def test_pc_euler_constant_acceleration_closed_form(orc):
+    """Heun's predictor-corrector is exact for constant acceleration (a + g)."""
+    n = 7
+    rng = np.random.default_rng(25)
+    s = {k: rng.normal(size=n) for k in orc.PC_STATE}
+    x0 = {k: s[k].copy() for k in s}
+    s["e"] = np.abs(s["e"]) + 1.0
+    x0["e"] = s["e"].copy()
+    backup = [np.zeros(n) for _ in range(11)]
+    dt = 0.01
+    orc.pc_predictor(dt, s, backup)
+    # the rhs on the predicted state returns the same (constant) derivatives
+    for k in ("ax", "ay", "az", "dedt"):
+        s[k] = x0[k].copy()
+    orc.pc_corrector(dt, s, backup)
+    for d in ("x", "y", "z"):
+        a = x0["a" + d] + x0["g_a" + d]
+        assert np.allclose(s[d], x0[d] + dt * x0["v" + d] + 0.5 * dt * dt * a, rtol=0, atol=1e-15)
+        assert np.allclose(s["v" + d], x0["v" + d] + dt * a, rtol=0, atol=1e-15)
+        assert np.allclose(s["a" + d], x0["a" + d] + x0["g_a" + d], rtol=0, atol=1e-15)
+    assert np.allclose(s["e"], np.maximum(x0["e"] + dt * x0["dedt"], 1e-6), rtol=0, atol=1e-14)
+
+
+def test_time_step_terms_hand_values(orc):
+    s = {"ax": np.array([3.0, 0.0]), "ay": np.array([4.0, 0.0]), "az": np.zeros(2),
+         "g_ax": np.zeros(2), "g_ay": np.zeros(2), "g_az": np.array([0.0, 0.25]),
+         "sml": np.array([0.5, 0.1]), "cs": np.array([2.0, 0.5]), "muijmax": np.array([0.0, 1.0]),
+         "vx": np.array([0.0, 3.0]), "vy": np.array([0.0, 4.0]), "vz": np.zeros(2)}
+    dt, t = orc.time_step(s, alpha=1.0, beta=2.0, courant_fact=0.3, forces_fact=0.2)
+    # |a| = 5 and 0.25: sqrt(0.5/5) = 0.316.., sqrt(0.1/0.25) = 0.632..
+    assert t[0] == pytest.approx(math.sqrt(0.1))
+    assert t[1] == pytest.approx(0.2)            # min(0.5/2, 0.1/0.5)
+    assert t[2] == pytest.approx(0.1 / (0.5 + 1.2 * (0.5 + 2.0)))   # 0.1/3.5
+    assert (t[3], t[4]) == (5.0, 0.5)
+    assert dt == pytest.approx(min(0.3 * 0.2, 0.2 * math.sqrt(0.1), 0.3 * 0.1 / 3.5, 0.5 / 5.0))
+
+
+def _lattice(nside, h_factor=2.5):
+    dx = 1.0 / nside
+    g = (np.arange(nside) + 0.5) * dx
+    x, y, z = [a.ravel().copy() for a in np.meshgrid(g, g, g, indexing="ij")]
+    return x, y, z, dx, h_factor * dx
+
+
+def test_sph_features_linear_fields(orc):
+    """Rigid rotation v = Omega x r has curl v = 2 Omega and a linear density rho0 + g.r has
+    grad rho = g: the oracle's SPH estimates on a lattice interior reproduce both (to the
+    lattice's kernel-moment error), in direction and sign."""
+    x, y, z, dx, h0 = _lattice(20)
+    n = len(x)
+    h = np.full(n, h0)
+    om = np.array([0.3, -0.2, 1.0])
+    r = np.stack([x, y, z], axis=1) - 0.5
+    v = np.cross(om, r)
+    gvec = np.array([0.7, -0.4, 0.25])
+    rho = 1.0 + r @ gvec
+    m = rho * dx ** 3
+    st, nnl, noi = orc.nn_grid(x, y, z, h, 180)
+    assert st == 0
+    curl, grad = orc.sph_features(x, y, z, v[:, 0], v[:, 1], v[:, 2], m, h, rho, nnl, noi)
+    inner = np.all(np.abs(r) < 0.5 - 1.5 * h0, axis=1)
+    assert inner.sum() > 500
+    assert np.allclose(curl[inner], 2.0 * om, rtol=0, atol=0.03 * np.linalg.norm(2 * om))
+    assert np.allclose(grad[inner], gvec, rtol=0, atol=0.03 * np.linalg.norm(gvec))
+    # uniform state: both vanish by symmetry
+    curl0, grad0 = orc.sph_features(x, y, z, np.zeros(n), np.zeros(n), np.zeros(n), np.full(n, dx ** 3), h,
+                                    np.ones(n), nnl, noi)
+    assert np.abs(curl0).max() == 0.0
+    assert np.abs(grad0[inner]).max() < 1e-10
