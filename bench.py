@@ -232,7 +232,9 @@ def run_learned(args, lib, ics, result):
     n = len(s["x"])
     for k in ("ax", "ay", "az", "dedt", "g_ax", "g_ay", "g_az", "muijmax", "drhodt", "dsmldt"):
         s.setdefault(k, np.zeros(n))
-    p = ics.params_sedov()
+    # free boundary: bound h at 4 h0 so the corner particles (which can never reach the band)
+    # do not grow h without limit over the run
+    p = ics.params_sedov(sml_max=4.0 * float(s["sml"].max()))
     base = lib.RHS_SPH | lib.RHS_VARIABLE_SML
     train = dict(nsamples=65536, iters=args.learned_train_iters, lr=2e-3, seed=5)
     out = {}

@@ -85,6 +85,9 @@ typedef struct {
     int sml_avg;            /* kernel length (h_i + h_j)/2 (VARIABLE_SML || INTEGRATE_SML) */
     int precision;          /* MHPC_PREC_* for gravity and SPH pair sums */
     int cs_before_p;        /* 1: sound speed from the previous p, then p (miluphpc.cpp:2938-2951) */
+    double sml_max;         /* build extension: upper bound of h in the h-iteration; 0 = none (the
+                               reference).  On a free boundary a particle that cannot reach the
+                               band under d < min(h_i, h_j) otherwise grows h on every pass. */
 } mhpc_params;
 
 /* Field ids for upload / download / device pointers (reference Entry::Name, particles.cuh).

@@ -69,6 +69,7 @@ static int check_params(const mhpc_params *p) {
     if (p->max_nb < 1 || p->max_counts < 1 || p->max_counts > 64) return 0;
     if (p->n_target < 1 || p->band_lo > p->band_hi) return 0;
     if (p->precision != MHPC_PREC_FP64 && p->precision != MHPC_PREC_FP32_KAHAN) return 0;
+    if (!(p->sml_max >= 0.0)) return 0;
     return 1;
 }
 

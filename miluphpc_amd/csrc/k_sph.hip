@@ -52,6 +52,7 @@ struct NNArgs {
     const double *table;
     int tmax;
     int pass, max_counts, band_lo, band_hi;
+    double sml_max;           // upper bound of h (0: none)
     // compacted lists of the particles still iterating: pass p reads list_in / count_in
     // (per-particle mode) and appends the particles that stay unconverged to list_out
     const int32_t *list_in;
@@ -193,6 +194,7 @@ __device__ __forceinline__ bool h_update(const NNArgs &A, int N, double hi, doub
         int Nc = N < 1 ? 1 : N;
         Nc = Nc > A.tmax ? A.tmax : Nc;
         hn = hi * A.table[Nc];
+        if (A.sml_max > 0.0 && hn > A.sml_max) hn = A.sml_max;
         if (last) fin = true;
     }
     if (fin && (N >= A.max_nb || N == 0)) set_err(A.err, ERRF_CAP_NEIGHBOURS);
@@ -488,6 +490,7 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     a.max_counts = passes;
     a.band_lo = c->p.band_lo;
     a.band_hi = c->p.band_hi;
+    a.sml_max = c->p.sml_max;
     // pass 0, group mode; its list of unconverged particles (hlist[2]) is also the list
     // of particles whose h changed after pass 0
     a.pass = 0;

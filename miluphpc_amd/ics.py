@@ -124,11 +124,13 @@ def params_plummer_small() -> lib.Params:
                               mac=lib.MAC_CLASSIC, quadrupole=0, precision=lib.PREC_FP64)
 
 
-def params_sedov() -> lib.Params:
+def params_sedov(**overrides) -> lib.Params:
     """Ideal gas gamma 5/3, cubic spline, 64 neighbours (band 50-80), alpha 1, beta 2, FP64."""
-    return lib.default_params(eos=lib.EOS_IDEAL_GAS, gamma=5.0 / 3.0, alpha=1.0, beta=2.0,
-                              n_target=64, band_lo=50, band_hi=80, max_counts=7, max_nb=180,
-                              sml_avg=1, precision=lib.PREC_FP64, curve=lib.CURVE_LEBESGUE)
+    kw = dict(eos=lib.EOS_IDEAL_GAS, gamma=5.0 / 3.0, alpha=1.0, beta=2.0, n_target=64, band_lo=50,
+              band_hi=80, max_counts=7, max_nb=180, sml_avg=1, precision=lib.PREC_FP64,
+              curve=lib.CURVE_LEBESGUE)
+    kw.update(overrides)
+    return lib.default_params(**kw)
 
 
 def params_boss_bodenheimer() -> lib.Params:

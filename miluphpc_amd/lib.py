@@ -77,6 +77,7 @@ class Params(C.Structure):
         ("sml_avg", C.c_int),
         ("precision", C.c_int),
         ("cs_before_p", C.c_int),
+        ("sml_max", C.c_double),
     ]
 
     def as_dict(self) -> dict:

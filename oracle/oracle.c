@@ -755,7 +755,7 @@ void orc_h_factor_table(int n_target, int tmax, double *table) {
 
 int orc_h_iterate(int64_t n, const double *x, const double *y, const double *z, double *h,
                   int n_target, int band_lo, int band_hi, int max_counts, int max_nb,
-                  const double *table, int tmax, int32_t *iters, int32_t *noi) {
+                  const double *table, int tmax, double sml_max, int32_t *iters, int32_t *noi) {
     (void)n_target;
     if (n <= 0) return ORC_OK;
     double *hn = (double *)malloc(sizeof(double) * (size_t)n);
@@ -780,6 +780,7 @@ int orc_h_iterate(int64_t n, const double *x, const double *y, const double *z, 
             } else {
                 int Nc = N < 1 ? 1 : (N > tmax ? tmax : N);
                 hn[i] = h[i] * table[Nc];
+                if (sml_max > 0.0 && hn[i] > sml_max) hn[i] = sml_max;
                 if (pass == max_counts - 1) done[i] = 1;
                 active++;
             }

@@ -101,11 +101,12 @@ int orc_nn_refwalk(int64_t n, const double *x, const double *y, const double *z,
                    const double *ny, const double *nz, double diam, int max_nb,
                    int32_t *nnl, int32_t *noi);
 
-/* ---- a9: plain h-iteration, Jacobi form of sph.cu:739-961 ---- */
+/* ---- a9: plain h-iteration, Jacobi form of sph.cu:739-961; sml_max > 0 bounds h (a
+ * build extension; 0 is the reference) ---- */
 void orc_h_factor_table(int n_target, int tmax, double *table);
 int orc_h_iterate(int64_t n, const double *x, const double *y, const double *z, double *h,
                   int n_target, int band_lo, int band_hi, int max_counts, int max_nb,
-                  const double *table, int tmax, int32_t *iters, int32_t *noi);
+                  const double *table, int tmax, double sml_max, int32_t *iters, int32_t *noi);
 
 /* ---- a10-a14: SPH (FP64) ---- */
 void orc_cubic_spline(double r2, double dx, double dy, double dz, double h, double *W,

@@ -46,7 +46,7 @@ def _load() -> C.CDLL:
         "orc_nn_grid": (i32, [i64, vp, vp, vp, vp, i32, vp, vp]),
         "orc_nn_refwalk": (i32, [i64, vp, vp, vp, vp, vp, vp, vp, vp, d, i32, vp, vp]),
         "orc_h_factor_table": (None, [i32, i32, vp]),
-        "orc_h_iterate": (i32, [i64, vp, vp, vp, vp, i32, i32, i32, i32, i32, vp, i32, vp, vp]),
+        "orc_h_iterate": (i32, [i64, vp, vp, vp, vp, i32, i32, i32, i32, i32, vp, i32, d, vp, vp]),
         "orc_cubic_spline": (None, [d, d, d, d, d, vp, vp, vp]),
         "orc_density": (i32, [i64, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp]),
         "orc_soundspeed": (None, [i64, i32, d, d, vp, vp, vp]),
@@ -224,7 +224,7 @@ def h_factor_table(n_target: int, tmax: int = TMAX) -> np.ndarray:
     return t
 
 
-def h_iterate(x, y, z, h, n_target=64, band=(50, 80), max_counts=7, max_nb=180):
+def h_iterate(x, y, z, h, n_target=64, band=(50, 80), max_counts=7, max_nb=180, sml_max=0.0):
     x, y, z = f64(x), f64(y), f64(z)
     h = f64(h).copy()
     n = len(x)
@@ -232,7 +232,7 @@ def h_iterate(x, y, z, h, n_target=64, band=(50, 80), max_counts=7, max_nb=180):
     iters = np.empty(n, np.int32)
     noi = np.empty(n, np.int32)
     st = LIB.orc_h_iterate(n, P(x), P(y), P(z), P(h), int(n_target), int(band[0]), int(band[1]),
-                           int(max_counts), int(max_nb), P(table), TMAX, P(iters), P(noi))
+                           int(max_counts), int(max_nb), P(table), TMAX, float(sml_max), P(iters), P(noi))
     return st, h, iters, noi
 
 
@@ -358,12 +358,12 @@ def leapfrog_update_v(dt, s: dict) -> None:
 
 # ---- composite passes used by the tests and the host-core baseline ------------------
 def sph_rhs(s: dict, n_target=64, band=(50, 80), max_counts=7, max_nb=180, eos=9, gamma=5.0 / 3.0,
-            K=0.0, alpha=1.0, beta=2.0, variable_sml=True, cs_before_p=True):
+            K=0.0, alpha=1.0, beta=2.0, variable_sml=True, cs_before_p=True, sml_max=0.0):
     """h-iteration -> neighbours -> density -> sound speed -> pressure -> forces, FP64."""
     h = f64(s["sml"])
     out = {}
     if variable_sml:
-        st, h, iters, _ = h_iterate(s["x"], s["y"], s["z"], h, n_target, band, max_counts, max_nb)
+        st, h, iters, _ = h_iterate(s["x"], s["y"], s["z"], h, n_target, band, max_counts, max_nb, sml_max)
         out["h_status"] = st
         out["h_iters"] = iters
     st, nnl, noi = nn_grid(s["x"], s["y"], s["z"], h, max_nb)

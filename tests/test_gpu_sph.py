@@ -55,18 +55,22 @@ def test_neighbour_sets_bitexact(gpu_lib, orc, sedov24):
     assert sets(nnl, noi) == sets(onnl, onoi)
 
 
-def test_h_iteration_bitexact(gpu_lib, orc, sedov24):
-    print_case("h-iteration sedov 24^3 perturbed")
+@pytest.mark.parametrize("cap", [0.0, 1.3])
+def test_h_iteration_bitexact(gpu_lib, orc, sedov24, cap):
+    print_case(f"h-iteration sedov 24^3 perturbed, sml_max {cap} x max h0")
     s = sedov24
-    p = gpu_lib.default_params(curve=0)
     h0 = s["sml"] * 1.25
+    sml_max = cap * h0.max()
+    p = gpu_lib.default_params(curve=0, sml_max=sml_max)
     with make_ctx(gpu_lib, {**s, "sml": h0}, p) as ctx:
         ctx.bbox(); ctx.keys(); ctx.sort(); ctx.build_tree()
         ctx.h_iterate()
         h = ctx.download("sml")
         iters = ctx.download("h_iters")
         noi = ctx.download("noi")
-    st, oh, oiters, onoi = orc.h_iterate(s["x"], s["y"], s["z"], h0)
+    st, oh, oiters, onoi = orc.h_iterate(s["x"], s["y"], s["z"], h0, sml_max=sml_max)
+    if cap > 0:
+        assert h.max() == sml_max
     assert st == 0
     assert np.array_equal(iters, oiters)
     assert np.array_equal(noi, onoi)

@@ -202,6 +202,25 @@ def test_h_iteration_uniform_lattice_converges_to_one_h(orc):
     assert np.all(noi[inner] == noi[inner][0])
 
 
+def test_h_iteration_sml_max_bounds_free_boundary(orc):
+    """On a free lattice boundary the corner particles cannot reach the band under
+    d < min(h_i, h_j): without a bound their h grows on every pass; with sml_max it stops
+    exactly at the bound, and particles that converge below the bound are unchanged."""
+    s = ics.sedov_lattice(12)
+    h0 = s["sml"].copy()
+    st, h_free, it_free, _ = orc.h_iterate(s["x"], s["y"], s["z"], h0)
+    hmax = 1.5 * h0.max()
+    st, h_cap, it_cap, _ = orc.h_iterate(s["x"], s["y"], s["z"], h0, sml_max=hmax)
+    corner = np.argmin(s["x"] + s["y"] + s["z"])
+    assert h_free[corner] > 2.0 * hmax            # unbounded growth without the cap
+    assert h_cap[corner] == hmax                   # stops exactly at the bound
+    assert h_cap.max() == hmax
+    below = h_free < hmax
+    # particles whose own and neighbours' iterations stayed below the bound: same h
+    assert np.mean(h_cap[below] == h_free[below]) > 0.9
+    assert np.all(it_cap[below & (it_free == 1)] == 1)
+
+
 def test_leapfrog_constant_acceleration(orc):
     n = 10
     s = {k: np.zeros(n) for k in ("x", "y", "z", "vx", "vy", "vz", "g_ax", "g_ay", "g_az",
