@@ -183,6 +183,13 @@ def run_sedov(args, lib, ics, torch, result):
     with lib.Context(n, p) as ctx:
         ctx.load(**{k: v for k, v in s.items() if k in lib.FIELD_ID})
         wall, stats = time_rhs(ctx, lib, flags, args.steps, args.warmup, "sedov", restore={"sml": h0})
+        # the same RHS with speculative lists (same h and neighbour sets, one walk fewer)
+        wall_s, stats_s = time_rhs(ctx, lib, flags | lib.RHS_SPECULATIVE_NN, args.steps, 1, "sedov-speculative",
+                                   restore={"sml": h0})
+    med_s = median_stats(stats_s)
+    result["sedov_speculative"] = {
+        "value": n / wall_s, "ms_per_step": wall_s * 1e3,
+        "phases_ms": {k[3:]: round(v, 4) for k, v in med_s.items() if k.startswith("ms_")}}
     med = median_stats(stats)
     nnb = med["n_neighbours"] / n
     bpp = sph_bytes_per_particle(nnb)

@@ -9,8 +9,10 @@
 //     itself is parallel across lanes;
 //   * leaves and accepted cells go to two LDS interaction lists shared by the wave;
 //   * each time a list holds 64 entries all 64 lanes evaluate it, every lane reading
-//     the same LDS entry (broadcast) for its own particle: FP32 pair math on FP64
-//     position differences, Kahan-compensated sums (or plain FP64 for the FP64 mode).
+//     the same LDS entry (broadcast) for its own particle.  FP32 mode: entries are
+//     staged as FP32 positions relative to the group's centre (each rounded once from the
+//     FP64 difference), each batch of up to 64 entries is summed in FP32 and the batch
+//     sums are added with Kahan compensation.  FP64 mode: plain FP64 throughout.
 // The stack lives in LDS; the number of nodes popped per round shrinks as the stack
 // fills so it cannot overflow (a hard bound still sets ERRF_STACK).
 #include "mhpc_internal.h"
@@ -20,7 +22,8 @@
 #define BH_STACK_PHYS (BH_STACK + 192)   // hard capacity
 #define BH_LIST 128
 
-struct BHWaveLds {
+// FP64 mode: list entries keep FP64 positions
+struct BHWaveLds64 {
     int stack[BH_STACK_PHYS];
     // particle list
     double px[BH_LIST], py[BH_LIST], pz[BH_LIST];
@@ -32,6 +35,24 @@ struct BHWaveLds {
     float4 q0[BH_LIST];
     float2 q1[BH_LIST];
 };
+
+// FP32 mode: list entries are FP32 {x, y, z, m} relative to the group's centre (rounded
+// once per entry from the FP64 difference, instead of once per lane and interaction)
+struct BHWaveLds32 {
+    int stack[BH_STACK_PHYS];
+    float4 p[BH_LIST];
+    int pid[BH_LIST];
+    float4 c[BH_LIST];
+    float4 q0[BH_LIST];
+    float2 q1[BH_LIST];
+};
+
+template <bool FP64>
+struct BHLds;
+template <>
+struct BHLds<true> { typedef BHWaveLds64 type; };
+template <>
+struct BHLds<false> { typedef BHWaveLds32 type; };
 
 template <bool FP64>
 struct BHAcc;
@@ -46,9 +67,70 @@ struct BHAcc<true> {
     Plain64 x, y, z, phi;
 };
 
-// particle-particle (monopole) batch
+// FP32 batches: plain FP32 sums over the (at most 64) entries of one batch, then one
+// compensated (Kahan) addition of each batch sum into the running total
+template <bool POT>
+__device__ __forceinline__ void eval_particles32(const BHWaveLds32 &L, int cnt, float px, float py, float pz,
+                                                 int self, float eps2, BHAcc<false> &acc) {
+    float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
+    for (int e = 0; e < cnt; e++) {
+        const float4 q = L.p[e];
+        const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+        const float rinv = r2 > 0.f ? rsqrtf(r2) : 0.f;
+        const float mr = q.w * rinv;
+        const float mr3 = mr * rinv * rinv;
+        bx = __builtin_fmaf(mr3, dx, bx);
+        by = __builtin_fmaf(mr3, dy, by);
+        bz = __builtin_fmaf(mr3, dz, bz);
+        if (POT && L.pid[e] != self) bp -= mr;
+    }
+    acc.x.add(bx);
+    acc.y.add(by);
+    acc.z.add(bz);
+    if (POT) acc.phi.add(bp);
+}
+
+template <bool POT, bool QUAD>
+__device__ __forceinline__ void eval_cells32(const BHWaveLds32 &L, int cnt, float px, float py, float pz,
+                                             float eps2, BHAcc<false> &acc) {
+    float bx = 0.f, by = 0.f, bz = 0.f, bp = 0.f;
+    for (int e = 0; e < cnt; e++) {
+        const float4 q = L.c[e];
+        const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+        const float rinv = r2 > 0.f ? rsqrtf(r2) : 0.f;
+        const float rinv2 = rinv * rinv;
+        const float mr = q.w * rinv;
+        const float mr3 = mr * rinv2;
+        bx = __builtin_fmaf(mr3, dx, bx);
+        by = __builtin_fmaf(mr3, dy, by);
+        bz = __builtin_fmaf(mr3, dz, bz);
+        if (POT) bp -= mr;
+        if (QUAD) {
+            const float4 a = L.q0[e];
+            const float2 b = L.q1[e];
+            const float qdx = __builtin_fmaf(a.x, dx, __builtin_fmaf(a.y, dy, a.z * dz));
+            const float qdy = __builtin_fmaf(a.y, dx, __builtin_fmaf(a.w, dy, b.x * dz));
+            const float qdz = __builtin_fmaf(a.z, dx, __builtin_fmaf(b.x, dy, b.y * dz));
+            const float dqd = __builtin_fmaf(dx, qdx, __builtin_fmaf(dy, qdy, dz * qdz));
+            const float rinv5 = rinv2 * rinv2 * rinv;
+            const float t = 2.5f * dqd * rinv5 * rinv2;
+            bx = __builtin_fmaf(t, dx, __builtin_fmaf(-qdx, rinv5, bx));
+            by = __builtin_fmaf(t, dy, __builtin_fmaf(-qdy, rinv5, by));
+            bz = __builtin_fmaf(t, dz, __builtin_fmaf(-qdz, rinv5, bz));
+            if (POT) bp = __builtin_fmaf(-0.5f * dqd, rinv5, bp);
+        }
+    }
+    acc.x.add(bx);
+    acc.y.add(by);
+    acc.z.add(bz);
+    if (POT) acc.phi.add(bp);
+}
+
+// particle-particle (monopole) batch, FP64 mode
 template <bool FP64, bool POT>
-__device__ __forceinline__ void eval_particles(const BHWaveLds &L, int cnt, double px, double py,
+__device__ __forceinline__ void eval_particles(const BHWaveLds64 &L, int cnt, double px, double py,
                                                double pz, int self, double eps2, BHAcc<FP64> &acc) {
     for (int e = 0; e < cnt; e++) {
         if constexpr (FP64) {
@@ -77,7 +159,7 @@ __device__ __forceinline__ void eval_particles(const BHWaveLds &L, int cnt, doub
 
 // particle-cell batch: monopole (+ traceless quadrupole)
 template <bool FP64, bool POT, bool QUAD>
-__device__ __forceinline__ void eval_cells(const BHWaveLds &L, int cnt, double px, double py,
+__device__ __forceinline__ void eval_cells(const BHWaveLds64 &L, int cnt, double px, double py,
                                            double pz, double eps2, BHAcc<FP64> &acc) {
     for (int e = 0; e < cnt; e++) {
         if constexpr (FP64) {
@@ -158,9 +240,10 @@ struct BHArgs {
 
 template <bool FP64, bool POT, bool QUAD>
 __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
-    __shared__ BHWaveLds lds_all[BH_WAVES];
+    typedef typename BHLds<FP64>::type Lds;
+    __shared__ Lds lds_all[BH_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
-    BHWaveLds &L = lds_all[w];
+    Lds &L = lds_all[w];
     const int64_t group = xcd_block(blockIdx.x, gridDim.x) * BH_WAVES + w;
     const int64_t k = group * 64 + lane;
     const int64_t n = A.n;
@@ -179,6 +262,10 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
     const double dp_1 = R * R / A.theta;
     const double theta2 = A.theta * A.theta;
     const double eps2 = A.eps2;
+    // FP32 mode: positions relative to the group's centre
+    const double refx = 0.5 * (glo_x + ghi_x), refy = 0.5 * (glo_y + ghi_y), refz = 0.5 * (glo_z + ghi_z);
+    const float pxr = (float)(px - refx), pyr = (float)(py - refy), pzr = (float)(pz - refz);
+    const float eps2f = (float)eps2;
 
     BHAcc<FP64> acc;
     int top = 1, pl_n = 0, cl_n = 0;
@@ -224,18 +311,27 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
         if (isPart) {
             const int pos = pl_n + popc64(mP & lt);
             const double4 r = A.sxm[node];
-            L.px[pos] = r.x;
-            L.py[pos] = r.y;
-            L.pz[pos] = r.z;
-            L.pm[pos] = r.w;
+            if constexpr (FP64) {
+                L.px[pos] = r.x;
+                L.py[pos] = r.y;
+                L.pz[pos] = r.z;
+                L.pm[pos] = r.w;
+            } else {
+                L.p[pos] = make_float4((float)(r.x - refx), (float)(r.y - refy), (float)(r.z - refz), (float)r.w);
+            }
             L.pid[pos] = node;
         }
         if (accept) {
             const int pos = cl_n + popc64(mA & lt);
-            L.cx[pos] = ccx;
-            L.cy[pos] = ccy;
-            L.cz[pos] = ccz;
-            L.cm[pos] = A.cmass[c];
+            if constexpr (FP64) {
+                L.cx[pos] = ccx;
+                L.cy[pos] = ccy;
+                L.cz[pos] = ccz;
+                L.cm[pos] = A.cmass[c];
+            } else {
+                L.c[pos] = make_float4((float)(ccx - refx), (float)(ccy - refy), (float)(ccz - refz),
+                                       (float)A.cmass[c]);
+            }
             if (QUAD) {
                 const float4 a = A.cquadf[2 * c], b = A.cquadf[2 * c + 1];
                 L.q0[pos] = a;
@@ -272,33 +368,42 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
         top += tot;
         wave_sync();
         if (pl_n >= 64) {
-            eval_particles<FP64, POT>(L, 64, px, py, pz, (int)k, eps2, acc);
+            if constexpr (FP64) eval_particles<FP64, POT>(L, 64, px, py, pz, (int)k, eps2, acc);
+            else eval_particles32<POT>(L, 64, pxr, pyr, pzr, (int)k, eps2f, acc);
             wave_sync();
             const int rest = pl_n - 64;
             if (lane < rest) {
-                const double a = L.px[64 + lane], b = L.py[64 + lane], d = L.pz[64 + lane];
-                const double m = L.pm[64 + lane];
+                if constexpr (FP64) {
+                    const double a = L.px[64 + lane], b = L.py[64 + lane], d = L.pz[64 + lane];
+                    const double m = L.pm[64 + lane];
+                    L.px[lane] = a; L.py[lane] = b; L.pz[lane] = d; L.pm[lane] = m;
+                } else {
+                    const float4 a = L.p[64 + lane];
+                    L.p[lane] = a;
+                }
                 const int id = L.pid[64 + lane];
-                L.px[lane] = a; L.py[lane] = b; L.pz[lane] = d; L.pm[lane] = m; L.pid[lane] = id;
+                L.pid[lane] = id;
             }
             pl_n = rest;
             wave_sync();
         }
         if (cl_n >= 64) {
-            eval_cells<FP64, POT, QUAD>(L, 64, px, py, pz, eps2, acc);
+            if constexpr (FP64) eval_cells<FP64, POT, QUAD>(L, 64, px, py, pz, eps2, acc);
+            else eval_cells32<POT, QUAD>(L, 64, pxr, pyr, pzr, eps2f, acc);
             wave_sync();
             const int rest = cl_n - 64;
             if (lane < rest) {
-                const double a = L.cx[64 + lane], b = L.cy[64 + lane], d = L.cz[64 + lane];
-                const double m = L.cm[64 + lane];
-                float4 q0 = make_float4(0, 0, 0, 0);
-                float2 q1 = make_float2(0, 0);
-                if (QUAD) {
-                    q0 = L.q0[64 + lane];
-                    q1 = L.q1[64 + lane];
+                if constexpr (FP64) {
+                    const double a = L.cx[64 + lane], b = L.cy[64 + lane], d = L.cz[64 + lane];
+                    const double m = L.cm[64 + lane];
+                    L.cx[lane] = a; L.cy[lane] = b; L.cz[lane] = d; L.cm[lane] = m;
+                } else {
+                    const float4 a = L.c[64 + lane];
+                    L.c[lane] = a;
                 }
-                L.cx[lane] = a; L.cy[lane] = b; L.cz[lane] = d; L.cm[lane] = m;
                 if (QUAD) {
+                    const float4 q0 = L.q0[64 + lane];
+                    const float2 q1 = L.q1[64 + lane];
                     L.q0[lane] = q0;
                     L.q1[lane] = q1;
                 }
@@ -311,8 +416,13 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
         if (lane == 0) set_err(A.err, ERRF_STACK);
         return;
     }
-    if (pl_n > 0) eval_particles<FP64, POT>(L, pl_n, px, py, pz, (int)k, eps2, acc);
-    if (cl_n > 0) eval_cells<FP64, POT, QUAD>(L, cl_n, px, py, pz, eps2, acc);
+    if constexpr (FP64) {
+        if (pl_n > 0) eval_particles<FP64, POT>(L, pl_n, px, py, pz, (int)k, eps2, acc);
+        if (cl_n > 0) eval_cells<FP64, POT, QUAD>(L, cl_n, px, py, pz, eps2, acc);
+    } else {
+        if (pl_n > 0) eval_particles32<POT>(L, pl_n, pxr, pyr, pzr, (int)k, eps2f, acc);
+        if (cl_n > 0) eval_cells32<POT, QUAD>(L, cl_n, pxr, pyr, pzr, eps2f, acc);
+    }
     if (valid) {
         const int32_t i = A.perm[k];
         A.gax[i] = A.G * acc.x.value();

@@ -46,7 +46,7 @@ static_assert(MLP_NPARAM == MHPC_MLP_NPARAM, "parameter layout of include/mhpc.h
 int mlp_num_params() { return MLP_NPARAM; }
 
 // weights as the kernels stage them: W1 [64][16] bf16 (cols >= 8 zero), W2 [64][64] bf16
-struct MlpLds {
+struct alignas(16) MlpLds {
     unsigned short w1[MLP_H][MLP_KPAD];
     unsigned short w2[MLP_H][MLP_H];
     float b1[MLP_H], b2[MLP_H], w3[MLP_H];
@@ -68,11 +68,19 @@ __device__ void load_weights(MlpLds &W, const float *__restrict__ p) {
 }
 
 // per-wave activations
-struct MlpWave {
+struct alignas(16) MlpWave {
     unsigned short x[64][MLP_KPAD];
     unsigned short a1[64][MLP_H];
     unsigned short a2[64][MLP_H];
 };
+
+// tanh(x) = 1 - 2 / (exp(2x) + 1) from the hardware exp2 and reciprocal estimates: absolute
+// error ~1e-7, far below the BF16 rounding the activations are stored with; saturates to
+// +-1 for large |x|
+__device__ __forceinline__ float tanh_fast(float x) {
+    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);   // exp(2x) = 2^(2x log2 e)
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+}
 
 // Z(64 x 64) = act(Ain(64 x K) Wt(K x 64) + b); Ain row-major [sample][k], W row-major
 // [out][k] (so B[k][n] = W[n][k]); writes tanh(Z) as BF16 into Aout.
@@ -86,20 +94,17 @@ __device__ __forceinline__ void dense_tanh(const unsigned short (*Ain)[LDA],
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kt = 0; kt < K / 16; kt++) {
-                bf16x4 a, b;
+                // four consecutive k of one row: one 8-byte LDS read per operand
                 const int k0 = kt * 16 + 4 * q;
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    a[u] = (short)Ain[mt * 16 + r16][k0 + u];
-                    b[u] = (short)Wt[nt * 16 + r16][k0 + u];
-                }
+                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(&Ain[mt * 16 + r16][k0]);
+                const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&Wt[nt * 16 + r16][k0]);
                 acc = mfma16(a, b, acc);
             }
             const int n = nt * 16 + r16;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int m = mt * 16 + 4 * q + r;
-                Aout[m][n] = f2bf(tanhf(acc[r] + bias[n]));
+                Aout[m][n] = f2bf(tanh_fast(acc[r] + bias[n]));
             }
         }
     }

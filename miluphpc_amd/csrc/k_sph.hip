@@ -18,6 +18,8 @@
 #define NN_STACK_PHYS (NN_STACK + 192)
 #define NN_RANGES 128
 #define NN_LEAF 16
+#define NN_OUTLIER 2.0       // group-mode outlier: h above this factor x the group's smallest
+#define NN_OUTLIER_MAX 16    // at most this many outliers per group are split off
 
 struct NNWaveLds {
     int stack[NN_STACK_PHYS];
@@ -62,7 +64,10 @@ struct NNArgs {
     double *h_slot;           // per-particle mode: new h of list slot t
     const uint8_t *only;      // list mode: when set, only particles with only[k] != 0
     uint8_t *dirty;           // dirty marking: particles whose list must be rebuilt
-    const double *hmax_p;     // dirty marking: largest h of all particles
+    const double *hmax_p;     // dirty marking: largest pass-0 h
+    unsigned long long *hmax0_bits;   // pass 0 with lists: largest h (bit pattern, atomicMax)
+    int32_t *out_list;        // group mode: outlier particles deferred to the per-particle mode
+    uint32_t *out_count;
     uint32_t *err;
     unsigned long long *counters;
 };
@@ -221,7 +226,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     bool active = valid;
     if (MODE >= 1 && valid) active = A.done[k] == 0;
     if (MODE == 0 && valid && A.only) active = A.only[k] != 0;
-    const uint64_t mActive = wave_ballot(active);
+    uint64_t mActive = wave_ballot(active);
     if (mActive == 0) {
         if (MODE >= 1 && valid) A.h_next[k] = A.h[k];
         return;
@@ -230,6 +235,23 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     const double xi = own.x, yi = own.y, zi = own.z;
     const double hi = valid ? A.h[k] : 0.0;
     const double hi2 = hi * hi;
+    // outliers: a few active particles with h above NN_OUTLIER x the group's smallest would
+    // grow the search box of all 64; they go to the per-particle mode instead
+    bool outlier = false;
+    if (A.out_list) {
+        const double hmin = wave_min(active ? hi : 1e300);
+        const bool big = active && hi > NN_OUTLIER * hmin;
+        const uint64_t mO = wave_ballot(big);
+        if (mO != 0 && popc64(mO) <= NN_OUTLIER_MAX) {
+            outlier = big;
+            uint32_t b = 0;
+            if (lane == 0) b = atomicAdd(A.out_count, (uint32_t)popc64(mO));
+            b = __shfl(b, 0);
+            if (outlier) A.out_list[b + popc64(mO & lanemask_lt())] = (int32_t)k;
+            active = active && !outlier;
+            mActive = wave_ballot(active);
+        }
+    }
     // search box: bounding box of the active lanes grown by their largest h (plus a
     // relative margin far above FP64 rounding, so pruning never drops a true pair)
     const double hmax = wave_max(active ? hi : 0.0);
@@ -242,6 +264,16 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     int b0 = 0, b1 = 0, b2 = 0;   // pending list entries, flushed four at a time as one int4
     const bool ok = nn_walk_box(L, A, lox, hix, loy, hiy, loz, hiz, [&](int m) {
         if (!active) return;
+        if (MODE == 1) {
+            for (int e = 0; e < m; e++) {
+                const double dx = xi - L.tx[e];
+                const double dy = yi - L.ty[e];
+                const double dz = zi - L.tz[e];
+                const double d = dx * dx + dy * dy + dz * dz;
+                count += (d < hi2 && d < L.th2[e] && L.tj[e] != (int)k) ? 1 : 0;
+            }
+            return;
+        }
         for (int e = 0; e < m; e++) {
             const double dx = xi - L.tx[e];
             const double dy = yi - L.ty[e];
@@ -249,18 +281,16 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             const double d = dx * dx + dy * dy + dz * dz;
             const int j = L.tj[e];
             if (d < hi2 && d < L.th2[e] && j != (int)k) {
-                if (MODE != 1) {
-                    const int slot = count & 3;
-                    if (slot == 3) {
-                        if (count < A.max_nb)
-                            A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, j);
-                    } else if (slot == 0) {
-                        b0 = j;
-                    } else if (slot == 1) {
-                        b1 = j;
-                    } else {
-                        b2 = j;
-                    }
+                const int slot = count & 3;
+                if (slot == 3) {
+                    if (count < A.max_nb)
+                        A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, j);
+                } else if (slot == 0) {
+                    b0 = j;
+                } else if (slot == 1) {
+                    b1 = j;
+                } else {
+                    b2 = j;
                 }
                 count++;
             }
@@ -282,9 +312,10 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             if (count > A.max_nb) set_err(A.err, ERRF_CAP_NEIGHBOURS);
         }
     } else if (MODE == 0) {
-        if (valid && (count & 3) != 0 && count < A.max_nb)
+        const bool own_list = valid && !outlier;
+        if (own_list && (count & 3) != 0 && count < A.max_nb)
             A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, 0);
-        if (valid) {
+        if (own_list) {
             A.snoi[k] = count;
             const int32_t i = A.perm[k];
             A.noi_out[i] = count;
@@ -292,15 +323,15 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             if (A.iters_out) A.iters_out[i] = A.siters[k];
             if (count > A.max_nb) set_err(A.err, ERRF_CAP_NEIGHBOURS);
         }
-        unsigned long long s = (unsigned long long)wave_sum(valid ? count : 0);
-        int mx = wave_max(valid ? count : 0);
+        unsigned long long s = (unsigned long long)wave_sum(own_list ? count : 0);
+        int mx = wave_max(own_list ? count : 0);
         if (lane == 0) {
             atomicAdd(&A.counters[CNT_NEIGHBOURS], s);
             atomicMax(&A.counters[CNT_MAX_NOI], (unsigned long long)mx);
         }
     } else {
         bool stay = false;
-        if (valid) {
+        if (valid && !outlier) {
             if (active) {
                 A.iters[k] = A.pass + 1;
                 A.snoi[k] = count;
@@ -312,6 +343,12 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             } else {
                 A.h_next[k] = hi;
             }
+        }
+        if (MODE == 2) {
+            // largest pass-0 h: the search radius of the dirty marking (a particle whose list
+            // from this pass stays in use keeps this h)
+            const double wmax = wave_max(valid ? hi : 0.0);
+            if (lane == 0) atomicMax(A.hmax0_bits, (unsigned long long)__double_as_longlong(wmax));
         }
         if (A.list_out) {
             // append the unconverged particles (wave-aggregated; list order is free, the
@@ -326,19 +363,24 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     }
 }
 
-// Per-particle mode of an h-iteration pass (passes after the first): one wavefront per
-// listed particle, searching the particle's own box x +- h with the 64 lanes testing 64
-// candidates at a time.  Same criterion and update as the group mode, so the counts and
-// the new h are identical; only the unconverged particles are visited.  The new h goes
-// to h_slot[t] and k_h_apply writes it after the pass (Jacobi: every count of pass p
-// sees the h of pass p).
+// Per-particle mode: one wavefront per listed particle, searching the particle's own box
+// x +- h with the 64 lanes testing 64 candidates at a time.  Used for the h-iteration
+// passes after the first (only the unconverged particles are visited) and for the
+// outliers of the group mode (particles whose h is far above the rest of their group).
+// MODE 1: count and h update (identical to the group mode's; with h_slot set the new h
+// goes to h_slot[t] and k_h_apply writes it after the pass, else to h_next[k]);
+// MODE 2: MODE 1 plus the list; MODE 0: the list only.  Lists are written in candidate
+// order, one 4-byte entry per lane.
 #define NN1_WAVES 4
-__global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one_pass(NNArgs A) {
+template <int MODE>
+__global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
     NNWaveLds &L = lds_all[w];
     const uint32_t cnt = *A.count_in;
     const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
+    const uint64_t lt = lanemask_lt();
+    int32_t *nbr32 = reinterpret_cast<int32_t *>(A.nbr4);
     for (int64_t t = (int64_t)blockIdx.x * NN1_WAVES + w; t < cnt; t += nwaves) {
         const int64_t k = A.list_in[t];
         const double4 own = A.sxm[k];
@@ -356,21 +398,39 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one_pass(NNArgs A) {
                 const double d = dx * dx + dy * dy + dz * dz;
                 hit = d < hi2 && d < L.th2[lane] && L.tj[lane] != (int)k;
             }
-            count += popc64(wave_ballot(hit));
+            const uint64_t mh = wave_ballot(hit);
+            if (MODE != 1 && hit) {
+                const int s = count + popc64(mh & lt);
+                if (s < A.max_nb) nbr32[((int64_t)(s >> 2) * A.stride + k) * 4 + (s & 3)] = L.tj[lane];
+            }
+            count += popc64(mh);
         });
         if (!ok) {
             if (lane == 0) set_err(A.err, ERRF_STACK);
             return;
         }
-        double hn;
-        const bool fin = h_update(A, count, hi, hn);
-        if (lane == 0) {
-            A.iters[k] = A.pass + 1;
-            A.snoi[k] = count;
-            A.h_slot[t] = hn;
-            if (fin) A.done[k] = 1;
-            else A.list_out[atomicAdd(A.count_out, 1u)] = (int32_t)k;
-            atomicAdd(&A.counters[CNT_H_COUNTS], 1ull);
+        if (MODE == 0) {
+            if (lane == 0) {
+                A.snoi[k] = count;
+                if (A.noi_out) A.noi_out[A.perm[k]] = count;
+                if (count > A.max_nb) set_err(A.err, ERRF_CAP_NEIGHBOURS);
+                if (A.counters) {
+                    atomicAdd(&A.counters[CNT_NEIGHBOURS], (unsigned long long)count);
+                    atomicMax(&A.counters[CNT_MAX_NOI], (unsigned long long)count);
+                }
+            }
+        } else {
+            double hn;
+            const bool fin = h_update(A, count, hi, hn);
+            if (lane == 0) {
+                A.iters[k] = A.pass + 1;
+                A.snoi[k] = count;
+                if (A.h_slot) A.h_slot[t] = hn;
+                else A.h_next[k] = hn;
+                if (fin) A.done[k] = 1;
+                else A.list_out[atomicAdd(A.count_out, 1u)] = (int32_t)k;
+                atomicAdd(&A.counters[CNT_H_COUNTS], 1ull);
+            }
         }
         wave_sync();
     }
@@ -378,8 +438,9 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one_pass(NNArgs A) {
 
 // Dirty marking after a speculative pass 0: for every particle j whose h changed after
 // pass 0 (list_in), flag each particle i with d_ij < h_i (the only pairs whose membership
-// d < h_i and d < h_j can differ from pass 0), and j itself.  One wavefront per changed
-// particle, searching x_j +- max h; lanes are candidates.
+// d < h_i and d < h_j can differ from pass 0), and j itself.  An unflagged particle keeps
+// its pass-0 h, so searching x_j +- (largest pass-0 h) finds every such i.  One wavefront
+// per changed particle; lanes are candidates.
 __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
@@ -408,16 +469,6 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
         }
         wave_sync();
     }
-}
-
-// largest h (positive doubles order like their bit patterns)
-__global__ __launch_bounds__(256) void k_hmax(int64_t n, const double *__restrict__ h,
-                                              unsigned long long *__restrict__ out) {
-    double m = 0.0;
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-        m = fmax(m, h[k]);
-    m = wave_max(m);
-    if (wave_lane() == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
 // caller-order counts and the neighbour statistics from the sorted counts
@@ -465,13 +516,15 @@ static NNArgs nn_common(mhpc_ctx *c) {
 static unsigned nn_grid(mhpc_ctx *c) { return (unsigned)ceil_div(ceil_div(c->n, 64), NN_WAVES); }
 
 // done flags and iteration counts to zero; the per-pass list counters to zero
-__global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *list_counts, int nlc) {
+__global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *list_counts, int nlc,
+                         unsigned long long *hmax0_bits) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k < n) {
         done[k] = 0;
         iters[k] = 0;
     }
     if (k < nlc) list_counts[k] = 0;
+    if (k == 0) *hmax0_bits = 0ull;
 }
 
 // a9: the plain h-iteration as Jacobi passes; sh holds the final h afterwards.  Pass 0
@@ -480,7 +533,7 @@ __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *lis
 void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_h_init, dim3((unsigned)ceil_div(c->n > 64 ? c->n : 64, 256)), dim3(256), 0, c->stream,
-                       c->n, c->sdone, c->siters, c->hlist_count, MHPC_MAX_PASSES + 1);
+                       c->n, c->sdone, c->siters, c->hlist_count, MHPC_MAX_PASSES + 1, c->hmax_bits);
     NNArgs a = nn_common(c);
     a.done = c->sdone;
     a.iters = c->siters;
@@ -498,17 +551,32 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     a.h_next = c->sh2;
     a.list_out = c->hlist[2];
     a.count_out = c->hlist_count + 0;
+    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), 2048);
+    launch_fill_u32(c, c->ocount, 0u, 1);
+    a.out_list = c->olist;
+    a.out_count = c->ocount;
     if (speculative) {
         a.nbr4 = reinterpret_cast<int4 *>(c->nbr);
         a.stride = c->cap_pad;
+        a.hmax0_bits = c->hmax_bits;
         hipLaunchKernelGGL(k_nn_walk<2>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
-        a.nbr4 = nullptr;
     } else {
         hipLaunchKernelGGL(k_nn_walk<1>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
     }
+    // the group mode's outliers, same pass, per-particle mode
+    {
+        NNArgs o = a;
+        o.out_list = nullptr;
+        o.list_in = c->olist;
+        o.count_in = c->ocount;
+        o.h_slot = nullptr;
+        if (speculative) hipLaunchKernelGGL(k_nn_one<2>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, o);
+        else hipLaunchKernelGGL(k_nn_one<1>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, o);
+    }
+    a.nbr4 = nullptr;
+    a.out_list = nullptr;
     std::swap(c->sh, c->sh2);
     // passes 1.., per-particle mode over the list of the previous pass
-    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), 2048);
     for (int pass = 1; pass < passes; pass++) {
         a.pass = pass;
         a.h = c->sh;
@@ -517,7 +585,7 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
         a.list_out = c->hlist[pass & 1];
         a.count_out = c->hlist_count + pass;
         a.h_slot = c->hslot;
-        hipLaunchKernelGGL(k_nn_one_pass, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
+        hipLaunchKernelGGL(k_nn_one<1>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
         hipLaunchKernelGGL(k_h_apply, dim3(256), dim3(256), 0, c->stream, a.list_in, a.count_in, c->hslot, c->sh);
     }
 }
@@ -529,8 +597,6 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
 void launch_neighbours_fixup(mhpc_ctx *c) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->sdirty, c->n);
-    launch_fill_u64(c, c->hmax_bits, 0ull, 1);
-    hipLaunchKernelGGL(k_hmax, dim3(512), dim3(256), 0, c->stream, c->n, c->sh, c->hmax_bits);
     NNArgs a = nn_common(c);
     a.h = c->sh;
     a.list_in = c->hlist[2];
@@ -545,7 +611,16 @@ void launch_neighbours_fixup(mhpc_ctx *c) {
     b.stride = c->cap_pad;
     b.snoi = c->snoi;
     b.only = c->sdirty;
+    launch_fill_u32(c, c->ocount, 0u, 1);
+    b.out_list = c->olist;
+    b.out_count = c->ocount;
     hipLaunchKernelGGL(k_nn_walk<0>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, b);
+    b.out_list = nullptr;
+    b.list_in = c->olist;
+    b.count_in = c->ocount;
+    b.counters = nullptr;   // k_publish_noi counts every particle
+    b.noi_out = nullptr;
+    hipLaunchKernelGGL(k_nn_one<0>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, b);
     hipLaunchKernelGGL(k_publish_noi, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->n, c->perm,
                        c->snoi, c->noi, c->counters);
 }
@@ -561,7 +636,15 @@ void launch_neighbours(mhpc_ctx *c) {
     a.sml_out = nullptr;
     a.siters = c->siters;
     a.iters_out = nullptr;
+    launch_fill_u32(c, c->ocount, 0u, 1);
+    a.out_list = c->olist;
+    a.out_count = c->ocount;
     hipLaunchKernelGGL(k_nn_walk<0>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
+    a.out_list = nullptr;
+    a.list_in = c->olist;
+    a.count_in = c->ocount;
+    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), 2048);
+    hipLaunchKernelGGL(k_nn_one<0>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
 }
 
 // ------------------------------------------------------------------------------------
@@ -592,6 +675,26 @@ __device__ __forceinline__ void cubic32(float r, float h, float &W, float &dWdr)
 // wave keeps several independent gathers in flight.
 #define K8PI 2.5464790894703255   // 8 / pi
 
+// 1/x and 1/sqrt(x) from the hardware estimates (v_rcp_f64, v_rsq_f64) refined by two
+// Newton-Raphson steps each, for positive normal x: a few FP64 ulp, several instructions
+// fewer than a correctly rounded division / square root
+__device__ __forceinline__ double rcp_nr(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+__device__ __forceinline__ double rsq_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    double e = fma(-hx, y * y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-hx, y * y, 0.5);
+    return fma(y, e, y);
+}
+
 template <bool FP64>
 __global__ __launch_bounds__(256) void k_density(int64_t n, const double4 *__restrict__ sxm,
                                                  const double *__restrict__ sh,
@@ -609,27 +712,36 @@ __global__ __launch_bounds__(256) void k_density(int64_t n, const double4 *__res
     double r;
     const int4 *nbr4 = reinterpret_cast<const int4 *>(nbr);
     if constexpr (FP64) {
+        // the next neighbour's {x,y,z,m} and h are loaded while the current one is summed
         double acc = 0.0;
-        for (int c4 = 0; c4 < cnt; c4 += 4) {
-            const int4 q4 = nbr4[(int64_t)(c4 >> 2) * stride + k];
-            const int jj[4] = {q4.x, q4.y, q4.z, q4.w};
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                if (c4 + u >= cnt) break;
-                const int32_t j = jj[u];
-                const double4 pj = sxm[j];
-                const double hj = sh[j];
-                const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
-                const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                const double rr = sqrt(dx * dx + dy * dy + dz * dz);
-                const double ih = 1.0 / hb;
-                const double q = rr * ih;
-                const double f = K8PI * ih * ih * ih;
-                double W = 0.0;
-                if (q <= 0.5) W = f * (6.0 * q * q * (q - 1.0) + 1.0);
-                else if (q <= 1.0) { const double t = 1.0 - q; W = 2.0 * f * t * t * t; }
-                acc += pj.w * W;
+        int4 q4 = make_int4(0, 0, 0, 0);
+        double4 pn = make_double4(0.0, 0.0, 0.0, 0.0);
+        double hn = 0.0;
+        if (cnt > 0) {
+            q4 = nbr4[k];
+            pn = sxm[q4.x];
+            hn = sh[q4.x];
+        }
+        for (int s = 0; s < cnt; s++) {
+            const double4 pj = pn;
+            const double hj = hn;
+            if (s + 1 < cnt) {
+                const int u = (s + 1) & 3;
+                if (u == 0) q4 = nbr4[(int64_t)((s + 1) >> 2) * stride + k];
+                const int32_t jn = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
+                pn = sxm[jn];
+                hn = sh[jn];
             }
+            const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            const double ih = rcp_nr(hb);
+            const double q = r2 * rsq_nr(r2) * ih;
+            const double f = K8PI * (ih * ih * ih);
+            double W = 0.0;
+            if (q <= 0.5) W = f * fma(6.0 * q * q, q - 1.0, 1.0);
+            else if (q <= 1.0) { const double t = 1.0 - q; W = 2.0 * f * (t * t * t); }
+            acc = fma(pj.w, W, acc);
         }
         r = pi.w * (K8PI / (hi * hi * hi)) + acc;
     } else {
@@ -698,6 +810,7 @@ __global__ __launch_bounds__(256) void k_eos(int64_t n, int eos, double gamma, d
                                              const double *__restrict__ se, double *__restrict__ sp,
                                              double *__restrict__ scs, const double4 *__restrict__ sxm,
                                              double4 *__restrict__ sv4, double4 *__restrict__ sr4,
+                                             const double *__restrict__ sh,
                                              const int32_t *__restrict__ perm,
                                              double *__restrict__ p_out, double *__restrict__ cs_out) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -714,7 +827,7 @@ __global__ __launch_bounds__(256) void k_eos(int64_t n, int eos, double gamma, d
     sp[k] = p;
     scs[k] = cs;
     sv4[k].w = cs;
-    sr4[k] = make_double4(rho, p / (rho * rho), sxm[k].w / rho, p);
+    sr4[k] = make_double4(rho, p / (rho * rho), sxm[k].w / rho, sh[k]);
     const int32_t i = perm[k];
     p_out[i] = p;
     cs_out[i] = cs;
@@ -724,7 +837,7 @@ void launch_eos(mhpc_ctx *c) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_eos, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->n,
                        c->p.eos, c->p.gamma, c->p.K, c->p.cs_before_p, c->srho, c->se, c->sp, c->scs,
-                       c->sxm, c->sv4, c->sr4, c->perm, c->f[MHPC_F_P], c->f[MHPC_F_CS]);
+                       c->sxm, c->sv4, c->sr4, c->sh, c->perm, c->f[MHPC_F_P], c->f[MHPC_F_CS]);
 }
 
 // ------------------------------------------------------------------------------------
@@ -761,48 +874,62 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
     double mumax = 0.0;
     double c0 = 0, c1 = 0, c2 = 0, gr0 = 0, gr1 = 0, gr2 = 0;
     if constexpr (FP64) {
-        double a0 = 0, a1 = 0, a2 = 0, drho = 0, de = 0, dsml = 0;
+        // pair loop: the j records {x,y,z,m}, {vx,vy,vz,cs}, {rho, p/rho^2, m/rho, h} of the
+        // next neighbour are loaded while the current one is evaluated
+        double a0 = 0, a1 = 0, a2 = 0, dvv = 0, de = 0;
         const int4 *nbr4 = reinterpret_cast<const int4 *>(A.nbr);
         int4 q4 = make_int4(0, 0, 0, 0);
+        double4 pn = make_double4(0.0, 0.0, 0.0, 0.0), vn = pn, rn = pn;
+        if (cnt > 0) {
+            q4 = nbr4[k];
+            pn = A.sxm[q4.x];
+            vn = A.sv4[q4.x];
+            rn = A.sr4[q4.x];
+        }
         for (int s = 0; s < cnt; s++) {
-            if ((s & 3) == 0) q4 = nbr4[(int64_t)(s >> 2) * A.stride + k];
-            const int u = s & 3;
-            const int32_t j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
-            const double4 pj = A.sxm[j], vj = A.sv4[j], rj = A.sr4[j];
-            const double hj = A.sh[j];
+            const double4 pj = pn, vj = vn, rj = rn;
+            if (s + 1 < cnt) {
+                const int u = (s + 1) & 3;
+                if (u == 0) q4 = nbr4[(int64_t)((s + 1) >> 2) * A.stride + k];
+                const int32_t jn = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
+                pn = A.sxm[jn];
+                vn = A.sv4[jn];
+                rn = A.sr4[jn];
+            }
+            const double hj = rj.w;
             const double hb = 0.5 * (hi + hj);
             const double sml = A.sml_avg ? hb : hi;
             const double d0 = pi.x - pj.x, d1 = pi.y - pj.y, d2 = pi.z - pj.z;
-            const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-            const double r = sqrt(r2);
-            const double ih = 1.0 / sml;
-            const double q = r * ih;
-            const double fih = K8PI * ih * ih * ih * ih;
-            double dWdr = 0.0;
-            if (q <= 0.5) dWdr = 6.0 * fih * q * (3.0 * q - 2.0);
-            else if (q <= 1.0) { const double t = 1.0 - q; dWdr = -6.0 * fih * t * t; }
-            const double gs = dWdr / r;
+            const double r2 = fma(d0, d0, fma(d1, d1, d2 * d2));
+            const double rinv = rsq_nr(r2);
+            const double ih = rcp_nr(sml);
+            const double q = r2 * rinv * ih;
+            const double ih2 = ih * ih;
+            const double fih = K8PI * (ih2 * ih2);
+            // grad W = (dW/dr / r) r_ij; for q <= 1/2, dW/dr / r = 6 f/h (3q - 2) q / r with
+            // q / r = 1/h
+            double gs = 0.0;
+            if (q <= 0.5) gs = fih * ih * fma(18.0, q, -12.0);
+            else if (q <= 1.0) { const double t = 1.0 - q; gs = -6.0 * fih * (t * t) * rinv; }
             const double g0 = gs * d0, g1 = gs * d1, g2 = gs * d2;
             const double v0 = vi.x - vj.x, v1 = vi.y - vj.y, v2 = vi.z - vj.z;
-            const double vv = v0 * g0 + v1 * g1 + v2 * g2;
-            const double vr = v0 * d0 + v1 * d1 + v2 * d2;
+            const double vv = fma(v0, g0, fma(v1, g1, v2 * g2));
+            const double vr = fma(v0, d0, fma(v1, d1, v2 * d2));
             double mpij = 0.0;
             if (vr < 0) {
                 const double csbar = 0.5 * (vi.w + vj.w);
-                const double mu = hb * vr / (r2 + hb * hb * 1e-2);
+                const double mu = hb * vr * rcp_nr(fma(hb * hb, 1e-2, r2));
                 mumax = fmax(mumax, mu);
                 const double rhobar = 0.5 * (ri.x + rj.x);
-                mpij = pj.w * (A.beta * mu - A.alpha * csbar) * mu / rhobar;
+                mpij = pj.w * fma(A.beta, mu, -A.alpha * csbar) * mu * rcp_nr(rhobar);
             }
             const double pf = -pj.w * (ri.y + rj.y);
             const double fa = pf - mpij;
-            a0 += fa * g0;
-            a1 += fa * g1;
-            a2 += fa * g2;
-            const double mr_vv = rj.z * vv;
-            drho += mr_vv;
-            dsml += mr_vv;
-            de += (mpij - pf) * vv;
+            a0 = fma(fa, g0, a0);
+            a1 = fma(fa, g1, a1);
+            a2 = fma(fa, g2, a2);
+            dvv = fma(rj.z, vv, dvv);   // the drho/dt and dh/dt sums are the same sum
+            de = fma(mpij - pf, vv, de);
             if (FEAT) {
                 c0 += rj.z * (v1 * g2 - v2 * g1);
                 c1 += rj.z * (v2 * g0 - v0 * g2);
@@ -813,9 +940,9 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
             }
         }
         out[0] = a0; out[1] = a1; out[2] = a2;
-        out[3] = ri.x * drho;
+        out[3] = ri.x * dvv;
         out[4] = 0.5 * de;
-        out[5] = -(1.0 / 3.0) * hi * dsml;
+        out[5] = -(1.0 / 3.0) * hi * dvv;
     } else {
         Kahan a0, a1, a2, drho, de, dsml;
         const float hif = (float)hi, rhoif = (float)ri.x, csif = (float)vi.w, prif = (float)ri.y;
@@ -827,7 +954,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
             const int u = s & 3;
             const int32_t j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
             const double4 pj = A.sxm[j], vj = A.sv4[j], rj = A.sr4[j];
-            const float hj = (float)A.sh[j];
+            const float hj = (float)rj.w;
             const float hb = 0.5f * (hif + hj);
             const float sml = A.sml_avg ? hb : hif;
             const float d0 = (float)(pi.x - pj.x), d1 = (float)(pi.y - pj.y), d2 = (float)(pi.z - pj.z);

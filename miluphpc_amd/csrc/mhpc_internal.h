@@ -108,6 +108,8 @@ struct mhpc_ctx {
     double last_dt = 0.0;                        // last CFL time step (learned-h feature)
     int32_t *hlist[3] = {nullptr, nullptr, nullptr};   // h-iteration: unconverged particles per pass
     uint8_t *sdirty = nullptr;                   // speculative lists: particles to re-list
+    int32_t *olist = nullptr;                    // neighbour search: group-mode outliers
+    uint32_t *ocount = nullptr;
     unsigned long long *hmax_bits = nullptr;     // largest h (bit pattern)
     uint32_t *hlist_count = nullptr;             // [MHPC_MAX_PASSES + 1]
     double *hslot = nullptr;                     // new h per list slot

@@ -189,3 +189,37 @@ def test_speculative_lists_equal_plain(gpu_lib, orc, sedov24, precision):
     st, onnl, onoi = orc.nn_grid(s["x"], s["y"], s["z"], b["sml"], 180)
     assert st == 0
     assert sets(*b["lists"]) == sets(onnl, onoi)
+
+
+def test_outlier_particles_match_oracle(gpu_lib, orc, sedov24):
+    """Particles whose h is several times their group's smallest are split off the group
+    walk into the per-particle mode: neighbour sets, h-iteration and speculative lists stay
+    exactly the oracle's."""
+    print_case("outliers: 5% of particles with 4x h, sedov 24^3")
+    rng = np.random.default_rng(77)
+    s = dict(sedov24)
+    big = rng.uniform(size=len(s["x"])) < 0.05
+    h0 = s["sml"] * np.where(big, 4.0, 1.0)
+    p = gpu_lib.default_params(curve=1, max_nb=2000)
+    with make_ctx(gpu_lib, {**s, "sml": h0}, p) as ctx:
+        ctx.bbox(); ctx.keys(); ctx.sort(); ctx.build_tree()
+        ctx.neighbours()
+        nnl, noi = ctx.neighbour_lists()
+    st, onnl, onoi = orc.nn_grid(s["x"], s["y"], s["z"], h0, 2000)
+    assert st == 0
+    assert np.array_equal(noi, onoi)
+    assert sets(nnl, noi) == sets(onnl, onoi)
+    p = gpu_lib.default_params(curve=1)
+    base = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML
+    out = {}
+    for name, flags in (("plain", base), ("spec", base | gpu_lib.RHS_SPECULATIVE_NN)):
+        with make_ctx(gpu_lib, {**s, "sml": h0}, p) as ctx:
+            ctx.rhs(flags)
+            out[name] = (ctx.download("sml"), ctx.download("h_iters"), ctx.neighbour_lists())
+    st, oh, oiters, _ = orc.h_iterate(s["x"], s["y"], s["z"], h0)
+    for name in out:
+        assert np.array_equal(out[name][0], oh), name
+        assert np.array_equal(out[name][1], oiters), name
+    st, onnl, onoi = orc.nn_grid(s["x"], s["y"], s["z"], oh, 180)
+    for name in out:
+        assert sets(*out[name][2]) == sets(onnl, onoi), name
