@@ -124,6 +124,7 @@ def main() -> int:
     ap.add_argument("--no-sedov", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-speculative", action="store_true")
     ap.add_argument("--learned-nside", type=int, default=128)
     ap.add_argument("--learned-steps", type=int, default=10)
     ap.add_argument("--learned-train-iters", type=int, default=100)
@@ -183,13 +184,15 @@ def run_sedov(args, lib, ics, torch, result):
     with lib.Context(n, p) as ctx:
         ctx.load(**{k: v for k, v in s.items() if k in lib.FIELD_ID})
         wall, stats = time_rhs(ctx, lib, flags, args.steps, args.warmup, "sedov", restore={"sml": h0})
-        # the same RHS with speculative lists (same h and neighbour sets, one walk fewer)
-        wall_s, stats_s = time_rhs(ctx, lib, flags | lib.RHS_SPECULATIVE_NN, args.steps, 1, "sedov-speculative",
-                                   restore={"sml": h0})
-    med_s = median_stats(stats_s)
-    result["sedov_speculative"] = {
-        "value": n / wall_s, "ms_per_step": wall_s * 1e3,
-        "phases_ms": {k[3:]: round(v, 4) for k, v in med_s.items() if k.startswith("ms_")}}
+        if not args.no_speculative:
+            # the same RHS with speculative lists (same h and neighbour sets, one walk fewer)
+            wall_s, stats_s = time_rhs(ctx, lib, flags | lib.RHS_SPECULATIVE_NN, args.steps, 1,
+                                       "sedov-speculative", restore={"sml": h0})
+    if not args.no_speculative:
+        med_s = median_stats(stats_s)
+        result["sedov_speculative"] = {
+            "value": n / wall_s, "ms_per_step": wall_s * 1e3,
+            "phases_ms": {k[3:]: round(v, 4) for k, v in med_s.items() if k.startswith("ms_")}}
     med = median_stats(stats)
     nnb = med["n_neighbours"] / n
     bpp = sph_bytes_per_particle(nnb)

@@ -471,18 +471,40 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
     }
 }
 
-// caller-order counts and the neighbour statistics from the sorted counts
+// caller-order counts and the neighbour statistics from the sorted counts: grid-stride,
+// one block-level sum / max and one pair of atomics per block (a pair per wave would
+// serialise tens of thousands of atomics on one address)
+#define PUBLISH_BLOCKS 512
 __global__ __launch_bounds__(256) void k_publish_noi(int64_t n, const int32_t *__restrict__ perm,
                                                      const int32_t *__restrict__ snoi, int32_t *__restrict__ noi,
                                                      unsigned long long *__restrict__ counters) {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int v = k < n ? snoi[k] : 0;
-    if (k < n) noi[perm[k]] = v;
-    const unsigned long long s = (unsigned long long)wave_sum(v);
-    const int mx = wave_max(v);
+    __shared__ unsigned long long ssum[4];
+    __shared__ int smax[4];
+    unsigned long long sum = 0;
+    int mx = 0;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const int v = snoi[k];
+        noi[perm[k]] = v;
+        sum += (unsigned long long)v;
+        mx = v > mx ? v : mx;
+    }
+    sum = wave_sum(sum);
+    mx = wave_max(mx);
+    const int w = threadIdx.x / 64;
     if (wave_lane() == 0) {
-        atomicAdd(&counters[CNT_NEIGHBOURS], s);
-        atomicMax(&counters[CNT_MAX_NOI], (unsigned long long)mx);
+        ssum[w] = sum;
+        smax[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        int m = 0;
+        for (int i = 0; i < 4; i++) {
+            t += ssum[i];
+            m = smax[i] > m ? smax[i] : m;
+        }
+        atomicAdd(&counters[CNT_NEIGHBOURS], t);
+        atomicMax(&counters[CNT_MAX_NOI], (unsigned long long)m);
     }
 }
 
@@ -621,8 +643,8 @@ void launch_neighbours_fixup(mhpc_ctx *c) {
     b.counters = nullptr;   // k_publish_noi counts every particle
     b.noi_out = nullptr;
     hipLaunchKernelGGL(k_nn_one<0>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, b);
-    hipLaunchKernelGGL(k_publish_noi, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->n, c->perm,
-                       c->snoi, c->noi, c->counters);
+    hipLaunchKernelGGL(k_publish_noi, dim3((unsigned)std::min<int64_t>(ceil_div(c->n, 256), PUBLISH_BLOCKS)),
+                       dim3(256), 0, c->stream, c->n, c->perm, c->snoi, c->noi, c->counters);
 }
 
 void launch_neighbours(mhpc_ctx *c) {

@@ -48,6 +48,21 @@ def test_keys_and_sort_bitexact(gpu_lib, orc, plummer64k, curve):
     assert np.array_equal(perm, orc.sort(ko))
 
 
+def test_keys_golden_fixture(gpu_lib):
+    """Lebesgue keys of dyadic points against the committed closed-form vectors."""
+    from pathlib import Path
+    print_case("keys golden fixture (closed form, 4100 dyadic points)")
+    g = np.load(Path(__file__).parent / "golden" / "keys_lebesgue_dyadic.npz")
+    n = len(g["keys"])
+    s = {"x": g["x"].copy(), "y": g["y"].copy(), "z": g["z"].copy(), "mass": np.full(n, 1.0 / n)}
+    with make_ctx(gpu_lib, s, gpu_lib.default_params(curve=0)) as ctx:
+        box = ctx.bbox()
+        assert np.array_equal(box, [-1.0, 1.0, -1.0, 1.0, -1.0, 1.0])
+        ctx.keys()
+        k = ctx.download("key")
+    assert np.array_equal(k, g["keys"])
+
+
 def test_sort_ties_and_ragged_sizes(gpu_lib, orc):
     """Duplicated keys keep index order; sizes that are not multiples of the tile."""
     rng = np.random.default_rng(31)

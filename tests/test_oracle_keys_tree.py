@@ -49,6 +49,15 @@ def test_lebesgue_keys_dyadic_golden(orc):
     assert np.array_equal(k, closed_form_lebesgue(ix, iy, iz))
 
 
+def test_lebesgue_keys_golden_fixture(orc):
+    """Committed closed-form vectors (tests/golden/make_golden.py)."""
+    from pathlib import Path
+    g = np.load(Path(__file__).parent / "golden" / "keys_lebesgue_dyadic.npz")
+    box = orc.bbox(g["x"], g["y"], g["z"])
+    assert np.array_equal(box, [-1.0, 1.0, -1.0, 1.0, -1.0, 1.0])
+    assert np.array_equal(orc.keys(g["x"], g["y"], g["z"], box, 0), g["keys"])
+
+
 def test_lebesgue_hand_points(orc):
     box = np.array([-1.0, 1.0, -1.0, 1.0, -1.0, 1.0])
     # the lowest corner is in the lower half on every axis at every level: all digits 7
