@@ -518,6 +518,7 @@ static int ensure_mlp(mhpc_ctx *c) {
     if (c->mlp_params) return MHPC_OK;
     const int np = mlp_num_params();
     const int64_t max_waves = ceil_div(MLP_MAX_SAMPLES, 64);
+    const int64_t max_chunks = ceil_div(max_waves, 32);   // stage-1 reduction sums
     HIPC(dalloc(&c->mlp_params, np));
     HIPC(dalloc(&c->mlp_m1, np));
     HIPC(dalloc(&c->mlp_m2, np));
@@ -527,8 +528,9 @@ static int ensure_mlp(mhpc_ctx *c) {
     HIPC(dalloc(&c->mlp_t, c->cap));
     HIPC(dalloc(&c->mlp_hprev, c->cap));
     HIPC(dalloc(&c->mlp_idx, MLP_MAX_SAMPLES));
-    HIPC(dalloc(&c->mlp_partial, max_waves * np));
-    HIPC(dalloc(&c->mlp_loss_partial, max_waves));
+    HIPC(dalloc(&c->mlp_partial, (max_waves + max_chunks) * np));
+    HIPC(dalloc(&c->mlp_loss_partial, max_waves + max_chunks));
+    c->mlp_max_waves = max_waves;
     HIPC(dalloc(&c->mlp_loss, 1));
     float norm[18];
     for (int k = 0; k < 18; k++) norm[k] = (k >= 8 && k < 15) || k == 17 ? 1.f : 0.f;

@@ -349,20 +349,40 @@ __global__ __launch_bounds__(64 * MLP_TR_WAVES) void k_mlp_train_grad(int64_t nb
     }
 }
 
-// fold per-wave partials in wave order (deterministic): grad[p] = sum_w partial[w][p];
+// fold per-wave partials in a fixed order (deterministic), in two levels: chunks of
+// MLP_RED_CHUNK waves in parallel (stage 1), then the chunk sums (stage 2);
 // loss = mean of 0.5 (y - t)^2
-__global__ __launch_bounds__(256) void k_mlp_reduce(int nw, int64_t nb, const float *__restrict__ partial,
-                                                    const float *__restrict__ loss_partial,
-                                                    float *__restrict__ grad, float *__restrict__ loss) {
+#define MLP_RED_CHUNK 32
+__global__ __launch_bounds__(256) void k_mlp_reduce1(int nw, const float *__restrict__ partial,
+                                                     const float *__restrict__ loss_partial,
+                                                     float *__restrict__ part2, float *__restrict__ loss2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int w0 = blockIdx.y * MLP_RED_CHUNK;
+    const int w1 = w0 + MLP_RED_CHUNK < nw ? w0 + MLP_RED_CHUNK : nw;
+    if (p < MLP_NPARAM) {
+        float s = 0.f;
+        for (int w = w0; w < w1; w++) s += partial[(int64_t)w * MLP_NPARAM + p];
+        part2[(int64_t)blockIdx.y * MLP_NPARAM + p] = s;
+    }
+    if (p == 0) {
+        float s = 0.f;
+        for (int w = w0; w < w1; w++) s += loss_partial[w];
+        loss2[blockIdx.y] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_reduce2(int nchunks, int64_t nb, const float *__restrict__ part2,
+                                                     const float *__restrict__ loss2, float *__restrict__ grad,
+                                                     float *__restrict__ loss) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < MLP_NPARAM) {
         float s = 0.f;
-        for (int w = 0; w < nw; w++) s += partial[(int64_t)w * MLP_NPARAM + p];
+        for (int c = 0; c < nchunks; c++) s += part2[(int64_t)c * MLP_NPARAM + p];
         grad[p] = s;
     }
     if (p == 0) {
         float s = 0.f;
-        for (int w = 0; w < nw; w++) s += loss_partial[w];
+        for (int c = 0; c < nchunks; c++) s += loss2[c];
         *loss = s / (float)nb;
     }
 }
@@ -410,8 +430,15 @@ void launch_mlp_gradient(mhpc_ctx *c, int64_t nb, const float *X, const float *T
     const int64_t nw = ceil_div(nb, 64);
     hipLaunchKernelGGL(k_mlp_train_grad, dim3((unsigned)ceil_div(nw, MLP_TR_WAVES)), dim3(64 * MLP_TR_WAVES), 0,
                        c->stream, nb, X, T, idx, c->mlp_params, c->mlp_partial, c->mlp_loss_partial);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)ceil_div(MLP_NPARAM, 256)), dim3(256), 0, c->stream,
-                       (int)nw, nb, c->mlp_partial, c->mlp_loss_partial, grad, loss);
+    // stage-1 sums go behind the per-wave partials in the same buffer (sized for the
+    // maximum wave count plus its chunk sums)
+    const int nchunks = (int)ceil_div(nw, MLP_RED_CHUNK);
+    float *part2 = c->mlp_partial + (int64_t)c->mlp_max_waves * MLP_NPARAM;
+    float *loss2 = c->mlp_loss_partial + c->mlp_max_waves;
+    hipLaunchKernelGGL(k_mlp_reduce1, dim3((unsigned)ceil_div(MLP_NPARAM, 256), (unsigned)nchunks), dim3(256), 0,
+                       c->stream, (int)nw, c->mlp_partial, c->mlp_loss_partial, part2, loss2);
+    hipLaunchKernelGGL(k_mlp_reduce2, dim3((unsigned)ceil_div(MLP_NPARAM, 256)), dim3(256), 0, c->stream, nchunks,
+                       nb, part2, loss2, grad, loss);
 }
 
 void launch_mlp_adam(mhpc_ctx *c, float lr, int step) {

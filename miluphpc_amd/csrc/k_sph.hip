@@ -372,6 +372,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
 // MODE 2: MODE 1 plus the list; MODE 0: the list only.  Lists are written in candidate
 // order, one 4-byte entry per lane.
 #define NN1_WAVES 4
+#define NN1_MAX_BLOCKS 8192   // one wavefront per listed particle up to 32768 particles
 template <int MODE>
 __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
@@ -573,7 +574,7 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     a.h_next = c->sh2;
     a.list_out = c->hlist[2];
     a.count_out = c->hlist_count + 0;
-    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), 2048);
+    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), NN1_MAX_BLOCKS);
     launch_fill_u32(c, c->ocount, 0u, 1);
     a.out_list = c->olist;
     a.out_count = c->ocount;
@@ -625,7 +626,7 @@ void launch_neighbours_fixup(mhpc_ctx *c) {
     a.count_in = c->hlist_count + 0;
     a.dirty = c->sdirty;
     a.hmax_p = reinterpret_cast<const double *>(c->hmax_bits);
-    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), 2048);
+    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), NN1_MAX_BLOCKS);
     hipLaunchKernelGGL(k_nn_mark_dirty, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
     NNArgs b = nn_common(c);
     b.h = c->sh;
@@ -665,7 +666,7 @@ void launch_neighbours(mhpc_ctx *c) {
     a.out_list = nullptr;
     a.list_in = c->olist;
     a.count_in = c->ocount;
-    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), 2048);
+    const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), NN1_MAX_BLOCKS);
     hipLaunchKernelGGL(k_nn_one<0>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
 }
 

@@ -124,6 +124,7 @@ struct mhpc_ctx {
     float *mlp_partial = nullptr, *mlp_loss_partial = nullptr, *mlp_loss = nullptr;
     float *mlp_io_x = nullptr, *mlp_io_y = nullptr;   // test-hook staging
     int64_t mlp_io_cap = 0;
+    int64_t mlp_max_waves = 0;                   // per-wave gradient partials allocated
     int mlp_adam_step = 0;
     bool mlp_x_valid = false, mlp_norm_set = false;
     double *box = nullptr;                       // [6]
