@@ -262,6 +262,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
 
     int count = 0;
     int b0 = 0, b1 = 0, b2 = 0;   // pending list entries, flushed four at a time as one int4
+    int4 *wp = A.nbr4 + k;        // where the next four go
     const bool ok = nn_walk_box(L, A, lox, hix, loy, hiy, loz, hiz, [&](int m) {
         if (!active) return;
         if (MODE == 1) {
@@ -274,26 +275,24 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             }
             return;
         }
+        // list modes: the append is predicated (selects into the three pending entries);
+        // only the 16-byte store of every fourth entry is a branch
         for (int e = 0; e < m; e++) {
             const double dx = xi - L.tx[e];
             const double dy = yi - L.ty[e];
             const double dz = zi - L.tz[e];
             const double d = dx * dx + dy * dy + dz * dz;
             const int j = L.tj[e];
-            if (d < hi2 && d < L.th2[e] && j != (int)k) {
-                const int slot = count & 3;
-                if (slot == 3) {
-                    if (count < A.max_nb)
-                        A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, j);
-                } else if (slot == 0) {
-                    b0 = j;
-                } else if (slot == 1) {
-                    b1 = j;
-                } else {
-                    b2 = j;
-                }
-                count++;
+            const bool hit = (d < hi2) & (d < L.th2[e]) & (j != (int)k);
+            const int slot = count & 3;
+            b0 = (hit && slot == 0) ? j : b0;
+            b1 = (hit && slot == 1) ? j : b1;
+            b2 = (hit && slot == 2) ? j : b2;
+            if (hit && slot == 3) {
+                if (count < A.max_nb) *wp = make_int4(b0, b1, b2, j);
+                wp += A.stride;
             }
+            count += hit ? 1 : 0;
         }
     });
     if (!ok) {
