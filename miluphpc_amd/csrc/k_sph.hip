@@ -717,167 +717,72 @@ __device__ __forceinline__ double rsq_nr(double x) {
     return fma(y, e, y);
 }
 
-// one density pair term, the arithmetic of both density kernels: FP64 adds m_j W into acc;
-// FP32 adds it (in FP32, from FP64 differences) into the Kahan accumulator acc32
 template <bool FP64>
-__device__ __forceinline__ void density_pair(const double4 &pi, double hi, const double4 &pj, double hj,
-                                             int sml_avg, double &acc, Kahan &acc32) {
-    if constexpr (FP64) {
-        const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
-        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double ih = rcp_nr(hb);
-        const double q = r2 * rsq_nr(r2) * ih;
-        const double f = K8PI * (ih * ih * ih);
-        double W = 0.0;
-        if (q <= 0.5) W = f * fma(6.0 * q * q, q - 1.0, 1.0);
-        else if (q <= 1.0) { const double t = 1.0 - q; W = 2.0 * f * (t * t * t); }
-        acc = fma(pj.w, W, acc);
-    } else {
-        const float hb = (float)(sml_avg ? 0.5 * (hi + hj) : hi);
-        const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y), dz = (float)(pi.z - pj.z);
-        const float rr = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
-        float W, dW;
-        cubic32(rr, hb, W, dW);
-        acc32.add((float)pj.w * W);
-    }
-}
-
-// ------------------------------------------------------------------------------------
-// Neighbour staging.  Per-lane gathers of neighbour records touch ~50 distinct cache lines
-// per wave instruction, which makes the pair loops bound by the L1 address path.  The
-// staged form lets a wavefront (64 consecutive sorted particles) collect the union of its
-// lanes' neighbour lists in an LDS hash set (open addressing, keys = sorted indices), fetch
-// each member's record once into LDS, and read neighbours from LDS in the pair loop.  The
-// per-lane arithmetic and its order are unchanged, so results are bit-identical to the
-// gather form, which remains the fallback for a wave whose union exceeds the LDS capacity.
-#define STG_HT 1024            // hash slots (power of two)
-#define STG_CAP 768            // union capacity (records in LDS)
-#define STG_WAVES 2
-
-__device__ __forceinline__ int stg_hash(int j) { return (int)(((uint32_t)j * 2654435761u) >> (32 - 10)); }
-
-struct StageSet {
-    int key[STG_HT];
-    int slot[STG_HT];
-};
-
-// Build the wave's neighbour union: returns the union size, or -1 if it does not fit.
-// Union members get slots 0..U-1 in hash-table order (deterministic for a given set).
-__device__ int stage_union(StageSet &S, const int4 *__restrict__ nbr4, int64_t stride, int64_t k, int cnt) {
-    const int lane = wave_lane();
-    for (int e = lane; e < STG_HT; e += 64) S.key[e] = -1;
-    wave_sync();
-    bool full = false;
-    int4 q4 = make_int4(0, 0, 0, 0);
-    for (int s = 0; s < cnt; s++) {
-        if ((s & 3) == 0) q4 = nbr4[(int64_t)(s >> 2) * stride + k];
-        const int u = s & 3;
-        const int j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
-        int h = stg_hash(j);
-        int probes = 0;
-        for (; probes < STG_HT; probes++) {
-            const int old = atomicCAS(&S.key[h], -1, j);
-            if (old == -1 || old == j) break;
-            h = (h + 1) & (STG_HT - 1);
-        }
-        if (probes == STG_HT) full = true;
-    }
-    wave_sync();
-    // slots in table order: each lane owns STG_HT / 64 consecutive entries
-    const int per = STG_HT / 64;
-    int c = 0;
-    for (int e = 0; e < per; e++) c += S.key[lane * per + e] >= 0;
-    const int incl = wave_incl_scan(c);
-    const int total = __shfl(incl, 63);
-    int sl = incl - c;
-    for (int e = 0; e < per; e++) {
-        const int idx = lane * per + e;
-        if (S.key[idx] >= 0) S.slot[idx] = sl++;
-    }
-    wave_sync();
-    if (wave_ballot(full) != 0 || total > STG_CAP) return -1;
-    return total;
-}
-
-__device__ __forceinline__ int stage_lookup(const StageSet &S, int j) {
-    int h = stg_hash(j);
-    for (int probes = 0; probes < STG_HT && S.key[h] != j; probes++) h = (h + 1) & (STG_HT - 1);
-    return S.slot[h];
-}
-
-struct DensStage {
-    StageSet set;
-    double4 p[STG_CAP];
-    double h[STG_CAP];
-};
-
-template <bool FP64>
-__global__ __launch_bounds__(64 * STG_WAVES) void k_density_staged(int64_t n, const double4 *__restrict__ sxm,
-                                                                  const double *__restrict__ sh,
-                                                                  const int32_t *__restrict__ snoi,
-                                                                  const int32_t *__restrict__ nbr, int64_t stride,
-                                                                  int max_nb, int sml_avg,
-                                                                  const int32_t *__restrict__ perm,
-                                                                  double *__restrict__ srho, double *__restrict__ rho,
-                                                                  uint32_t *err) {
-    __shared__ DensStage lds_all[STG_WAVES];
-    const int w = threadIdx.x / 64, lane = wave_lane();
-    DensStage &L = lds_all[w];
-    const int64_t group = xcd_block(blockIdx.x, gridDim.x) * STG_WAVES + w;
-    const int64_t k = group * 64 + lane;
-    if (group * 64 >= n) return;
-    const bool valid = k < n;
-    const int4 *nbr4 = reinterpret_cast<const int4 *>(nbr);
-    int cnt = valid ? snoi[k] : 0;
-    cnt = cnt > max_nb ? max_nb : cnt;
-    const int U = stage_union(L.set, nbr4, stride, k, cnt);
-    if (U < 0) {
-        // the union does not fit: per-lane gathers (the same arithmetic)
-        if (!valid) return;
-        const double4 pi = sxm[k];
-        const double hi = sh[k];
-        double acc = 0.0;
-        Kahan acc32;
-        int4 q4 = make_int4(0, 0, 0, 0);
-        for (int s = 0; s < cnt; s++) {
-            if ((s & 3) == 0) q4 = nbr4[(int64_t)(s >> 2) * stride + k];
-            const int u = s & 3;
-            const int32_t j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
-            density_pair<FP64>(pi, hi, sxm[j], sh[j], sml_avg, acc, acc32);
-        }
-        const double r = pi.w * (K8PI / (hi * hi * hi)) + (FP64 ? acc : acc32.value());
-        srho[k] = r;
-        rho[perm[k]] = r;
-        if (!(r > 0.0)) set_err(err, ERRF_RHO);
-        return;
-    }
-    // fetch the union's records once
-    const int per = STG_HT / 64;
-    for (int e = 0; e < per; e++) {
-        const int idx = lane * per + e;
-        const int j = L.set.key[idx];
-        if (j >= 0) {
-            const int sl = L.set.slot[idx];
-            L.p[sl] = sxm[j];
-            L.h[sl] = sh[j];
-        }
-    }
-    wave_sync();
-    if (!valid) return;
+__global__ __launch_bounds__(256) void k_density(int64_t n, const double4 *__restrict__ sxm,
+                                                 const double *__restrict__ sh,
+                                                 const int32_t *__restrict__ snoi,
+                                                 const int32_t *__restrict__ nbr, int64_t stride,
+                                                 int max_nb, int sml_avg, const int32_t *__restrict__ perm,
+                                                 double *__restrict__ srho, double *__restrict__ rho,
+                                                 uint32_t *err) {
+    const int64_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
     const double4 pi = sxm[k];
     const double hi = sh[k];
-    double acc = 0.0;
-    Kahan acc32;
-    int4 q4 = make_int4(0, 0, 0, 0);
-    for (int s = 0; s < cnt; s++) {
-        if ((s & 3) == 0) q4 = nbr4[(int64_t)(s >> 2) * stride + k];
-        const int u = s & 3;
-        const int32_t j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
-        const int sl = stage_lookup(L.set, j);
-        density_pair<FP64>(pi, hi, L.p[sl], L.h[sl], sml_avg, acc, acc32);
+    int cnt = snoi[k];
+    cnt = cnt > max_nb ? max_nb : cnt;
+    double r;
+    const int4 *nbr4 = reinterpret_cast<const int4 *>(nbr);
+    if constexpr (FP64) {
+        // the next neighbour's {x,y,z,m} and h are loaded while the current one is summed
+        double acc = 0.0;
+        int4 q4 = make_int4(0, 0, 0, 0);
+        double4 pn = make_double4(0.0, 0.0, 0.0, 0.0);
+        double hn = 0.0;
+        if (cnt > 0) {
+            q4 = nbr4[k];
+            pn = sxm[q4.x];
+            hn = sh[q4.x];
+        }
+        for (int s = 0; s < cnt; s++) {
+            const double4 pj = pn;
+            const double hj = hn;
+            if (s + 1 < cnt) {
+                const int u = (s + 1) & 3;
+                if (u == 0) q4 = nbr4[(int64_t)((s + 1) >> 2) * stride + k];
+                const int32_t jn = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
+                pn = sxm[jn];
+                hn = sh[jn];
+            }
+            const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            const double ih = rcp_nr(hb);
+            const double q = r2 * rsq_nr(r2) * ih;
+            const double f = K8PI * (ih * ih * ih);
+            double W = 0.0;
+            if (q <= 0.5) W = f * fma(6.0 * q * q, q - 1.0, 1.0);
+            else if (q <= 1.0) { const double t = 1.0 - q; W = 2.0 * f * (t * t * t); }
+            acc = fma(pj.w, W, acc);
+        }
+        r = pi.w * (K8PI / (hi * hi * hi)) + acc;
+    } else {
+        Kahan acc;
+        for (int s = 0; s < cnt; s++) {
+            const int4 q4 = nbr4[(int64_t)(s >> 2) * stride + k];
+            const int u = s & 3;
+            const int32_t j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
+            const double4 pj = sxm[j];
+            const double hj = sh[j];
+            const float hb = (float)(sml_avg ? 0.5 * (hi + hj) : hi);
+            const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y), dz = (float)(pi.z - pj.z);
+            const float rr = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+            float W, dW;
+            cubic32(rr, hb, W, dW);
+            acc.add((float)pj.w * W);
+        }
+        r = pi.w * (K8PI / (hi * hi * hi)) + acc.value();
     }
-    const double r = pi.w * (K8PI / (hi * hi * hi)) + (FP64 ? acc : acc32.value());
     srho[k] = r;
     rho[perm[k]] = r;
     if (!(r > 0.0)) set_err(err, ERRF_RHO);
@@ -885,21 +790,21 @@ __global__ __launch_bounds__(64 * STG_WAVES) void k_density_staged(int64_t n, co
 
 void launch_density(mhpc_ctx *c) {
     if (c->n <= 0) return;
-    const unsigned g = (unsigned)ceil_div(ceil_div(c->n, 64), STG_WAVES);
+    const unsigned g = (unsigned)ceil_div(c->n, 256);
     if (c->p.precision == MHPC_PREC_FP64)
-        hipLaunchKernelGGL(k_density_staged<true>, dim3(g), dim3(64 * STG_WAVES), 0, c->stream, c->n, c->sxm, c->sh,
-                           c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
+        hipLaunchKernelGGL(k_density<true>, dim3(g), dim3(256), 0, c->stream, c->n, c->sxm, c->sh, c->snoi,
+                           c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
                            c->f[MHPC_F_RHO], c->err);
     else
-        hipLaunchKernelGGL(k_density_staged<false>, dim3(g), dim3(64 * STG_WAVES), 0, c->stream, c->n, c->sxm,
-                           c->sh, c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
+        hipLaunchKernelGGL(k_density<false>, dim3(g), dim3(256), 0, c->stream, c->n, c->sxm, c->sh, c->snoi,
+                           c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
                            c->f[MHPC_F_RHO], c->err);
 }
 
 // ------------------------------------------------------------------------------------
 // a12 / a13 sound speed (soundspeed.cu:38-97) and pressure (pressure.cu:6-87), fused;
 // also writes the records the force loop gathers: cs into {vx,vy,vz,cs} and
-// {rho, p/rho^2, m/rho, h}.
+// {rho, p/rho^2, m/rho, p}.
 __device__ __forceinline__ double eos_cs(int eos, double gamma, double K, double rho, double p,
                                          double cs) {
     switch (eos) {
