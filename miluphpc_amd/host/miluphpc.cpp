@@ -12,6 +12,7 @@ static mhpc_params to_params(const SimulationParameters &sp, const Material &m) 
     p.quadrupole = sp.quadrupole ? 1 : 0;
     p.curve = sp.sfcSelection == Curve::hilbert ? MHPC_CURVE_HILBERT : MHPC_CURVE_LEBESGUE;
     p.precision = sp.precision;
+    p.sml_max = sp.smlMax;
     p.eos = m.eos.type;
     p.gamma = m.eos.polytropic_gamma;
     p.K = m.eos.polytropic_K;
@@ -56,7 +57,11 @@ real Miluphpc::rhs(int step, bool selfGravity, bool assignParticlesToProcess) {
     uint32_t flags = 0;
     if (simulationParameters.gravitySim && selfGravity) flags |= MHPC_RHS_GRAVITY;
     if (simulationParameters.sphSim) flags |= MHPC_RHS_SPH;
-    if (simulationParameters.sphSim && simulationParameters.variableSML) flags |= MHPC_RHS_VARIABLE_SML;
+    if (simulationParameters.sphSim && simulationParameters.variableSML) {
+        flags |= MHPC_RHS_VARIABLE_SML;
+        if (simulationParameters.speculativeLists) flags |= MHPC_RHS_SPECULATIVE_NN;
+        if (simulationParameters.learnedSML) flags |= MHPC_RHS_LEARNED_H;
+    }
     if (simulationParameters.calculateEnergy && (flags & MHPC_RHS_GRAVITY)) flags |= MHPC_RHS_POTENTIAL;
     check(mhpc_rhs(ctx_, flags, &stats_), "rhs");
     return (real)stats_.ms_total;

@@ -78,4 +78,7 @@ struct SimulationParameters {
     // build extensions
     bool quadrupole = false;       // monopole + quadrupole far field
     int precision = 0;             // 0 FP64 pair sums, 1 FP32 + Kahan
+    bool speculativeLists = true;  // neighbour lists from pass 0 of the h-iteration (same sets)
+    bool learnedSML = false;       // MLP start for the h-iteration (mhpc_mlp_* must have trained it)
+    real smlMax = 0.0;             // upper bound of h in the h-iteration (0: none)
 };
