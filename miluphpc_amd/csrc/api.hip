@@ -90,16 +90,8 @@ static void upload_htable(mhpc_ctx *c) {
 static int ensure_nbr(mhpc_ctx *c) {
     if (c->nbr && c->nbr_rows >= c->p.max_nb) return MHPC_OK;
     if (c->nbr) (void)hipFree(c->nbr);
-    if (c->nslot) (void)hipFree(c->nslot);
     c->nbr = nullptr;
-    c->nslot = nullptr;
     HIPC(dalloc(&c->nbr, ceil_div(c->p.max_nb, 4) * 4 * c->cap_pad));
-    HIPC(dalloc(&c->nslot, ceil_div(c->p.max_nb, 8) * 8 * c->cap_pad));
-    if (!c->gunion) {
-        const int64_t groups = ceil_div(c->cap, 64);
-        HIPC(dalloc(&c->gunion, groups * nn_union_capacity()));
-        HIPC(dalloc(&c->gucount, groups));
-    }
     c->nbr_rows = c->p.max_nb;
     return MHPC_OK;
 }
@@ -115,7 +107,7 @@ static void free_all(mhpc_ctx *c) {
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
                     c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->hslot, c->sdirty,
-                    c->hmax_bits, c->olist, c->ocount, c->gunion, c->gucount, c->nslot};
+                    c->hmax_bits, c->olist, c->ocount};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)

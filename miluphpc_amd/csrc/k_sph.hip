@@ -20,7 +20,6 @@
 #define NN_LEAF 16
 #define NN_OUTLIER 2.0       // group-mode outlier: h above this factor x the group's smallest
 #define NN_OUTLIER_MAX 16    // at most this many outliers per group are split off
-#define NN_UCAP 480          // neighbour-union capacity per group (records staged in LDS)
 
 struct NNWaveLds {
     int stack[NN_STACK_PHYS];
@@ -69,11 +68,6 @@ struct NNArgs {
     unsigned long long *hmax0_bits;   // pass 0 with lists: largest h (bit pattern, atomicMax)
     int32_t *out_list;        // group mode: outlier particles deferred to the per-particle mode
     uint32_t *out_count;
-    // list modes, group mode: the union of the group's lists in candidate order
-    // (gunion[g * NN_UCAP + u]), its size (gucount[g], -1 when the group has none) and each
-    // entry's position in it (16-bit, eight per chunk: nslot8[(s / 8) * stride + k])
-    int32_t *gunion, *gucount;
-    uint4 *nslot8;
     uint32_t *err;
     unsigned long long *counters;
 };
@@ -231,10 +225,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     const bool valid = k < n;
     bool active = valid;
     if (MODE >= 1 && valid) active = A.done[k] == 0;
-    if (MODE == 0 && A.only) {
-        // re-list whole groups that hold a flagged particle (their union is rebuilt)
-        active = wave_ballot(valid && A.only[k] != 0) != 0 && valid;
-    }
+    if (MODE == 0 && valid && A.only) active = A.only[k] != 0;
     uint64_t mActive = wave_ballot(active);
     if (mActive == 0) {
         if (MODE >= 1 && valid) A.h_next[k] = A.h[k];
@@ -247,7 +238,6 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     // outliers: a few active particles with h above NN_OUTLIER x the group's smallest would
     // grow the search box of all 64; they go to the per-particle mode instead
     bool outlier = false;
-    bool deferred = false;   // some lane of this group went to the per-particle mode
     if (A.out_list) {
         const double hmin = wave_min(active ? hi : 1e300);
         const bool big = active && hi > NN_OUTLIER * hmin;
@@ -260,7 +250,6 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             if (outlier) A.out_list[b + popc64(mO & lanemask_lt())] = (int32_t)k;
             active = active && !outlier;
             mActive = wave_ballot(active);
-            deferred = true;
         }
     }
     // search box: bounding box of the active lanes grown by their largest h (plus a
@@ -274,14 +263,9 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     int count = 0;
     int b0 = 0, b1 = 0, b2 = 0;   // pending list entries, flushed four at a time as one int4
     int4 *wp = A.nbr4 + k;        // where the next four go
-    uint32_t u0 = 0, u1 = 0, u2 = 0, u3 = 0;   // pending union positions, eight per chunk
-    uint4 *up = A.nslot8 + k;
-    int ubase = 0;                // union members so far (wave-uniform)
-    const bool track = MODE != 1 && A.gunion != nullptr;
-    const uint64_t lt = lanemask_lt();
     const bool ok = nn_walk_box(L, A, lox, hix, loy, hiy, loz, hiz, [&](int m) {
+        if (!active) return;
         if (MODE == 1) {
-            if (!active) return;
             for (int e = 0; e < m; e++) {
                 const double dx = xi - L.tx[e];
                 const double dy = yi - L.ty[e];
@@ -291,61 +275,26 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             }
             return;
         }
-        // list modes: test the tile into a hit mask, OR the lanes' masks into the tile's
-        // union, record the new union members, then append each lane's hits in candidate
-        // order (global index, and position in the union)
-        uint64_t hits = 0;
-        if (active) {
-            for (int e = 0; e < m; e++) {
-                const double dx = xi - L.tx[e];
-                const double dy = yi - L.ty[e];
-                const double dz = zi - L.tz[e];
-                const double d = dx * dx + dy * dy + dz * dz;
-                const bool hit = (d < hi2) & (d < L.th2[e]) & (L.tj[e] != (int)k);
-                hits |= (uint64_t)hit << e;
-            }
-        }
-        uint64_t um = hits;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) um |= __shfl_xor(um, o);
-        if (track && ((um >> lane) & 1ull)) {
-            const int t = ubase + popc64(um & lt);
-            if (t < NN_UCAP) A.gunion[group * NN_UCAP + t] = L.tj[lane];
-        }
-        while (hits) {
-            const int e = __builtin_ctzll(hits);
-            hits &= hits - 1;
+        // list modes: the append is predicated (selects into the three pending entries);
+        // only the 16-byte store of every fourth entry is a branch
+        for (int e = 0; e < m; e++) {
+            const double dx = xi - L.tx[e];
+            const double dy = yi - L.ty[e];
+            const double dz = zi - L.tz[e];
+            const double d = dx * dx + dy * dy + dz * dz;
             const int j = L.tj[e];
-            const int q4 = count & 3;
-            b0 = q4 == 0 ? j : b0;
-            b1 = q4 == 1 ? j : b1;
-            b2 = q4 == 2 ? j : b2;
-            if (q4 == 3) {
+            const bool hit = (d < hi2) & (d < L.th2[e]) & (j != (int)k);
+            const int slot = count & 3;
+            b0 = (hit && slot == 0) ? j : b0;
+            b1 = (hit && slot == 1) ? j : b1;
+            b2 = (hit && slot == 2) ? j : b2;
+            if (hit && slot == 3) {
                 if (count < A.max_nb) *wp = make_int4(b0, b1, b2, j);
                 wp += A.stride;
             }
-            if (track) {
-                const uint32_t u = (uint32_t)(ubase + popc64(um & ((1ull << e) - 1ull))) & 0xffffu;
-                const int q8 = count & 7;
-                const uint32_t sh = (q8 & 1) ? (u << 16) : u;
-                const int wd = q8 >> 1;
-                u0 = wd == 0 ? ((q8 & 1) ? (u0 | sh) : sh) : u0;
-                u1 = wd == 1 ? ((q8 & 1) ? (u1 | sh) : sh) : u1;
-                u2 = wd == 2 ? ((q8 & 1) ? (u2 | sh) : sh) : u2;
-                u3 = wd == 3 ? ((q8 & 1) ? (u3 | sh) : sh) : u3;
-                if (q8 == 7) {
-                    if ((count >> 3) < ((A.max_nb + 7) >> 3)) *up = make_uint4(u0, u1, u2, u3);
-                    up += A.stride;
-                }
-            }
-            count++;
+            count += hit ? 1 : 0;
         }
-        ubase += popc64(um);
     });
-    if (track && ok) {
-        if (active && (count & 7) != 0 && (count >> 3) < ((A.max_nb + 7) >> 3)) *up = make_uint4(u0, u1, u2, u3);
-        if (lane == 0) A.gucount[group] = (!deferred && ubase <= NN_UCAP) ? ubase : -1;
-    }
     if (!ok) {
         if (lane == 0) set_err(A.err, ERRF_STACK);
         return;
@@ -586,14 +535,6 @@ static NNArgs nn_common(mhpc_ctx *c) {
     return a;
 }
 
-int nn_union_capacity() { return NN_UCAP; }
-
-static void set_union_args(mhpc_ctx *c, NNArgs &a) {
-    a.gunion = c->gunion;
-    a.gucount = c->gucount;
-    a.nslot8 = reinterpret_cast<uint4 *>(c->nslot);
-}
-
 static unsigned nn_grid(mhpc_ctx *c) { return (unsigned)ceil_div(ceil_div(c->n, 64), NN_WAVES); }
 
 // done flags and iteration counts to zero; the per-pass list counters to zero
@@ -640,7 +581,6 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
         a.nbr4 = reinterpret_cast<int4 *>(c->nbr);
         a.stride = c->cap_pad;
         a.hmax0_bits = c->hmax_bits;
-        set_union_args(c, a);
         hipLaunchKernelGGL(k_nn_walk<2>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
     } else {
         hipLaunchKernelGGL(k_nn_walk<1>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
@@ -690,7 +630,6 @@ void launch_neighbours_fixup(mhpc_ctx *c) {
     NNArgs b = nn_common(c);
     b.h = c->sh;
     b.nbr4 = reinterpret_cast<int4 *>(c->nbr);
-    set_union_args(c, b);
     b.stride = c->cap_pad;
     b.snoi = c->snoi;
     b.only = c->sdirty;
@@ -714,7 +653,6 @@ void launch_neighbours(mhpc_ctx *c) {
     a.h = c->sh;
     a.nbr4 = reinterpret_cast<int4 *>(c->nbr);
     a.stride = c->cap_pad;
-    set_union_args(c, a);
     a.snoi = c->snoi;
     a.noi_out = c->noi;
     a.sml_out = nullptr;
@@ -779,99 +717,72 @@ __device__ __forceinline__ double rsq_nr(double x) {
     return fma(y, e, y);
 }
 
-// one density pair term: FP64 adds m_j W into acc; FP32 adds it (in FP32, from FP64
-// differences) into the Kahan accumulator acc32
 template <bool FP64>
-__device__ __forceinline__ void density_pair(const double4 &pi, double hi, const double4 &pj, double hj,
-                                             int sml_avg, double &acc, Kahan &acc32) {
-    if constexpr (FP64) {
-        const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
-        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double ih = rcp_nr(hb);
-        const double q = r2 * rsq_nr(r2) * ih;
-        const double f = K8PI * (ih * ih * ih);
-        double W = 0.0;
-        if (q <= 0.5) W = f * fma(6.0 * q * q, q - 1.0, 1.0);
-        else if (q <= 1.0) { const double t = 1.0 - q; W = 2.0 * f * (t * t * t); }
-        acc = fma(pj.w, W, acc);
-    } else {
-        const float hb = (float)(sml_avg ? 0.5 * (hi + hj) : hi);
-        const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y), dz = (float)(pi.z - pj.z);
-        const float rr = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
-        float W, dW;
-        cubic32(rr, hb, W, dW);
-        acc32.add((float)pj.w * W);
-    }
-}
-
-// the position in the group's union of list entry s (16-bit fields, eight per chunk)
-__device__ __forceinline__ int union_slot(const uint4 &c, int s) {
-    const int w = (s & 7) >> 1;
-    const uint32_t v = w == 0 ? c.x : (w == 1 ? c.y : (w == 2 ? c.z : c.w));
-    return (int)((s & 1) ? (v >> 16) : (v & 0xffffu));
-}
-
-// Neighbour staging.  Per-lane gathers of neighbour records touch ~50 distinct cache lines
-// per wave instruction, which bounds the pair loops by the L1 address path.  One wavefront
-// owns the 64 consecutive sorted particles of a group, whose list walk recorded the union
-// of their lists (in candidate order) and each entry's position in it: the union's records
-// are fetched once into LDS and the pair loop reads them there.  The per-lane order and
-// arithmetic are those of the list, so the result is bit-identical to gathering; groups
-// without a recorded union (outliers, union above NN_UCAP) gather.
-struct DensStage {
-    double4 p[NN_UCAP];
-    double h[NN_UCAP];
-};
-
-template <bool FP64>
-__global__ __launch_bounds__(64) void k_density(int64_t n, const double4 *__restrict__ sxm,
-                                                const double *__restrict__ sh, const int32_t *__restrict__ snoi,
-                                                const int32_t *__restrict__ nbr, const uint4 *__restrict__ nslot8,
-                                                const int32_t *__restrict__ gunion,
-                                                const int32_t *__restrict__ gucount, int64_t stride, int max_nb,
-                                                int sml_avg, const int32_t *__restrict__ perm,
-                                                double *__restrict__ srho, double *__restrict__ rho, uint32_t *err) {
-    __shared__ DensStage L;
-    const int lane = wave_lane();
-    const int64_t group = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t k = group * 64 + lane;
-    if (group * 64 >= n) return;
-    const int U = gucount[group];
-    if (U > 0) {
-        for (int t = lane; t < U; t += 64) {
-            const int j = gunion[group * NN_UCAP + t];
-            L.p[t] = sxm[j];
-            L.h[t] = sh[j];
-        }
-        wave_sync();
-    }
+__global__ __launch_bounds__(256) void k_density(int64_t n, const double4 *__restrict__ sxm,
+                                                 const double *__restrict__ sh,
+                                                 const int32_t *__restrict__ snoi,
+                                                 const int32_t *__restrict__ nbr, int64_t stride,
+                                                 int max_nb, int sml_avg, const int32_t *__restrict__ perm,
+                                                 double *__restrict__ srho, double *__restrict__ rho,
+                                                 uint32_t *err) {
+    const int64_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double4 pi = sxm[k];
     const double hi = sh[k];
     int cnt = snoi[k];
     cnt = cnt > max_nb ? max_nb : cnt;
-    double acc = 0.0;
-    Kahan acc32;
-    if (U >= 0) {
-        uint4 c8 = make_uint4(0, 0, 0, 0);
-        for (int s = 0; s < cnt; s++) {
-            if ((s & 7) == 0) c8 = nslot8[(int64_t)(s >> 3) * stride + k];
-            int u = union_slot(c8, s);
-            u = u < U ? u : U - 1;   // memory safety only: a valid list never exceeds U
-            density_pair<FP64>(pi, hi, L.p[u], L.h[u], sml_avg, acc, acc32);
-        }
-    } else {
-        const int4 *nbr4 = reinterpret_cast<const int4 *>(nbr);
+    double r;
+    const int4 *nbr4 = reinterpret_cast<const int4 *>(nbr);
+    if constexpr (FP64) {
+        // the next neighbour's {x,y,z,m} and h are loaded while the current one is summed
+        double acc = 0.0;
         int4 q4 = make_int4(0, 0, 0, 0);
-        for (int s = 0; s < cnt; s++) {
-            if ((s & 3) == 0) q4 = nbr4[(int64_t)(s >> 2) * stride + k];
-            const int q = s & 3;
-            const int32_t j = q == 0 ? q4.x : (q == 1 ? q4.y : (q == 2 ? q4.z : q4.w));
-            density_pair<FP64>(pi, hi, sxm[j], sh[j], sml_avg, acc, acc32);
+        double4 pn = make_double4(0.0, 0.0, 0.0, 0.0);
+        double hn = 0.0;
+        if (cnt > 0) {
+            q4 = nbr4[k];
+            pn = sxm[q4.x];
+            hn = sh[q4.x];
         }
+        for (int s = 0; s < cnt; s++) {
+            const double4 pj = pn;
+            const double hj = hn;
+            if (s + 1 < cnt) {
+                const int u = (s + 1) & 3;
+                if (u == 0) q4 = nbr4[(int64_t)((s + 1) >> 2) * stride + k];
+                const int32_t jn = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
+                pn = sxm[jn];
+                hn = sh[jn];
+            }
+            const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            const double ih = rcp_nr(hb);
+            const double q = r2 * rsq_nr(r2) * ih;
+            const double f = K8PI * (ih * ih * ih);
+            double W = 0.0;
+            if (q <= 0.5) W = f * fma(6.0 * q * q, q - 1.0, 1.0);
+            else if (q <= 1.0) { const double t = 1.0 - q; W = 2.0 * f * (t * t * t); }
+            acc = fma(pj.w, W, acc);
+        }
+        r = pi.w * (K8PI / (hi * hi * hi)) + acc;
+    } else {
+        Kahan acc;
+        for (int s = 0; s < cnt; s++) {
+            const int4 q4 = nbr4[(int64_t)(s >> 2) * stride + k];
+            const int u = s & 3;
+            const int32_t j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
+            const double4 pj = sxm[j];
+            const double hj = sh[j];
+            const float hb = (float)(sml_avg ? 0.5 * (hi + hj) : hi);
+            const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y), dz = (float)(pi.z - pj.z);
+            const float rr = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+            float W, dW;
+            cubic32(rr, hb, W, dW);
+            acc.add((float)pj.w * W);
+        }
+        r = pi.w * (K8PI / (hi * hi * hi)) + acc.value();
     }
-    const double r = pi.w * (K8PI / (hi * hi * hi)) + (FP64 ? acc : acc32.value());
     srho[k] = r;
     rho[perm[k]] = r;
     if (!(r > 0.0)) set_err(err, ERRF_RHO);
@@ -879,15 +790,14 @@ __global__ __launch_bounds__(64) void k_density(int64_t n, const double4 *__rest
 
 void launch_density(mhpc_ctx *c) {
     if (c->n <= 0) return;
-    const unsigned g = (unsigned)ceil_div(c->n, 64);
-    const uint4 *ns = reinterpret_cast<const uint4 *>(c->nslot);
+    const unsigned g = (unsigned)ceil_div(c->n, 256);
     if (c->p.precision == MHPC_PREC_FP64)
-        hipLaunchKernelGGL(k_density<true>, dim3(g), dim3(64), 0, c->stream, c->n, c->sxm, c->sh, c->snoi, c->nbr,
-                           ns, c->gunion, c->gucount, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
+        hipLaunchKernelGGL(k_density<true>, dim3(g), dim3(256), 0, c->stream, c->n, c->sxm, c->sh, c->snoi,
+                           c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
                            c->f[MHPC_F_RHO], c->err);
     else
-        hipLaunchKernelGGL(k_density<false>, dim3(g), dim3(64), 0, c->stream, c->n, c->sxm, c->sh, c->snoi, c->nbr,
-                           ns, c->gunion, c->gucount, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
+        hipLaunchKernelGGL(k_density<false>, dim3(g), dim3(256), 0, c->stream, c->n, c->sxm, c->sh, c->snoi,
+                           c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
                            c->f[MHPC_F_RHO], c->err);
 }
 
@@ -970,193 +880,147 @@ struct ForceArgs {
     const int32_t *perm;
     double *ax, *ay, *az, *drhodt, *dedt, *dsmldt, *muijmax;
     double *curlv, *gradrho;   // learned-h features (FEAT)
-    const int32_t *gunion, *gucount;   // the groups' neighbour unions (see the density kernel)
-    const uint4 *nslot8;
 };
 
-// Per-lane force accumulators.  FEAT also accumulates curl v = sum (m_j/rho_j) grad W x
-// (v_j - v_i) and grad rho = sum m_j grad W, the learned-h features (plain sums in the pair
-// precision).
-template <bool FP64>
-struct ForceAcc;
-template <>
-struct ForceAcc<true> {
-    double a0 = 0, a1 = 0, a2 = 0, dvv = 0, de = 0;
-};
-template <>
-struct ForceAcc<false> {
-    Kahan a0, a1, a2, drho, de, dsml;
-};
-
-struct ForceI {   // particle i's data and the constants of the pair loop
-    double4 p, v, r;
-    double h;
-    int sml_avg;
-    double alpha, beta;
-};
-
-// one pair (i, j) of the force loop; j's records {x,y,z,m}, {vx,vy,vz,cs},
-// {rho, p/rho^2, m/rho, h}
+// FEAT also accumulates curl v = sum (m_j/rho_j) grad W x (v_j - v_i) and
+// grad rho = sum m_j grad W, the learned-h features (plain sums in the pair precision)
 template <bool FP64, bool FEAT>
-__device__ __forceinline__ void force_pair(const ForceI &I, const double4 &pj, const double4 &vj, const double4 &rj,
-                                           ForceAcc<FP64> &acc, double &mumax, double *feat) {
-    const double4 pi = I.p, vi = I.v, ri = I.r;
-    if constexpr (FP64) {
-        const double hi = I.h;
-        const double hj = rj.w;
-        const double hb = 0.5 * (hi + hj);
-        const double sml = I.sml_avg ? hb : hi;
-        const double d0 = pi.x - pj.x, d1 = pi.y - pj.y, d2 = pi.z - pj.z;
-        const double r2 = fma(d0, d0, fma(d1, d1, d2 * d2));
-        const double rinv = rsq_nr(r2);
-        const double ih = rcp_nr(sml);
-        const double q = r2 * rinv * ih;
-        const double ih2 = ih * ih;
-        const double fih = K8PI * (ih2 * ih2);
-        // grad W = (dW/dr / r) r_ij; for q <= 1/2, dW/dr / r = 6 f/h (3q - 2) q / r with
-        // q / r = 1/h
-        double gs = 0.0;
-        if (q <= 0.5) gs = fih * ih * fma(18.0, q, -12.0);
-        else if (q <= 1.0) { const double t = 1.0 - q; gs = -6.0 * fih * (t * t) * rinv; }
-        const double g0 = gs * d0, g1 = gs * d1, g2 = gs * d2;
-        const double v0 = vi.x - vj.x, v1 = vi.y - vj.y, v2 = vi.z - vj.z;
-        const double vv = fma(v0, g0, fma(v1, g1, v2 * g2));
-        const double vr = fma(v0, d0, fma(v1, d1, v2 * d2));
-        double mpij = 0.0;
-        if (vr < 0) {
-            const double csbar = 0.5 * (vi.w + vj.w);
-            const double mu = hb * vr * rcp_nr(fma(hb * hb, 1e-2, r2));
-            mumax = fmax(mumax, mu);
-            const double rhobar = 0.5 * (ri.x + rj.x);
-            mpij = pj.w * fma(I.beta, mu, -I.alpha * csbar) * mu * rcp_nr(rhobar);
-        }
-        const double pf = -pj.w * (ri.y + rj.y);
-        const double fa = pf - mpij;
-        acc.a0 = fma(fa, g0, acc.a0);
-        acc.a1 = fma(fa, g1, acc.a1);
-        acc.a2 = fma(fa, g2, acc.a2);
-        acc.dvv = fma(rj.z, vv, acc.dvv);   // the drho/dt and dh/dt sums are the same sum
-        acc.de = fma(mpij - pf, vv, acc.de);
-        if (FEAT) {
-            feat[0] += rj.z * (v1 * g2 - v2 * g1);
-            feat[1] += rj.z * (v2 * g0 - v0 * g2);
-            feat[2] += rj.z * (v0 * g1 - v1 * g0);
-            feat[3] += pj.w * g0;
-            feat[4] += pj.w * g1;
-            feat[5] += pj.w * g2;
-        }
-    } else {
-        const float hif = (float)I.h, rhoif = (float)ri.x, csif = (float)vi.w, prif = (float)ri.y;
-        const float alpha = (float)I.alpha, beta = (float)I.beta;
-        const float hj = (float)rj.w;
-        const float hb = 0.5f * (hif + hj);
-        const float sml = I.sml_avg ? hb : hif;
-        const float d0 = (float)(pi.x - pj.x), d1 = (float)(pi.y - pj.y), d2 = (float)(pi.z - pj.z);
-        const float v0 = (float)(vi.x - vj.x), v1 = (float)(vi.y - vj.y), v2 = (float)(vi.z - vj.z);
-        const float r2 = __builtin_fmaf(d0, d0, __builtin_fmaf(d1, d1, d2 * d2));
-        const float r = sqrtf(r2);
-        float W, dWdr;
-        cubic32(r, sml, W, dWdr);
-        const float gs = dWdr / r;
-        const float g0 = gs * d0, g1 = gs * d1, g2 = gs * d2;
-        const float vv = __builtin_fmaf(v0, g0, __builtin_fmaf(v1, g1, v2 * g2));
-        const float vr = __builtin_fmaf(v0, d0, __builtin_fmaf(v1, d1, v2 * d2));
-        const float mj = (float)pj.w;
-        float mpij = 0.f;
-        if (vr < 0.f) {
-            const float csbar = 0.5f * (csif + (float)vj.w);
-            const float mu = hb * vr / __builtin_fmaf(hb * hb, 1e-2f, r2);
-            mumax = fmax(mumax, (double)mu);
-            const float rhobar = 0.5f * (rhoif + (float)rj.x);
-            mpij = mj * (beta * mu - alpha * csbar) * mu / rhobar;
-        }
-        const float pf = -mj * (prif + (float)rj.y);
-        const float fa = pf - mpij;
-        acc.a0.add(fa * g0);
-        acc.a1.add(fa * g1);
-        acc.a2.add(fa * g2);
-        const float mr_vv = (float)rj.z * vv;
-        acc.drho.add(mr_vv);
-        acc.dsml.add(mr_vv);
-        acc.de.add((mpij - pf) * vv);
-        if (FEAT) {
-            const float vjr = (float)rj.z;
-            feat[0] += vjr * (v1 * g2 - v2 * g1);
-            feat[1] += vjr * (v2 * g0 - v0 * g2);
-            feat[2] += vjr * (v0 * g1 - v1 * g0);
-            feat[3] += mj * g0;
-            feat[4] += mj * g1;
-            feat[5] += mj * g2;
-        }
-    }
-}
-
-struct ForceStage {
-    double4 p[NN_UCAP], v[NN_UCAP], r[NN_UCAP];
-};
-
-// One wavefront per group of 64 sorted particles; the group's neighbour union is staged in
-// LDS (see the density kernel), per-lane gathers where the group has none.
-template <bool FP64, bool FEAT>
-__global__ __launch_bounds__(64) void k_forces(ForceArgs A) {
-    __shared__ ForceStage L;
-    const int lane = wave_lane();
-    const int64_t group = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t k = group * 64 + lane;
-    if (group * 64 >= A.n) return;
-    const int U = A.gucount[group];
-    if (U > 0) {
-        for (int t = lane; t < U; t += 64) {
-            const int j = A.gunion[group * NN_UCAP + t];
-            L.p[t] = A.sxm[j];
-            L.v[t] = A.sv4[j];
-            L.r[t] = A.sr4[j];
-        }
-        wave_sync();
-    }
+__global__ __launch_bounds__(256) void k_forces(ForceArgs A) {
+    const int64_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= A.n) return;
-    ForceI I;
-    I.p = A.sxm[k];
-    I.v = A.sv4[k];
-    I.r = A.sr4[k];
-    I.h = A.sh[k];
-    I.sml_avg = A.sml_avg;
-    I.alpha = A.alpha;
-    I.beta = A.beta;
+    const double4 pi = A.sxm[k], vi = A.sv4[k], ri = A.sr4[k];
+    const double hi = A.sh[k];
     int cnt = A.snoi[k];
     cnt = cnt > A.max_nb ? A.max_nb : cnt;
-    ForceAcc<FP64> acc;
+    double out[6];
     double mumax = 0.0;
-    double feat[6] = {0, 0, 0, 0, 0, 0};
-    if (U >= 0) {
-        uint4 c8 = make_uint4(0, 0, 0, 0);
-        for (int s = 0; s < cnt; s++) {
-            if ((s & 7) == 0) c8 = A.nslot8[(int64_t)(s >> 3) * A.stride + k];
-            int u = union_slot(c8, s);
-            u = u < U ? u : U - 1;   // memory safety only
-            force_pair<FP64, FEAT>(I, L.p[u], L.v[u], L.r[u], acc, mumax, feat);
+    double c0 = 0, c1 = 0, c2 = 0, gr0 = 0, gr1 = 0, gr2 = 0;
+    if constexpr (FP64) {
+        // pair loop: the j records {x,y,z,m}, {vx,vy,vz,cs}, {rho, p/rho^2, m/rho, h} of the
+        // next neighbour are loaded while the current one is evaluated
+        double a0 = 0, a1 = 0, a2 = 0, dvv = 0, de = 0;
+        const int4 *nbr4 = reinterpret_cast<const int4 *>(A.nbr);
+        int4 q4 = make_int4(0, 0, 0, 0);
+        double4 pn = make_double4(0.0, 0.0, 0.0, 0.0), vn = pn, rn = pn;
+        if (cnt > 0) {
+            q4 = nbr4[k];
+            pn = A.sxm[q4.x];
+            vn = A.sv4[q4.x];
+            rn = A.sr4[q4.x];
         }
+        for (int s = 0; s < cnt; s++) {
+            const double4 pj = pn, vj = vn, rj = rn;
+            if (s + 1 < cnt) {
+                const int u = (s + 1) & 3;
+                if (u == 0) q4 = nbr4[(int64_t)((s + 1) >> 2) * A.stride + k];
+                const int32_t jn = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
+                pn = A.sxm[jn];
+                vn = A.sv4[jn];
+                rn = A.sr4[jn];
+            }
+            const double hj = rj.w;
+            const double hb = 0.5 * (hi + hj);
+            const double sml = A.sml_avg ? hb : hi;
+            const double d0 = pi.x - pj.x, d1 = pi.y - pj.y, d2 = pi.z - pj.z;
+            const double r2 = fma(d0, d0, fma(d1, d1, d2 * d2));
+            const double rinv = rsq_nr(r2);
+            const double ih = rcp_nr(sml);
+            const double q = r2 * rinv * ih;
+            const double ih2 = ih * ih;
+            const double fih = K8PI * (ih2 * ih2);
+            // grad W = (dW/dr / r) r_ij; for q <= 1/2, dW/dr / r = 6 f/h (3q - 2) q / r with
+            // q / r = 1/h
+            double gs = 0.0;
+            if (q <= 0.5) gs = fih * ih * fma(18.0, q, -12.0);
+            else if (q <= 1.0) { const double t = 1.0 - q; gs = -6.0 * fih * (t * t) * rinv; }
+            const double g0 = gs * d0, g1 = gs * d1, g2 = gs * d2;
+            const double v0 = vi.x - vj.x, v1 = vi.y - vj.y, v2 = vi.z - vj.z;
+            const double vv = fma(v0, g0, fma(v1, g1, v2 * g2));
+            const double vr = fma(v0, d0, fma(v1, d1, v2 * d2));
+            double mpij = 0.0;
+            if (vr < 0) {
+                const double csbar = 0.5 * (vi.w + vj.w);
+                const double mu = hb * vr * rcp_nr(fma(hb * hb, 1e-2, r2));
+                mumax = fmax(mumax, mu);
+                const double rhobar = 0.5 * (ri.x + rj.x);
+                mpij = pj.w * fma(A.beta, mu, -A.alpha * csbar) * mu * rcp_nr(rhobar);
+            }
+            const double pf = -pj.w * (ri.y + rj.y);
+            const double fa = pf - mpij;
+            a0 = fma(fa, g0, a0);
+            a1 = fma(fa, g1, a1);
+            a2 = fma(fa, g2, a2);
+            dvv = fma(rj.z, vv, dvv);   // the drho/dt and dh/dt sums are the same sum
+            de = fma(mpij - pf, vv, de);
+            if (FEAT) {
+                c0 += rj.z * (v1 * g2 - v2 * g1);
+                c1 += rj.z * (v2 * g0 - v0 * g2);
+                c2 += rj.z * (v0 * g1 - v1 * g0);
+                gr0 += pj.w * g0;
+                gr1 += pj.w * g1;
+                gr2 += pj.w * g2;
+            }
+        }
+        out[0] = a0; out[1] = a1; out[2] = a2;
+        out[3] = ri.x * dvv;
+        out[4] = 0.5 * de;
+        out[5] = -(1.0 / 3.0) * hi * dvv;
     } else {
+        Kahan a0, a1, a2, drho, de, dsml;
+        const float hif = (float)hi, rhoif = (float)ri.x, csif = (float)vi.w, prif = (float)ri.y;
+        const float alpha = (float)A.alpha, beta = (float)A.beta;
         const int4 *nbr4 = reinterpret_cast<const int4 *>(A.nbr);
         int4 q4 = make_int4(0, 0, 0, 0);
         for (int s = 0; s < cnt; s++) {
             if ((s & 3) == 0) q4 = nbr4[(int64_t)(s >> 2) * A.stride + k];
-            const int q = s & 3;
-            const int32_t j = q == 0 ? q4.x : (q == 1 ? q4.y : (q == 2 ? q4.z : q4.w));
-            force_pair<FP64, FEAT>(I, A.sxm[j], A.sv4[j], A.sr4[j], acc, mumax, feat);
+            const int u = s & 3;
+            const int32_t j = u == 0 ? q4.x : (u == 1 ? q4.y : (u == 2 ? q4.z : q4.w));
+            const double4 pj = A.sxm[j], vj = A.sv4[j], rj = A.sr4[j];
+            const float hj = (float)rj.w;
+            const float hb = 0.5f * (hif + hj);
+            const float sml = A.sml_avg ? hb : hif;
+            const float d0 = (float)(pi.x - pj.x), d1 = (float)(pi.y - pj.y), d2 = (float)(pi.z - pj.z);
+            const float v0 = (float)(vi.x - vj.x), v1 = (float)(vi.y - vj.y), v2 = (float)(vi.z - vj.z);
+            const float r2 = __builtin_fmaf(d0, d0, __builtin_fmaf(d1, d1, d2 * d2));
+            const float r = sqrtf(r2);
+            float W, dWdr;
+            cubic32(r, sml, W, dWdr);
+            const float gs = dWdr / r;
+            const float g0 = gs * d0, g1 = gs * d1, g2 = gs * d2;
+            const float vv = __builtin_fmaf(v0, g0, __builtin_fmaf(v1, g1, v2 * g2));
+            const float vr = __builtin_fmaf(v0, d0, __builtin_fmaf(v1, d1, v2 * d2));
+            const float mj = (float)pj.w;
+            float mpij = 0.f;
+            if (vr < 0.f) {
+                const float csbar = 0.5f * (csif + (float)vj.w);
+                const float mu = hb * vr / __builtin_fmaf(hb * hb, 1e-2f, r2);
+                mumax = fmax(mumax, (double)mu);
+                const float rhobar = 0.5f * (rhoif + (float)rj.x);
+                mpij = mj * (beta * mu - alpha * csbar) * mu / rhobar;
+            }
+            const float pf = -mj * (prif + (float)rj.y);
+            const float fa = pf - mpij;
+            a0.add(fa * g0);
+            a1.add(fa * g1);
+            a2.add(fa * g2);
+            const float mr_vv = (float)rj.z * vv;
+            drho.add(mr_vv);
+            dsml.add(mr_vv);
+            de.add((mpij - pf) * vv);
+            if (FEAT) {
+                const float vj = (float)rj.z;
+                c0 += vj * (v1 * g2 - v2 * g1);
+                c1 += vj * (v2 * g0 - v0 * g2);
+                c2 += vj * (v0 * g1 - v1 * g0);
+                gr0 += mj * g0;
+                gr1 += mj * g1;
+                gr2 += mj * g2;
+            }
         }
-    }
-    double out[6];
-    if constexpr (FP64) {
-        out[0] = acc.a0; out[1] = acc.a1; out[2] = acc.a2;
-        out[3] = I.r.x * acc.dvv;
-        out[4] = 0.5 * acc.de;
-        out[5] = -(1.0 / 3.0) * I.h * acc.dvv;
-    } else {
-        out[0] = acc.a0.value(); out[1] = acc.a1.value(); out[2] = acc.a2.value();
-        out[3] = I.r.x * acc.drho.value();
-        out[4] = 0.5 * acc.de.value();
-        out[5] = -(1.0 / 3.0) * I.h * acc.dsml.value();
+        out[0] = a0.value(); out[1] = a1.value(); out[2] = a2.value();
+        out[3] = ri.x * drho.value();
+        out[4] = 0.5 * de.value();
+        out[5] = -(1.0 / 3.0) * hi * dsml.value();
     }
     const int32_t i = A.perm[k];
     A.ax[i] = out[0];
@@ -1167,8 +1031,8 @@ __global__ __launch_bounds__(64) void k_forces(ForceArgs A) {
     A.dsmldt[i] = out[5];
     A.muijmax[i] = mumax;
     if (FEAT) {
-        A.curlv[i] = sqrt(feat[0] * feat[0] + feat[1] * feat[1] + feat[2] * feat[2]);
-        A.gradrho[i] = sqrt(feat[3] * feat[3] + feat[4] * feat[4] + feat[5] * feat[5]);
+        A.curlv[i] = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+        A.gradrho[i] = sqrt(gr0 * gr0 + gr1 * gr1 + gr2 * gr2);
     }
 }
 
@@ -1186,16 +1050,13 @@ void launch_forces(mhpc_ctx *c, bool features) {
     a.muijmax = c->f[MHPC_F_MUIJMAX];
     a.curlv = c->f[MHPC_F_CURLV];
     a.gradrho = c->f[MHPC_F_GRADRHO];
-    a.gunion = c->gunion;
-    a.gucount = c->gucount;
-    a.nslot8 = reinterpret_cast<const uint4 *>(c->nslot);
-    const unsigned g = (unsigned)ceil_div(c->n, 64);
+    const unsigned g = (unsigned)ceil_div(c->n, 256);
     if (c->p.precision == MHPC_PREC_FP64) {
-        if (features) hipLaunchKernelGGL((k_forces<true, true>), dim3(g), dim3(64), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_forces<true, false>), dim3(g), dim3(64), 0, c->stream, a);
+        if (features) hipLaunchKernelGGL((k_forces<true, true>), dim3(g), dim3(256), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_forces<true, false>), dim3(g), dim3(256), 0, c->stream, a);
     } else {
-        if (features) hipLaunchKernelGGL((k_forces<false, true>), dim3(g), dim3(64), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_forces<false, false>), dim3(g), dim3(64), 0, c->stream, a);
+        if (features) hipLaunchKernelGGL((k_forces<false, true>), dim3(g), dim3(256), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_forces<false, false>), dim3(g), dim3(256), 0, c->stream, a);
     }
 }
 

@@ -109,8 +109,6 @@ struct mhpc_ctx {
     int32_t *hlist[3] = {nullptr, nullptr, nullptr};   // h-iteration: unconverged particles per pass
     uint8_t *sdirty = nullptr;                   // speculative lists: particles to re-list
     int32_t *olist = nullptr;                    // neighbour search: group-mode outliers
-    int32_t *gunion = nullptr, *gucount = nullptr;   // per group of 64: union of the lists
-    uint16_t *nslot = nullptr;                   // per list entry: position in the union
     uint32_t *ocount = nullptr;
     unsigned long long *hmax_bits = nullptr;     // largest h (bit pattern)
     uint32_t *hlist_count = nullptr;             // [MHPC_MAX_PASSES + 1]
@@ -155,7 +153,6 @@ void launch_gravity(mhpc_ctx *c, bool potential);
 void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative);
 void launch_neighbours(mhpc_ctx *c);
 void launch_neighbours_fixup(mhpc_ctx *c);
-int nn_union_capacity();
 void launch_density(mhpc_ctx *c);
 void launch_eos(mhpc_ctx *c);
 void launch_forces(mhpc_ctx *c, bool features);
