@@ -107,7 +107,7 @@ static void free_all(mhpc_ctx *c) {
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
                     c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->hslot, c->sdirty,
-                    c->hmax_bits, c->olist, c->ocount};
+                    c->hmax_bits, c->olist, c->ocount, c->hcnt};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
@@ -165,6 +165,7 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->ocount, 4));
     HIPC(dalloc(&c->hlist_count, MHPC_MAX_PASSES + 1));
     HIPC(dalloc(&c->hslot, cap));
+    HIPC(dalloc(&c->hcnt, cap));
     HIPC(dalloc(&c->lcp, cap));
     c->tree_tiles_cap = ceil_div(cap, MHPC_TREE_TILE);
     HIPC(dalloc(&c->tcnt, (MHPC_LEVELS + 1) * c->tree_tiles_cap));

@@ -62,6 +62,7 @@ struct NNArgs {
     int32_t *list_out;
     uint32_t *count_out;
     double *h_slot;           // per-particle mode: new h of list slot t
+    uint32_t *cnt_slot;       // split counting: neighbour count of list slot t
     const uint8_t *only;      // list mode: when set, only particles with only[k] != 0
     uint8_t *dirty;           // dirty marking: particles whose list must be rebuilt
     const double *hmax_p;     // dirty marking: largest pass-0 h
@@ -72,17 +73,18 @@ struct NNArgs {
     unsigned long long *counters;
 };
 
-// Walk the tree for the axis-aligned box [lo, hi] and hand the candidate particles to
+// Walk the (sub)tree of node `start` (the root is node n) for the axis-aligned box [lo, hi]
+// and hand the candidate particles to
 // `tile` in LDS tiles of up to 64 (L.tx/ty/tz/th2/tj, count m).  Leaves of at most
 // NN_LEAF particles and single particles are taken as ranges of sorted positions; larger
 // cells are opened.  Returns false on stack overflow.  Wave-uniform control flow.
 template <typename TileFn>
 __device__ __forceinline__ bool nn_walk_box(NNWaveLds &L, const NNArgs &A, double lox, double hix, double loy,
-                                            double hiy, double loz, double hiz, TileFn &&tile) {
+                                            double hiy, double loz, double hiz, int start, TileFn &&tile) {
     const int lane = wave_lane();
     const int64_t n = A.n;
     int top = 1, nr = 0;
-    if (lane == 0) L.stack[0] = (int)n;
+    if (lane == 0) L.stack[0] = start;
     wave_sync();
     const uint64_t lt = lanemask_lt();
     for (;;) {
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     int count = 0;
     int b0 = 0, b1 = 0, b2 = 0;   // pending list entries, flushed four at a time as one int4
     int4 *wp = A.nbr4 + k;        // where the next four go
-    const bool ok = nn_walk_box(L, A, lox, hix, loy, hiy, loz, hiz, [&](int m) {
+    const bool ok = nn_walk_box(L, A, lox, hix, loy, hiy, loz, hiz, (int)n, [&](int m) {
         if (!active) return;
         if (MODE == 1) {
             for (int e = 0; e < m; e++) {
@@ -389,7 +391,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
         const double grow = hi * (1.0 + 1e-9);
         int count = 0;
         const bool ok = nn_walk_box(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, [&](int m) {
+                                    own.z + grow, (int)A.n, [&](int m) {
             bool hit = false;
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
@@ -454,7 +456,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
         const double4 own = A.sxm[jc];
         if (lane == 0) A.dirty[jc] = 1;
         const bool ok = nn_walk_box(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, [&](int m) {
+                                    own.z + grow, (int)A.n, [&](int m) {
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
                 const double dy = own.y - L.ty[lane];
@@ -513,6 +515,68 @@ __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, int64_t n) {
     if (k < n) p[k] = 0;
 }
 
+// Counting pass of the h-iteration for the listed particles, split over the root's eight
+// children: wavefront (t, o) counts particle list_in[t]'s neighbours in the subtree of the
+// root's child o and adds them to cnt_slot[t] (integer sums: exact and order-free).  A
+// particle with a large h (a free-boundary particle whose h keeps growing) is so counted
+// by up to eight wavefronts instead of one; k_h_finish then applies the update.
+__global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
+    __shared__ NNWaveLds lds_all[NN1_WAVES];
+    const int w = threadIdx.x / 64, lane = wave_lane();
+    NNWaveLds &L = lds_all[w];
+    const int64_t work = 8 * (int64_t)*A.count_in;
+    const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
+    for (int64_t tt = (int64_t)blockIdx.x * NN1_WAVES + w; tt < work; tt += nwaves) {
+        const int64_t t = tt >> 3;
+        const int start = A.cchild[tt & 7];   // child (tt & 7) of the root cell
+        if (start < 0) continue;
+        const int64_t k = A.list_in[t];
+        const double4 own = A.sxm[k];
+        const double hi = A.h[k];
+        const double hi2 = hi * hi;
+        const double grow = hi * (1.0 + 1e-9);
+        int count = 0;
+        const bool ok = nn_walk_box(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
+                                    own.z + grow, start, [&](int m) {
+            bool hit = false;
+            if (lane < m) {
+                const double dx = own.x - L.tx[lane];
+                const double dy = own.y - L.ty[lane];
+                const double dz = own.z - L.tz[lane];
+                const double d = dx * dx + dy * dy + dz * dz;
+                hit = d < hi2 && d < L.th2[lane] && L.tj[lane] != (int)k;
+            }
+            count += popc64(wave_ballot(hit));
+        });
+        if (!ok) {
+            if (lane == 0) set_err(A.err, ERRF_STACK);
+            return;
+        }
+        if (lane == 0 && count > 0) atomicAdd(&A.cnt_slot[t], (uint32_t)count);
+        wave_sync();
+    }
+}
+
+// the h update of the listed particles from their counts (after the counting pass, so every
+// count saw the h of this pass); clears cnt_slot for the next pass
+__global__ __launch_bounds__(256) void k_h_finish(NNArgs A) {
+    const uint32_t cnt = *A.count_in;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = A.list_in[t];
+        const int N = (int)A.cnt_slot[t];
+        A.cnt_slot[t] = 0u;
+        const double hi = A.h[k];
+        double hn;
+        const bool fin = h_update(A, N, hi, hn);
+        A.iters[k] = A.pass + 1;
+        A.snoi[k] = N;
+        A.h_slot[t] = hn;
+        if (fin) A.done[k] = 1;
+        else A.list_out[atomicAdd(A.count_out, 1u)] = (int32_t)k;
+        atomicAdd(&A.counters[CNT_H_COUNTS], 1ull);
+    }
+}
+
 // h[list[t]] = h_slot[t] after a per-particle pass
 __global__ __launch_bounds__(256) void k_h_apply(const int32_t *__restrict__ list, const uint32_t *__restrict__ count,
                                                  const double *__restrict__ h_slot, double *__restrict__ h) {
@@ -539,11 +603,12 @@ static unsigned nn_grid(mhpc_ctx *c) { return (unsigned)ceil_div(ceil_div(c->n, 
 
 // done flags and iteration counts to zero; the per-pass list counters to zero
 __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *list_counts, int nlc,
-                         unsigned long long *hmax0_bits) {
+                         unsigned long long *hmax0_bits, uint32_t *cnt_slot) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k < n) {
         done[k] = 0;
         iters[k] = 0;
+        cnt_slot[k] = 0u;
     }
     if (k < nlc) list_counts[k] = 0;
     if (k == 0) *hmax0_bits = 0ull;
@@ -555,7 +620,8 @@ __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *lis
 void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_h_init, dim3((unsigned)ceil_div(c->n > 64 ? c->n : 64, 256)), dim3(256), 0, c->stream,
-                       c->n, c->sdone, c->siters, c->hlist_count, MHPC_MAX_PASSES + 1, c->hmax_bits);
+                       c->n, c->sdone, c->siters, c->hlist_count, MHPC_MAX_PASSES + 1, c->hmax_bits,
+                       c->hcnt);
     NNArgs a = nn_common(c);
     a.done = c->sdone;
     a.iters = c->siters;
@@ -607,7 +673,9 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
         a.list_out = c->hlist[pass & 1];
         a.count_out = c->hlist_count + pass;
         a.h_slot = c->hslot;
-        hipLaunchKernelGGL(k_nn_one<1>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
+        a.cnt_slot = c->hcnt;
+        hipLaunchKernelGGL(k_nn_count_split, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
+        hipLaunchKernelGGL(k_h_finish, dim3(256), dim3(256), 0, c->stream, a);
         hipLaunchKernelGGL(k_h_apply, dim3(256), dim3(256), 0, c->stream, a.list_in, a.count_in, c->hslot, c->sh);
     }
 }

@@ -113,6 +113,7 @@ struct mhpc_ctx {
     unsigned long long *hmax_bits = nullptr;     // largest h (bit pattern)
     uint32_t *hlist_count = nullptr;             // [MHPC_MAX_PASSES + 1]
     double *hslot = nullptr;                     // new h per list slot
+    uint32_t *hcnt = nullptr;                    // neighbour count per list slot
 
     // learned smoothing length
     float *mlp_params = nullptr, *mlp_m1 = nullptr, *mlp_m2 = nullptr, *mlp_grad = nullptr;
