@@ -36,8 +36,7 @@ def sph_bytes_per_particle(nnb: float) -> dict:
     assumed reused through LDS / L2."""
     return {
         "ms_nn": 32 + 4 + 4 * nnb,           # read x,y,z,h; write list + count
-        "ms_density": 40 + 4 + 4 * nnb + 8,  # read x,y,z,h,m + list; write rho
-        "ms_eos": 24 + 16,                   # read rho,e,p; write p,cs
+        "ms_density": 40 + 4 + 4 * nnb + 8,  # read x,y,z,h,m + list; write rho (the EOS is fused in)
         "ms_forces": 88 + 4 + 4 * nnb + 48,  # read 11 fields + list; write 6 fields
     }
 

@@ -919,9 +919,8 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     if (spec) launch_neighbours_fixup(c);
     else if (sph) launch_neighbours(c);
     mark(9);
-    if (sph) launch_density(c);
+    if (sph) launch_density_eos(c);   // sound speed and pressure fused (ms_eos stays ~0)
     mark(10);
-    if (sph) launch_eos(c);
     mark(11);
     if (sph) launch_forces(c, feats);
     mark(12);

@@ -785,6 +785,78 @@ __device__ __forceinline__ double rsq_nr(double x) {
     return fma(y, e, y);
 }
 
+// ------------------------------------------------------------------------------------
+// a12 / a13 sound speed (soundspeed.cu:38-97) and pressure (pressure.cu:6-87), fused;
+// also writes the records the force loop gathers: cs into {vx,vy,vz,cs} and
+// {rho, p/rho^2, m/rho, h}.
+__device__ __forceinline__ double eos_cs(int eos, double gamma, double K, double rho, double p,
+                                         double cs) {
+    switch (eos) {
+        case MHPC_EOS_POLYTROPIC: return sqrt(K * pow(rho, gamma - 1.0));
+        case MHPC_EOS_IDEAL_GAS: return sqrt(gamma * p / rho);
+        case MHPC_EOS_LOCALLY_ISOTHERMAL: return 0.0;
+        default: return cs;
+    }
+}
+
+__device__ __forceinline__ double eos_p(int eos, double gamma, double K, double rho, double e,
+                                        double cs) {
+    switch (eos) {
+        case MHPC_EOS_POLYTROPIC: return K * pow(rho, gamma);
+        case MHPC_EOS_ISOTHERMAL: return 41255.407 * rho;
+        case MHPC_EOS_IDEAL_GAS: return (gamma - 1) * rho * e;
+        case MHPC_EOS_LOCALLY_ISOTHERMAL:
+        case MHPC_EOS_ISOTHERMAL_CONST_CS: return cs * cs * rho;
+        default: return 0.0;
+    }
+}
+
+// EOS of one particle and the records the force loop reads (shared by k_eos and the
+// density kernel's fused form)
+struct EosArgs {
+    int eos, cs_before_p;
+    double gamma, K;
+    const double *se;
+    double *sp, *scs;
+    double4 *sv4, *sr4;
+    double *p_out, *cs_out;
+};
+
+__device__ __forceinline__ void eos_particle(const EosArgs &E, int64_t k, int32_t i, double rho, double m,
+                                             double h) {
+    const double e = E.se[k];
+    double p = E.sp[k], cs = E.scs[k];
+    if (E.cs_before_p) {
+        cs = eos_cs(E.eos, E.gamma, E.K, rho, p, cs);
+        p = eos_p(E.eos, E.gamma, E.K, rho, e, cs);
+    } else {
+        p = eos_p(E.eos, E.gamma, E.K, rho, e, cs);
+        cs = eos_cs(E.eos, E.gamma, E.K, rho, p, cs);
+    }
+    E.sp[k] = p;
+    E.scs[k] = cs;
+    E.sv4[k].w = cs;
+    E.sr4[k] = make_double4(rho, p / (rho * rho), m / rho, h);
+    E.p_out[i] = p;
+    E.cs_out[i] = cs;
+}
+
+static EosArgs eos_args(mhpc_ctx *c) {
+    EosArgs E;
+    E.eos = c->p.eos;
+    E.cs_before_p = c->p.cs_before_p;
+    E.gamma = c->p.gamma;
+    E.K = c->p.K;
+    E.se = c->se;
+    E.sp = c->sp;
+    E.scs = c->scs;
+    E.sv4 = c->sv4;
+    E.sr4 = c->sr4;
+    E.p_out = c->f[MHPC_F_P];
+    E.cs_out = c->f[MHPC_F_CS];
+    return E;
+}
+
 // one density pair term: FP64 adds m_j W into acc; FP32 adds it (in FP32, from FP64
 // differences) into the Kahan accumulator acc32
 template <bool FP64>
@@ -820,14 +892,15 @@ __device__ __forceinline__ void density_pair(const double4 &pi, double hi, const
 // in a 256-particle one.  Same arithmetic and order as a pure gather.
 #define DENS_WIN 512
 
-template <bool FP64>
+// EOS: the sound speed / pressure of the particle follow in the same kernel (the RHS path)
+template <bool FP64, bool EOS>
 __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *__restrict__ sxm,
                                                       const double *__restrict__ sh,
                                                       const int32_t *__restrict__ snoi,
                                                       const int32_t *__restrict__ nbr, int64_t stride,
                                                       int max_nb, int sml_avg, const int32_t *__restrict__ perm,
                                                       double *__restrict__ srho, double *__restrict__ rho,
-                                                      uint32_t *err) {
+                                                      uint32_t *err, EosArgs E) {
     __shared__ double4 wp[DENS_WIN];
     __shared__ double wh[DENS_WIN];
     const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (int64_t)DENS_WIN;
@@ -865,82 +938,43 @@ __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *
     }
     const double r = pi.w * (K8PI / (hi * hi * hi)) + (FP64 ? acc : acc32.value());
     srho[k] = r;
-    rho[perm[k]] = r;
+    const int32_t i = perm[k];
+    rho[i] = r;
     if (!(r > 0.0)) set_err(err, ERRF_RHO);
+    if (EOS) eos_particle(E, k, i, r, pi.w, hi);
 }
 
-void launch_density(mhpc_ctx *c) {
+template <bool EOS>
+static void launch_density_t(mhpc_ctx *c) {
     if (c->n <= 0) return;
     const unsigned g = (unsigned)ceil_div(c->n, DENS_WIN);
+    const EosArgs E = eos_args(c);
     if (c->p.precision == MHPC_PREC_FP64)
-        hipLaunchKernelGGL(k_density<true>, dim3(g), dim3(DENS_WIN), 0, c->stream, c->n, c->sxm, c->sh, c->snoi,
-                           c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
-                           c->f[MHPC_F_RHO], c->err);
+        hipLaunchKernelGGL((k_density<true, EOS>), dim3(g), dim3(DENS_WIN), 0, c->stream, c->n, c->sxm, c->sh,
+                           c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
+                           c->f[MHPC_F_RHO], c->err, E);
     else
-        hipLaunchKernelGGL(k_density<false>, dim3(g), dim3(DENS_WIN), 0, c->stream, c->n, c->sxm, c->sh, c->snoi,
-                           c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
-                           c->f[MHPC_F_RHO], c->err);
+        hipLaunchKernelGGL((k_density<false, EOS>), dim3(g), dim3(DENS_WIN), 0, c->stream, c->n, c->sxm, c->sh,
+                           c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
+                           c->f[MHPC_F_RHO], c->err, E);
 }
 
-// ------------------------------------------------------------------------------------
-// a12 / a13 sound speed (soundspeed.cu:38-97) and pressure (pressure.cu:6-87), fused;
-// also writes the records the force loop gathers: cs into {vx,vy,vz,cs} and
-// {rho, p/rho^2, m/rho, p}.
-__device__ __forceinline__ double eos_cs(int eos, double gamma, double K, double rho, double p,
-                                         double cs) {
-    switch (eos) {
-        case MHPC_EOS_POLYTROPIC: return sqrt(K * pow(rho, gamma - 1.0));
-        case MHPC_EOS_IDEAL_GAS: return sqrt(gamma * p / rho);
-        case MHPC_EOS_LOCALLY_ISOTHERMAL: return 0.0;
-        default: return cs;
-    }
-}
+void launch_density(mhpc_ctx *c) { launch_density_t<false>(c); }
 
-__device__ __forceinline__ double eos_p(int eos, double gamma, double K, double rho, double e,
-                                        double cs) {
-    switch (eos) {
-        case MHPC_EOS_POLYTROPIC: return K * pow(rho, gamma);
-        case MHPC_EOS_ISOTHERMAL: return 41255.407 * rho;
-        case MHPC_EOS_IDEAL_GAS: return (gamma - 1) * rho * e;
-        case MHPC_EOS_LOCALLY_ISOTHERMAL:
-        case MHPC_EOS_ISOTHERMAL_CONST_CS: return cs * cs * rho;
-        default: return 0.0;
-    }
-}
+void launch_density_eos(mhpc_ctx *c) { launch_density_t<true>(c); }
 
-__global__ __launch_bounds__(256) void k_eos(int64_t n, int eos, double gamma, double K,
-                                             int cs_before_p, const double *__restrict__ srho,
-                                             const double *__restrict__ se, double *__restrict__ sp,
-                                             double *__restrict__ scs, const double4 *__restrict__ sxm,
-                                             double4 *__restrict__ sv4, double4 *__restrict__ sr4,
-                                             const double *__restrict__ sh,
-                                             const int32_t *__restrict__ perm,
-                                             double *__restrict__ p_out, double *__restrict__ cs_out) {
+__global__ __launch_bounds__(256) void k_eos(int64_t n, EosArgs E, const double *__restrict__ srho,
+                                             const double4 *__restrict__ sxm, const double *__restrict__ sh,
+                                             const int32_t *__restrict__ perm) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const double rho = srho[k], e = se[k];
-    double p = sp[k], cs = scs[k];
-    if (cs_before_p) {
-        cs = eos_cs(eos, gamma, K, rho, p, cs);
-        p = eos_p(eos, gamma, K, rho, e, cs);
-    } else {
-        p = eos_p(eos, gamma, K, rho, e, cs);
-        cs = eos_cs(eos, gamma, K, rho, p, cs);
-    }
-    sp[k] = p;
-    scs[k] = cs;
-    sv4[k].w = cs;
-    sr4[k] = make_double4(rho, p / (rho * rho), sxm[k].w / rho, sh[k]);
-    const int32_t i = perm[k];
-    p_out[i] = p;
-    cs_out[i] = cs;
+    eos_particle(E, k, perm[k], srho[k], sxm[k].w, sh[k]);
 }
 
 void launch_eos(mhpc_ctx *c) {
     if (c->n <= 0) return;
-    hipLaunchKernelGGL(k_eos, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->n,
-                       c->p.eos, c->p.gamma, c->p.K, c->p.cs_before_p, c->srho, c->se, c->sp, c->scs,
-                       c->sxm, c->sv4, c->sr4, c->sh, c->perm, c->f[MHPC_F_P], c->f[MHPC_F_CS]);
+    hipLaunchKernelGGL(k_eos, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->n, eos_args(c),
+                       c->srho, c->sxm, c->sh, c->perm);
 }
 
 // ------------------------------------------------------------------------------------
