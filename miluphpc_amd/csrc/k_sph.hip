@@ -21,13 +21,23 @@
 #define NN_OUTLIER 2.0       // group-mode outlier: h above this factor x the group's smallest
 #define NN_OUTLIER_MAX 16    // at most this many outliers per group are split off
 
-struct NNWaveLds {
+// per-wave LDS of the walker; TCAP = 128 gives the filtered walk room to compact the
+// candidates of two 64-particle loads before a full tile of 64 is handed on
+template <int TCAP>
+struct NNWaveLdsT {
     int stack[NN_STACK_PHYS];
     int rstart[NN_RANGES];
     int rcnt[NN_RANGES];
     int rpre[64];
-    double tx[64], ty[64], tz[64], th2[64];
-    int tj[64];
+    double tx[TCAP], ty[TCAP], tz[TCAP], th2[TCAP];
+    int tj[TCAP];
+};
+using NNWaveLds = NNWaveLdsT<64>;
+using NNWaveLdsF = NNWaveLdsT<128>;
+
+// candidate filter that keeps everything (per-particle walks test each candidate once)
+struct KeepAll {
+    __device__ __forceinline__ bool operator()(double, double, double, double) const { return true; }
 };
 
 struct NNArgs {
@@ -77,16 +87,20 @@ struct NNArgs {
 // and hand the candidate particles to
 // `tile` in LDS tiles of up to 64 (L.tx/ty/tz/th2/tj, count m).  Leaves of at most
 // NN_LEAF particles and single particles are taken as ranges of sorted positions; larger
-// cells are opened.  Returns false on stack overflow.  Wave-uniform control flow.
-template <typename TileFn>
-__device__ __forceinline__ bool nn_walk_box(NNWaveLds &L, const NNArgs &A, double lox, double hix, double loy,
-                                            double hiy, double loz, double hiz, int start, TileFn &&tile) {
+// cells are opened.  With FILTER, only candidates with keep(x, y, z, h^2) true reach the
+// tiles (compacted in walk order, so the tile sequence is the unfiltered one minus the
+// dropped candidates).  Returns false on stack overflow.  Wave-uniform control flow.
+template <bool FILTER, typename LdsT, typename KeepFn, typename TileFn>
+__device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
+                                            double hiy, double loz, double hiz, int start, KeepFn &&keep,
+                                            TileFn &&tile) {
     const int lane = wave_lane();
     const int64_t n = A.n;
     int top = 1, nr = 0;
     if (lane == 0) L.stack[0] = start;
     wave_sync();
     const uint64_t lt = lanemask_lt();
+    int fill = 0;   // FILTER: kept candidates waiting in L.t*[0, fill)
     for (;;) {
         // walk until 64 ranges are pending or the stack is empty
         while (top > 0 && nr < 64) {
@@ -147,7 +161,10 @@ __device__ __forceinline__ bool nn_walk_box(NNWaveLds &L, const NNArgs &A, doubl
             top += tot;
             wave_sync();
         }
-        if (nr == 0 && top == 0) return true;
+        if (nr == 0 && top == 0) {
+            if (FILTER && fill > 0) tile(fill);
+            return true;
+        }
         // stream the candidates of up to 64 ranges through LDS
         const int np = nr < 64 ? nr : 64;
         const int v = lane < np ? L.rcnt[lane] : 0;
@@ -157,24 +174,74 @@ __device__ __forceinline__ bool nn_walk_box(NNWaveLds &L, const NNArgs &A, doubl
         wave_sync();
         for (int base = 0; base < total; base += 64) {
             const int t = base + lane;
-            if (t < total) {
-                int rlo = 0, rhi = np - 1;
-                while (rlo < rhi) {
-                    const int mid = (rlo + rhi + 1) >> 1;
-                    if (L.rpre[mid] <= t) rlo = mid; else rhi = mid - 1;
+            if constexpr (!FILTER) {
+                if (t < total) {
+                    int rlo = 0, rhi = np - 1;
+                    while (rlo < rhi) {
+                        const int mid = (rlo + rhi + 1) >> 1;
+                        if (L.rpre[mid] <= t) rlo = mid; else rhi = mid - 1;
+                    }
+                    const int j = L.rstart[rlo] + (t - L.rpre[rlo]);
+                    const double hj = A.h[j];
+                    const double4 r = A.sxm[j];
+                    L.tx[lane] = r.x;
+                    L.ty[lane] = r.y;
+                    L.tz[lane] = r.z;
+                    L.th2[lane] = hj * hj;
+                    L.tj[lane] = j;
                 }
-                const int j = L.rstart[rlo] + (t - L.rpre[rlo]);
-                const double hj = A.h[j];
-                const double4 r = A.sxm[j];
-                L.tx[lane] = r.x;
-                L.ty[lane] = r.y;
-                L.tz[lane] = r.z;
-                L.th2[lane] = hj * hj;
-                L.tj[lane] = j;
+                wave_sync();
+                tile((total - base) < 64 ? (total - base) : 64);
+                wave_sync();
+            } else {
+                bool kp = false;
+                double rx = 0.0, ry = 0.0, rz = 0.0, h2 = 0.0;
+                int j = 0;
+                if (t < total) {
+                    int rlo = 0, rhi = np - 1;
+                    while (rlo < rhi) {
+                        const int mid = (rlo + rhi + 1) >> 1;
+                        if (L.rpre[mid] <= t) rlo = mid; else rhi = mid - 1;
+                    }
+                    j = L.rstart[rlo] + (t - L.rpre[rlo]);
+                    const double hj = A.h[j];
+                    const double4 r = A.sxm[j];
+                    rx = r.x;
+                    ry = r.y;
+                    rz = r.z;
+                    h2 = hj * hj;
+                    kp = keep(rx, ry, rz, h2);
+                }
+                const uint64_t mk = wave_ballot(kp);
+                if (kp) {
+                    const int pos = fill + popc64(mk & lt);   // < fill + 64 <= 127
+                    L.tx[pos] = rx;
+                    L.ty[pos] = ry;
+                    L.tz[pos] = rz;
+                    L.th2[pos] = h2;
+                    L.tj[pos] = j;
+                }
+                fill += popc64(mk);
+                wave_sync();
+                if (fill >= 64) {
+                    tile(64);
+                    wave_sync();
+                    // move the overflow [64, fill) to the front (disjoint ranges)
+                    const int rest = fill - 64;
+                    if (lane < rest) {
+                        const double a0 = L.tx[64 + lane], a1 = L.ty[64 + lane], a2 = L.tz[64 + lane];
+                        const double a3 = L.th2[64 + lane];
+                        const int a4 = L.tj[64 + lane];
+                        L.tx[lane] = a0;
+                        L.ty[lane] = a1;
+                        L.tz[lane] = a2;
+                        L.th2[lane] = a3;
+                        L.tj[lane] = a4;
+                    }
+                    fill = rest;
+                    wave_sync();
+                }
             }
-            wave_sync();
-            tile((total - base) < 64 ? (total - base) : 64);
-            wave_sync();
         }
         const int rest = nr - np;
         if (lane < rest) {
@@ -217,9 +284,9 @@ __device__ __forceinline__ bool h_update(const NNArgs &A, int N, double hi, doub
 // (speculative: kept for the particles whose list cannot change afterwards)
 template <int MODE>
 __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
-    __shared__ NNWaveLds lds_all[NN_WAVES];
+    __shared__ NNWaveLdsF lds_all[NN_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
-    NNWaveLds &L = lds_all[w];
+    NNWaveLdsF &L = lds_all[w];
     const int64_t group = xcd_block(blockIdx.x, gridDim.x) * NN_WAVES + w;
     const int64_t k = group * 64 + lane;
     const int64_t n = A.n;
@@ -258,14 +325,29 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     // relative margin far above FP64 rounding, so pruning never drops a true pair)
     const double hmax = wave_max(active ? hi : 0.0);
     const double grow = hmax * (1.0 + 1e-9);
-    const double lox = wave_min(active ? xi : 1e300) - grow, hix = wave_max(active ? xi : -1e300) + grow;
-    const double loy = wave_min(active ? yi : 1e300) - grow, hiy = wave_max(active ? yi : -1e300) + grow;
-    const double loz = wave_min(active ? zi : 1e300) - grow, hiz = wave_max(active ? zi : -1e300) + grow;
+    const double bx0 = wave_min(active ? xi : 1e300), bx1 = wave_max(active ? xi : -1e300);
+    const double by0 = wave_min(active ? yi : 1e300), by1 = wave_max(active ? yi : -1e300);
+    const double bz0 = wave_min(active ? zi : 1e300), bz1 = wave_max(active ? zi : -1e300);
+    const double lox = bx0 - grow, hix = bx1 + grow;
+    const double loy = by0 - grow, hiy = by1 + grow;
+    const double loz = bz0 - grow, hiz = bz1 + grow;
+    // candidate filter: squared distance from the box of the active positions, below both
+    // hmax^2 and h_j^2.  Per axis fl(b0 - x) <= fl(x_i - x) for every x_i >= b0 (rounding is
+    // monotone), so this distance never exceeds the pair test's d for any active i, and every
+    // pair with d < h_i^2 <= hmax^2 and d < h_j^2 passes: the tiles lose only non-pairs.
+    const double hm2 = hmax * hmax;
+    auto keep = [&](double x, double y, double z, double h2) {
+        const double ex = fmax(fmax(bx0 - x, x - bx1), 0.0);
+        const double ey = fmax(fmax(by0 - y, y - by1), 0.0);
+        const double ez = fmax(fmax(bz0 - z, z - bz1), 0.0);
+        const double d = ex * ex + ey * ey + ez * ez;
+        return d < hm2 && d < h2;
+    };
 
     int count = 0;
     int b0 = 0, b1 = 0, b2 = 0;   // pending list entries, flushed four at a time as one int4
     int4 *wp = A.nbr4 + k;        // where the next four go
-    const bool ok = nn_walk_box(L, A, lox, hix, loy, hiy, loz, hiz, (int)n, [&](int m) {
+    const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, (int)n, keep, [&](int m) {
         if (!active) return;
         if (MODE == 1) {
             for (int e = 0; e < m; e++) {
@@ -390,8 +472,8 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
         const double hi2 = hi * hi;
         const double grow = hi * (1.0 + 1e-9);
         int count = 0;
-        const bool ok = nn_walk_box(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, (int)A.n, [&](int m) {
+        const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
+                                    own.z + grow, (int)A.n, KeepAll(), [&](int m) {
             bool hit = false;
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
@@ -455,8 +537,8 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
         const int64_t jc = A.list_in[t];
         const double4 own = A.sxm[jc];
         if (lane == 0) A.dirty[jc] = 1;
-        const bool ok = nn_walk_box(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, (int)A.n, [&](int m) {
+        const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
+                                    own.z + grow, (int)A.n, KeepAll(), [&](int m) {
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
                 const double dy = own.y - L.ty[lane];
@@ -536,8 +618,8 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
         const double hi2 = hi * hi;
         const double grow = hi * (1.0 + 1e-9);
         int count = 0;
-        const bool ok = nn_walk_box(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, start, [&](int m) {
+        const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
+                                    own.z + grow, start, KeepAll(), [&](int m) {
             bool hit = false;
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];

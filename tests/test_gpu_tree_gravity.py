@@ -83,6 +83,26 @@ def test_sort_ties_and_ragged_sizes(gpu_lib, orc):
         assert np.array_equal(perm, np.argsort(ko, kind="stable"))
 
 
+def test_sort_many_tiles(gpu_lib, orc):
+    """Hundreds of 4096-key tiles chained by the look-back: stable order against numpy."""
+    rng = np.random.default_rng(32)
+    n = 1_500_007
+    print_case(f"sort many tiles n={n}")
+    g = rng.random((n, 3))
+    g[::7] = np.round(g[::7] * 16) / 16      # some duplicated keys across tiles
+    s = {"x": g[:, 0].copy(), "y": g[:, 1].copy(), "z": g[:, 2].copy(), "mass": np.full(n, 1.0 / n)}
+    with make_ctx(gpu_lib, s, gpu_lib.default_params()) as ctx:
+        box = ctx.bbox()
+        ctx.keys()
+        for _ in range(2):                    # a second sort reuses the look-back words
+            ctx.sort()
+        k = ctx.download("key")
+        perm = ctx.download("sorted")
+    ko = orc.keys(s["x"], s["y"], s["z"], box, 0)
+    assert np.array_equal(k, ko)
+    assert np.array_equal(perm, np.argsort(ko, kind="stable"))
+
+
 def canonical_gpu(tree, n, perm, lkeys):
     """(level, Lebesgue prefix) -> (cell index) and canonical children of every GPU cell."""
     lev, start = tree["level"], tree["start"]
