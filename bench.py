@@ -147,6 +147,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-two-walk", action="store_true")
+    ap.add_argument("--no-shard", action="store_true", help="skip the N/8 Plummer line")
     ap.add_argument("--learned-nside", type=int, default=128)
     ap.add_argument("--learned-steps", type=int, default=10)
     ap.add_argument("--learned-train-iters", type=int, default=100)
@@ -338,18 +339,19 @@ def run_plummer(args, lib, ics, result):
         }
         log(f"plummer: {args.plummer_n / pw:.4e} particles/s, gravity {pm['ms_gravity']:.2f} ms, {tflops:.2f} TFLOP/s")
         del pl
-        # the per-GPU particle count of the same model on an 8-GPU node: a whole Plummer sphere
-        # of N / 8 particles (a proxy for one shard's kernel shape, not a spatial sub-domain)
-        n8 = args.plummer_n // 8
-        pl = ics.plummer(n8)
-        with lib.Context(n8, pp) as ctx:
-            ctx.load(**pl)
-            pw8, pst8 = time_rhs(ctx, lib, lib.RHS_GRAVITY, max(1, args.steps // 2), 1, "plummer-shard")
-        pm8 = median_stats(pst8)
-        result["plummer"]["shard_of_8"] = {
-            "n_particles": n8, "value": n8 / pw8, "ms_per_step": pw8 * 1e3,
-            "gravity_ms": pm8["ms_gravity"]}
-        del pl
+        if not args.no_shard:
+            # the per-GPU particle count of the same model on an 8-GPU node: a whole Plummer sphere
+            # of N / 8 particles (a proxy for one shard's kernel shape, not a spatial sub-domain)
+            n8 = args.plummer_n // 8
+            pl = ics.plummer(n8)
+            with lib.Context(n8, pp) as ctx:
+                ctx.load(**pl)
+                pw8, pst8 = time_rhs(ctx, lib, lib.RHS_GRAVITY, max(1, args.steps // 2), 1, "plummer-shard")
+            pm8 = median_stats(pst8)
+            result["plummer"]["shard_of_8"] = {
+                "n_particles": n8, "value": n8 / pw8, "ms_per_step": pw8 * 1e3,
+                "gravity_ms": pm8["ms_gravity"]}
+            del pl
 
 
 if __name__ == "__main__":

@@ -98,7 +98,7 @@ static int ensure_nbr(mhpc_ctx *c) {
 
 static void free_all(mhpc_ctx *c) {
     void *ptrs[] = {c->noi, c->level, c->h_iters, c->key, c->lkey, c->skey, c->keyB, c->perm,
-                    c->valB, c->sort_status, c->sort_ticket, c->digit_hist, c->slkey, c->sxm, c->sv4,
+                    c->valB, c->tile_hist, c->tile_off, c->digit_hist, c->slkey, c->sxm, c->sv4,
                     c->sr4, c->sh, c->sh2, c->se, c->srho, c->sp, c->scs, c->snoi, c->siters, c->sdone, c->lcp, c->tcnt, c->toff,
                     c->level_count, c->cell_base, c->cstart, c->cend, c->clevel, c->cchild,
                     c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax,
@@ -144,8 +144,8 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->perm, cap));
     HIPC(dalloc(&c->valB, cap));
     c->sort_tiles_cap = ceil_div(cap, MHPC_SORT_TILE);
-    HIPC(dalloc(&c->sort_status, MHPC_SORT_BUCKETS * c->sort_tiles_cap));
-    HIPC(dalloc(&c->sort_ticket, MHPC_SORT_PASSES));
+    HIPC(dalloc(&c->tile_hist, MHPC_SORT_BUCKETS * c->sort_tiles_cap));
+    HIPC(dalloc(&c->tile_off, MHPC_SORT_BUCKETS * c->sort_tiles_cap));
     HIPC(dalloc(&c->digit_hist, MHPC_SORT_PASSES * MHPC_SORT_BUCKETS));
     double **sorted[] = {&c->sh, &c->sh2, &c->se, &c->srho, &c->sp, &c->scs};
     for (double **p : sorted) HIPC(dalloc(p, cap));
@@ -349,9 +349,7 @@ static int finish(mhpc_ctx *c) {
     else if (e & ERRF_STACK) st = MHPC_E_STACK;
     else if (e & ERRF_CAP_NEIGHBOURS) st = MHPC_E_CAP_NEIGHBOURS;
     else if (e & ERRF_RHO) st = MHPC_E_RHO_NONPOSITIVE;
-    else if (e & ERRF_SPIN) st = MHPC_E_HIP;
-    c->last_error = (e & ERRF_SPIN) ? "radix sort look-back wait exceeded its bound"
-                                    : mhpc_status_string(st);
+    c->last_error = mhpc_status_string(st);
     return st;
 }
 

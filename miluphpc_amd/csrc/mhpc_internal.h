@@ -41,7 +41,6 @@ enum : uint32_t {
     ERRF_DEPTH = 4u,
     ERRF_RHO = 8u,
     ERRF_STACK = 16u,
-    ERRF_SPIN = 32u,        // a bounded inter-block wait ran out (look-back sort)
 };
 
 // device counters (uint64) for statistics
@@ -69,8 +68,7 @@ struct mhpc_ctx {
     // sort
     uint64_t *skey = nullptr, *keyB = nullptr;   // sorted keys / ping-pong buffer
     int32_t *perm = nullptr, *valB = nullptr;    // sorted position -> particle index
-    uint64_t *sort_status = nullptr;             // [tiles][256] look-back words of one pass
-    uint32_t *sort_ticket = nullptr;             // [passes] tile tickets
+    uint32_t *tile_hist = nullptr, *tile_off = nullptr;
     uint32_t *digit_hist = nullptr;              // [passes][256]
     int64_t sort_tiles_cap = 0;
     uint64_t *slkey = nullptr;                   // Lebesgue keys in sorted order
