@@ -1198,7 +1198,7 @@ __device__ __forceinline__ void force_pair(const ForceI &I, const double4 &pj, c
 
 // One block per window of FORCE_WIN consecutive sorted particles, their records staged in
 // LDS (see the density kernel): in-window neighbours are read there, the others gathered.
-#define FORCE_WIN 256
+#define FORCE_WIN 512
 
 template <bool FP64, bool FEAT>
 __global__ __launch_bounds__(FORCE_WIN) void k_forces(ForceArgs A) {
