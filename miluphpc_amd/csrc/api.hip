@@ -107,7 +107,7 @@ static void free_all(mhpc_ctx *c) {
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
                     c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->hslot, c->sdirty,
-                    c->hmax_bits, c->olist, c->ocount, c->hcnt};
+                    c->hmax_bits, c->olist, c->ocount, c->hcnt, c->nn_start};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
@@ -166,6 +166,11 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->hlist_count, MHPC_MAX_PASSES + 1));
     HIPC(dalloc(&c->hslot, cap));
     HIPC(dalloc(&c->hcnt, cap));
+    // start table levels: the smallest K with 8^K >= cap (one geometric cell per particle on
+    // average at the deepest level), 1 <= K <= MHPC_NN_KMAX
+    c->nn_kmax = 1;
+    while (c->nn_kmax < MHPC_NN_KMAX && (1ll << (3 * c->nn_kmax)) < cap) c->nn_kmax++;
+    HIPC(dalloc(&c->nn_start, nn_tab_off(c->nn_kmax + 1)));
     HIPC(dalloc(&c->lcp, cap));
     c->tree_tiles_cap = ceil_div(cap, MHPC_TREE_TILE);
     HIPC(dalloc(&c->tcnt, (MHPC_LEVELS + 1) * c->tree_tiles_cap));

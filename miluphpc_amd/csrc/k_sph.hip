@@ -48,6 +48,10 @@ struct NNArgs {
     const int32_t *cstart, *cend, *cchild;
     const double *cbmin, *cbmax;
     int64_t cap_cells;
+    // start table (levels 1..kmax) and the root box the keys were computed in
+    const int32_t *stab;
+    int kmax;
+    const double *box;
     // list mode: entry s of particle k is nbr4[(s / 4) * stride + k].{x,y,z,w}[s % 4]
     int4 *nbr4;
     int64_t stride;
@@ -83,6 +87,74 @@ struct NNArgs {
     unsigned long long *counters;
 };
 
+// Level-K index along one axis of the cell containing v: the keys' bisection of [mn, mx]
+// (lower half = key bit 1), counted upwards, so it is monotone in v.
+__device__ __forceinline__ int nn_axis_index(double v, double mn, double mx, int K) {
+    int idx = 0;
+    for (int l = 0; l < K; l++) {
+        const double m = 0.5 * (mn + mx);
+        if (v < m) {
+            mx = m;
+            idx <<= 1;
+        } else {
+            mn = m;
+            idx = (idx << 1) | 1;
+        }
+    }
+    return idx;
+}
+
+#define NN_START_ALL (-1)     // walk start: table cells covering the box
+#define NN_START_PART0 (-2)   // NN_START_PART0 - o: table cells of index = o (mod 8)
+
+// Push the start nodes of a walk for the box [lo, hi] onto L.stack: the deepest table level
+// K <= kmax at which the box spans at most 4 cells per axis, and the table's node for every
+// such cell (a cell of level K, a particle alone in it, or none).  Every particle inside
+// the box lies in one of these cells (the keys use the same bisection), and the nodes'
+// subtrees are disjoint, so the walk sees the same candidates as one from the root, in
+// another order.  With nparts = 8, only cells of index = part (mod 8).  Returns the count.
+template <typename LdsT>
+__device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
+                                             double hiy, double loz, double hiz, int part, int nparts) {
+    const int lane = wave_lane();
+    int node = -1;
+    const int K0 = A.stab ? A.kmax : 0;
+    int s = K0;
+    int ilx = 0, ihx = 0, ily = 0, ihy = 0, ilz = 0, ihz = 0;
+    if (K0 > 0) {
+        const double b0 = A.box[0], b1 = A.box[1], b2 = A.box[2], b3 = A.box[3], b4 = A.box[4], b5 = A.box[5];
+        ilx = nn_axis_index(lox, b0, b1, K0);
+        ihx = nn_axis_index(hix, b0, b1, K0);
+        ily = nn_axis_index(loy, b2, b3, K0);
+        ihy = nn_axis_index(hiy, b2, b3, K0);
+        ilz = nn_axis_index(loz, b4, b5, K0);
+        ihz = nn_axis_index(hiz, b4, b5, K0);
+        s = 0;
+        while (s < K0 && ((ihx >> s) - (ilx >> s) > 3 || (ihy >> s) - (ily >> s) > 3 || (ihz >> s) - (ilz >> s) > 3)) s++;
+    }
+    const int K = K0 - s;
+    if (K < 1) {
+        // box too large for the table (or no table): the whole tree
+        if (lane == 0) node = nparts == 1 ? (int)A.n : A.cchild[part];
+    } else {
+        const int bx = ilx >> s, by = ily >> s, bz = ilz >> s;
+        const int nx = (ihx >> s) - bx + 1, ny = (ihy >> s) - by + 1, nz = (ihz >> s) - bz + 1;
+        if (lane < nx * ny * nz && lane % nparts == part) {
+            const int ix = bx + lane % nx, iy = by + (lane / nx) % ny, iz = bz + lane / (nx * ny);
+            uint64_t P = 0;
+            for (int l = K - 1; l >= 0; l--) {
+                const uint64_t d = (uint64_t)(((ix >> l) & 1) ^ 1) | ((uint64_t)(((iy >> l) & 1) ^ 1) << 1) |
+                                   ((uint64_t)(((iz >> l) & 1) ^ 1) << 2);
+                P = (P << 3) | d;
+            }
+            node = A.stab[nn_tab_off(K) + (int64_t)P];
+        }
+    }
+    const uint64_t m = wave_ballot(node >= 0);
+    if (node >= 0) L.stack[popc64(m & lanemask_lt())] = node;
+    return popc64(m);
+}
+
 // Walk the (sub)tree of node `start` (the root is node n) for the axis-aligned box [lo, hi]
 // and hand the candidate particles to
 // `tile` in LDS tiles of up to 64 (L.tx/ty/tz/th2/tj, count m).  Leaves of at most
@@ -97,7 +169,12 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
     const int lane = wave_lane();
     const int64_t n = A.n;
     int top = 1, nr = 0;
-    if (lane == 0) L.stack[0] = start;
+    if (start >= 0) {
+        if (lane == 0) L.stack[0] = start;
+    } else {
+        top = nn_start_push(L, A, lox, hix, loy, hiy, loz, hiz, start == NN_START_ALL ? 0 : NN_START_PART0 - start,
+                            start == NN_START_ALL ? 1 : 8);
+    }
     wave_sync();
     const uint64_t lt = lanemask_lt();
     int fill = 0;   // FILTER: kept candidates waiting in L.t*[0, fill)
@@ -347,7 +424,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     int count = 0;
     int b0 = 0, b1 = 0, b2 = 0;   // pending list entries, flushed four at a time as one int4
     int4 *wp = A.nbr4 + k;        // where the next four go
-    const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, (int)n, keep, [&](int m) {
+    const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, keep, [&](int m) {
         if (!active) return;
         if (MODE == 1) {
             for (int e = 0; e < m; e++) {
@@ -473,7 +550,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
         const double grow = hi * (1.0 + 1e-9);
         int count = 0;
         const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, (int)A.n, KeepAll(), [&](int m) {
+                                    own.z + grow, NN_START_ALL, KeepAll(), [&](int m) {
             bool hit = false;
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
@@ -538,7 +615,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
         const double4 own = A.sxm[jc];
         if (lane == 0) A.dirty[jc] = 1;
         const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, (int)A.n, KeepAll(), [&](int m) {
+                                    own.z + grow, NN_START_ALL, KeepAll(), [&](int m) {
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
                 const double dy = own.y - L.ty[lane];
@@ -597,11 +674,12 @@ __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, int64_t n) {
     if (k < n) p[k] = 0;
 }
 
-// Counting pass of the h-iteration for the listed particles, split over the root's eight
-// children: wavefront (t, o) counts particle list_in[t]'s neighbours in the subtree of the
-// root's child o and adds them to cnt_slot[t] (integer sums: exact and order-free).  A
-// particle with a large h (a free-boundary particle whose h keeps growing) is so counted
-// by up to eight wavefronts instead of one; k_h_finish then applies the update.
+// Counting pass of the h-iteration for the listed particles, split eight ways: wavefront
+// (t, o) counts particle list_in[t]'s neighbours under the start cells of index = o
+// (mod 8) (the root's child o when the box is too large for the start table) and adds them
+// to cnt_slot[t] (integer sums: exact and order-free).  A particle with a large h (a
+// free-boundary particle whose h keeps growing) is so counted by up to eight wavefronts
+// instead of one; k_h_finish then applies the update.
 __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
@@ -610,8 +688,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
     const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
     for (int64_t tt = (int64_t)blockIdx.x * NN1_WAVES + w; tt < work; tt += nwaves) {
         const int64_t t = tt >> 3;
-        const int start = A.cchild[tt & 7];   // child (tt & 7) of the root cell
-        if (start < 0) continue;
+        const int start = NN_START_PART0 - (int)(tt & 7);   // eighth (tt & 7) of the start cells
         const int64_t k = A.list_in[t];
         const double4 own = A.sxm[k];
         const double hi = A.h[k];
@@ -667,8 +744,48 @@ __global__ __launch_bounds__(256) void k_h_apply(const int32_t *__restrict__ lis
         h[list[t]] = h_slot[t];
 }
 
+// Start table of the current tree: a particle alone in its level-l prefix for every
+// l >= its leaf level, and every cell of level <= kmax, written at their Lebesgue prefix.
+__global__ __launch_bounds__(256) void k_nn_start_table(int64_t n, int kmax, const uint8_t *__restrict__ lcp,
+                                                        const uint64_t *__restrict__ slkey,
+                                                        const int32_t *__restrict__ cell_base,
+                                                        const int32_t *__restrict__ cstart,
+                                                        const int32_t *__restrict__ clevel, int64_t cap_cells,
+                                                        int32_t *__restrict__ tab) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t < n) {
+        const int a = t > 0 ? (int)lcp[t - 1] : 0;
+        const int b = (int)lcp[t];
+        const int lp = (a > b ? a : b) + 1;
+        const uint64_t key = slkey[t];
+        for (int l = lp; l <= kmax; l++)
+            tab[nn_tab_off(l) + (int64_t)(key >> (3 * (MHPC_LEVELS - l)))] = (int32_t)t;
+    } else {
+        const int64_t ncells = cell_base[MHPC_LEVELS + 1];
+        const int64_t c = t - n + 1;   // cells 1 .. ncells-1 (cell 0 is the root)
+        if (ncells > cap_cells || c >= ncells) return;
+        const int L = clevel[c];
+        if (L < 1 || L > kmax) return;
+        tab[nn_tab_off(L) + (int64_t)(slkey[cstart[c]] >> (3 * (MHPC_LEVELS - L)))] = (int32_t)(n + c);
+    }
+}
+
+static void ensure_nn_start(mhpc_ctx *c) {
+    if (c->nn_start_valid || c->n <= 0) return;
+    launch_fill_u32(c, reinterpret_cast<uint32_t *>(c->nn_start), 0xffffffffu, nn_tab_off(c->nn_kmax + 1));
+    const uint64_t *slkey = (c->p.curve == MHPC_CURVE_HILBERT) ? c->slkey : c->skey;
+    const int64_t items = c->n + c->cap_cells;
+    hipLaunchKernelGGL(k_nn_start_table, dim3((unsigned)ceil_div(items, 256)), dim3(256), 0, c->stream, c->n,
+                       c->nn_kmax, c->lcp, slkey, c->cell_base, c->cstart, c->clevel, c->cap_cells, c->nn_start);
+    c->nn_start_valid = true;
+}
+
 static NNArgs nn_common(mhpc_ctx *c) {
+    ensure_nn_start(c);
     NNArgs a = {};
+    a.stab = c->nn_start;
+    a.kmax = c->nn_kmax;
+    a.box = c->box;
     a.n = c->n;
     a.perm = c->perm;
     a.sxm = c->sxm;

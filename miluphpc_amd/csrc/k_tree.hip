@@ -194,6 +194,7 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_link(
 
 void launch_tree(mhpc_ctx *c) {
     const int64_t n = c->n;
+    c->nn_start_valid = false;
     if (n <= 0) return;
     const int64_t ntiles = ceil_div(n, MHPC_TREE_TILE);
     const uint64_t *slkey = (c->p.curve == MHPC_CURVE_HILBERT) ? c->slkey : c->skey;

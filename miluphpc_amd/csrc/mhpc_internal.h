@@ -34,6 +34,10 @@
 #define MHPC_MAX_PASSES 64       // upper bound of max_counts
 #define MHPC_NUM_EVENTS 24
 
+// neighbour-walk start table: entries of levels 1 .. K-1 come before level K
+#define MHPC_NN_KMAX 8
+__host__ __device__ __forceinline__ int64_t nn_tab_off(int K) { return ((1ll << (3 * K)) - 8) / 7; }
+
 // device error flags (sticky word, mapped onto mhpc_status by the host)
 enum : uint32_t {
     ERRF_CAP_NODES = 1u,
@@ -136,6 +140,11 @@ struct mhpc_ctx {
     hipEvent_t ev[MHPC_NUM_EVENTS] = {};
 
     bool keys_valid = false, sorted_valid = false, tree_valid = false, com_valid = false;
+    // neighbour-walk start table: for levels 1..nn_kmax, the tree node covering each
+    // geometric cell of that level (Lebesgue prefix order), -1 where there is no particle
+    int32_t *nn_start = nullptr;
+    int nn_kmax = 0;
+    bool nn_start_valid = false;
     bool nn_valid = false, rho_valid = false, eos_valid = false;
     std::string last_error;
 };
