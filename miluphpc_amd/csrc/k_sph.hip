@@ -1115,25 +1115,38 @@ __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *
     const double hi = wh[tid];
     int cnt = snoi[k];
     cnt = cnt > max_nb ? max_nb : cnt;
-    const int4 *nbr4 = reinterpret_cast<const int4 *>(nbr);
+    const int4 *nrow = reinterpret_cast<const int4 *>(nbr) + k;   // row r at nrow[r * stride]
     double acc = 0.0;
     Kahan acc32;
-    int4 q4 = make_int4(0, 0, 0, 0);
-    for (int s = 0; s < cnt; s++) {
-        if ((s & 3) == 0) q4 = nbr4[(int64_t)(s >> 2) * stride + k];
-        const int q = s & 3;
-        const int32_t j = q == 0 ? q4.x : (q == 1 ? q4.y : (q == 2 ? q4.z : q4.w));
-        const uint64_t o = (uint64_t)((int64_t)j - base);
-        double4 pj;
-        double hj;
-        if (o < (uint64_t)DENS_WIN) {
-            pj = wp[o];
-            hj = wh[o];
-        } else {
-            pj = sxm[j];
-            hj = sh[j];
+    // one list row (four neighbours) per iteration: the four records are requested
+    // together, then the four pairs are evaluated in list order; the next row is
+    // requested a row ahead.  The window copy is read unconditionally (clamped index),
+    // the gather only where j is outside the window.
+    int4 qn = make_int4(0, 0, 0, 0);
+    if (cnt > 0) qn = nrow[0];
+    for (int r = 0; 4 * r < cnt; r++) {
+        const int4 q4 = qn;
+        const int m = cnt - 4 * r;   // entries left (>= 1)
+        if (m > 4) qn = nrow[(int64_t)(r + 1) * stride];
+        const int32_t jj[4] = {q4.x, q4.y, q4.z, q4.w};
+        double4 pj[4];
+        double hj[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int32_t j = e < m ? jj[e] : (int32_t)k;
+            const uint64_t o = (uint64_t)((int64_t)j - base);
+            const bool in = o < (uint64_t)DENS_WIN;
+            const int oc = in ? (int)o : 0;
+            pj[e] = wp[oc];
+            hj[e] = wh[oc];
+            if (!in) {
+                pj[e] = sxm[j];
+                hj[e] = sh[j];
+            }
         }
-        density_pair<FP64>(pi, hi, pj, hj, sml_avg, acc, acc32);
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (e < m) density_pair<FP64>(pi, hi, pj[e], hj[e], sml_avg, acc, acc32);
     }
     const double r = pi.w * (K8PI / (hi * hi * hi)) + (FP64 ? acc : acc32.value());
     srho[k] = r;
@@ -1343,24 +1356,39 @@ __global__ __launch_bounds__(FORCE_WIN) void k_forces(ForceArgs A) {
     ForceAcc<FP64> acc;
     double mumax = 0.0;
     double feat[6] = {0, 0, 0, 0, 0, 0};
-    const int4 *nbr4 = reinterpret_cast<const int4 *>(A.nbr);
-    int4 q4 = make_int4(0, 0, 0, 0);
-    for (int s = 0; s < cnt; s++) {
-        if ((s & 3) == 0) q4 = nbr4[(int64_t)(s >> 2) * A.stride + k];
-        const int q = s & 3;
-        const int32_t j = q == 0 ? q4.x : (q == 1 ? q4.y : (q == 2 ? q4.z : q4.w));
-        const uint64_t o = (uint64_t)((int64_t)j - base);
-        double4 pj, vj, rj;
-        if (o < (uint64_t)FORCE_WIN) {
-            pj = wp[o];
-            vj = wv[o];
-            rj = wr[o];
-        } else {
-            pj = A.sxm[j];
-            vj = A.sv4[j];
-            rj = A.sr4[j];
+    const int4 *nrow = reinterpret_cast<const int4 *>(A.nbr) + k;   // row r at nrow[r * stride]
+    const int64_t stride = A.stride;
+    // two neighbours per step, both requested before either pair is evaluated (see the
+    // density kernel); list rows a row ahead
+    int4 qn = make_int4(0, 0, 0, 0);
+    if (cnt > 0) qn = nrow[0];
+    for (int r = 0; 4 * r < cnt; r++) {
+        const int4 q4 = qn;
+        const int m = cnt - 4 * r;
+        if (m > 4) qn = nrow[(int64_t)(r + 1) * stride];
+        const int32_t jj[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+        for (int h = 0; h < 4; h += 2) {
+            double4 pj[2], vj[2], rj[2];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int32_t j = h + e < m ? jj[h + e] : (int32_t)k;
+                const uint64_t o = (uint64_t)((int64_t)j - base);
+                const bool in = o < (uint64_t)FORCE_WIN;
+                const int oc = in ? (int)o : 0;
+                pj[e] = wp[oc];
+                vj[e] = wv[oc];
+                rj[e] = wr[oc];
+                if (!in) {
+                    pj[e] = A.sxm[j];
+                    vj[e] = A.sv4[j];
+                    rj[e] = A.sr4[j];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 2; e++)
+                if (h + e < m) force_pair<FP64, FEAT>(I, pj[e], vj[e], rj[e], acc, mumax, feat);
         }
-        force_pair<FP64, FEAT>(I, pj, vj, rj, acc, mumax, feat);
     }
     double out[6];
     if constexpr (FP64) {
