@@ -1118,21 +1118,23 @@ __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *
     const int4 *nrow = reinterpret_cast<const int4 *>(nbr) + k;   // row r at nrow[r * stride]
     double acc = 0.0;
     Kahan acc32;
-    // one list row (four neighbours) per iteration: the four records are requested
-    // together, then the four pairs are evaluated in list order; the next row is
-    // requested a row ahead.  The window copy is read unconditionally (clamped index),
+    // two list rows (eight neighbours) per iteration: the eight records are requested
+    // together, then the pairs are evaluated in list order; the next two rows are
+    // requested a step ahead.  The window copy is read unconditionally (clamped index),
     // the gather only where j is outside the window.
-    int4 qn = make_int4(0, 0, 0, 0);
-    if (cnt > 0) qn = nrow[0];
-    for (int r = 0; 4 * r < cnt; r++) {
-        const int4 q4 = qn;
+    int4 qn0 = make_int4(0, 0, 0, 0), qn1 = qn0;
+    if (cnt > 0) qn0 = nrow[0];
+    if (cnt > 4) qn1 = nrow[stride];
+    for (int r = 0; 4 * r < cnt; r += 2) {
+        const int4 qa = qn0, qb = qn1;
         const int m = cnt - 4 * r;   // entries left (>= 1)
-        if (m > 4) qn = nrow[(int64_t)(r + 1) * stride];
-        const int32_t jj[4] = {q4.x, q4.y, q4.z, q4.w};
-        double4 pj[4];
-        double hj[4];
+        if (m > 8) qn0 = nrow[(int64_t)(r + 2) * stride];
+        if (m > 12) qn1 = nrow[(int64_t)(r + 3) * stride];
+        const int32_t jj[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+        double4 pj[8];
+        double hj[8];
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
+        for (int e = 0; e < 8; e++) {
             const int32_t j = e < m ? jj[e] : (int32_t)k;
             const uint64_t o = (uint64_t)((int64_t)j - base);
             const bool in = o < (uint64_t)DENS_WIN;
@@ -1145,7 +1147,7 @@ __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *
             }
         }
 #pragma unroll
-        for (int e = 0; e < 4; e++)
+        for (int e = 0; e < 8; e++)
             if (e < m) density_pair<FP64>(pi, hi, pj[e], hj[e], sml_avg, acc, acc32);
     }
     const double r = pi.w * (K8PI / (hi * hi * hi)) + (FP64 ? acc : acc32.value());
