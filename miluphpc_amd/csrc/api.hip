@@ -106,7 +106,7 @@ static void free_all(mhpc_ctx *c) {
                     c->diag_partial, c->diag_out, c->pc_backup, c->mlp_params, c->mlp_m1, c->mlp_m2,
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
-                    c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->hslot, c->sdirty,
+                    c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->sdirty,
                     c->hmax_bits, c->olist, c->ocount, c->hcnt, c->nn_start};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -164,7 +164,6 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->olist, cap));
     HIPC(dalloc(&c->ocount, 4));
     HIPC(dalloc(&c->hlist_count, MHPC_MAX_PASSES + 1));
-    HIPC(dalloc(&c->hslot, cap));
     HIPC(dalloc(&c->hcnt, cap));
     // start table levels: the smallest K with 8^K >= cap (one geometric cell per particle on
     // average at the deepest level), 1 <= K <= MHPC_NN_KMAX

@@ -75,7 +75,6 @@ struct NNArgs {
     const uint32_t *count_in;
     int32_t *list_out;
     uint32_t *count_out;
-    double *h_slot;           // per-particle mode: new h of list slot t
     uint32_t *cnt_slot;       // split counting: neighbour count of list slot t
     const uint8_t *only;      // list mode: when set, only particles with only[k] != 0
     uint8_t *dirty;           // dirty marking: particles whose list must be rebuilt
@@ -113,26 +112,42 @@ __device__ __forceinline__ int nn_axis_index(double v, double mn, double mx, int
 // the box lies in one of these cells (the keys use the same bisection), and the nodes'
 // subtrees are disjoint, so the walk sees the same candidates as one from the root, in
 // another order.  With nparts = 8, only cells of index = part (mod 8).  Returns the count.
+struct NNStart {
+    int K, s;                          // start level K = kmax - s (< 1: whole tree)
+    int ilx, ihx, ily, ihy, ilz, ihz;  // the box corners' cell indices at level kmax
+};
+
+__device__ __forceinline__ NNStart nn_start_level(const NNArgs &A, double lox, double hix, double loy,
+                                                  double hiy, double loz, double hiz) {
+    NNStart S = {};
+    const int K0 = A.stab ? A.kmax : 0;
+    S.s = K0;
+    if (K0 > 0) {
+        const double b0 = A.box[0], b1 = A.box[1], b2 = A.box[2], b3 = A.box[3], b4 = A.box[4], b5 = A.box[5];
+        S.ilx = nn_axis_index(lox, b0, b1, K0);
+        S.ihx = nn_axis_index(hix, b0, b1, K0);
+        S.ily = nn_axis_index(loy, b2, b3, K0);
+        S.ihy = nn_axis_index(hiy, b2, b3, K0);
+        S.ilz = nn_axis_index(loz, b4, b5, K0);
+        S.ihz = nn_axis_index(hiz, b4, b5, K0);
+        int s = 0;
+        while (s < K0 && ((S.ihx >> s) - (S.ilx >> s) > 3 || (S.ihy >> s) - (S.ily >> s) > 3 ||
+                          (S.ihz >> s) - (S.ilz >> s) > 3))
+            s++;
+        S.s = s;
+    }
+    S.K = K0 - S.s;
+    return S;
+}
+
 template <typename LdsT>
 __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
                                              double hiy, double loz, double hiz, int part, int nparts) {
     const int lane = wave_lane();
     int node = -1;
-    const int K0 = A.stab ? A.kmax : 0;
-    int s = K0;
-    int ilx = 0, ihx = 0, ily = 0, ihy = 0, ilz = 0, ihz = 0;
-    if (K0 > 0) {
-        const double b0 = A.box[0], b1 = A.box[1], b2 = A.box[2], b3 = A.box[3], b4 = A.box[4], b5 = A.box[5];
-        ilx = nn_axis_index(lox, b0, b1, K0);
-        ihx = nn_axis_index(hix, b0, b1, K0);
-        ily = nn_axis_index(loy, b2, b3, K0);
-        ihy = nn_axis_index(hiy, b2, b3, K0);
-        ilz = nn_axis_index(loz, b4, b5, K0);
-        ihz = nn_axis_index(hiz, b4, b5, K0);
-        s = 0;
-        while (s < K0 && ((ihx >> s) - (ilx >> s) > 3 || (ihy >> s) - (ily >> s) > 3 || (ihz >> s) - (ilz >> s) > 3)) s++;
-    }
-    const int K = K0 - s;
+    const NNStart S = nn_start_level(A, lox, hix, loy, hiy, loz, hiz);
+    const int K = S.K, s = S.s;
+    const int ilx = S.ilx, ihx = S.ihx, ily = S.ily, ihy = S.ihy, ilz = S.ilz, ihz = S.ihz;
     if (K < 1) {
         // box too large for the table (or no table): the whole tree
         if (lane == 0) node = nparts == 1 ? (int)A.n : A.cchild[part];
@@ -524,11 +539,9 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
 }
 
 // Per-particle mode: one wavefront per listed particle, searching the particle's own box
-// x +- h with the 64 lanes testing 64 candidates at a time.  Used for the h-iteration
-// passes after the first (only the unconverged particles are visited) and for the
-// outliers of the group mode (particles whose h is far above the rest of their group).
-// MODE 1: count and h update (identical to the group mode's; with h_slot set the new h
-// goes to h_slot[t] and k_h_apply writes it after the pass, else to h_next[k]);
+// x +- h with the 64 lanes testing 64 candidates at a time.  Used for the outliers of the
+// group mode (particles whose h is far above the rest of their group) and for list fix-ups.
+// MODE 1: count and h update (identical to the group mode's; the new h goes to h_next[k]);
 // MODE 2: MODE 1 plus the list; MODE 0: the list only.  Lists are written in candidate
 // order, one 4-byte entry per lane.
 #define NN1_WAVES 4
@@ -586,8 +599,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
             if (lane == 0) {
                 A.iters[k] = A.pass + 1;
                 A.snoi[k] = count;
-                if (A.h_slot) A.h_slot[t] = hn;
-                else A.h_next[k] = hn;
+                A.h_next[k] = hn;
                 if (fin) A.done[k] = 1;
                 else A.list_out[atomicAdd(A.count_out, 1u)] = (int32_t)k;
                 atomicAdd(&A.counters[CNT_H_COUNTS], 1ull);
@@ -677,9 +689,10 @@ __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, int64_t n) {
 // Counting pass of the h-iteration for the listed particles, split eight ways: wavefront
 // (t, o) counts particle list_in[t]'s neighbours under the start cells of index = o
 // (mod 8) (the root's child o when the box is too large for the start table) and adds them
-// to cnt_slot[t] (integer sums: exact and order-free).  A particle with a large h (a
-// free-boundary particle whose h keeps growing) is so counted by up to eight wavefronts
-// instead of one; k_h_finish then applies the update.
+// to cnt_slot[t] (integer sums: exact and order-free).  Eight short walks per particle keep
+// more wavefronts in flight than one long one (measured faster even for boxes of a few
+// start cells), and a particle with a large h (a free-boundary particle whose h keeps
+// growing) is shared by eight wavefronts; k_h_finish then applies the update.
 __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
@@ -688,12 +701,13 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
     const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
     for (int64_t tt = (int64_t)blockIdx.x * NN1_WAVES + w; tt < work; tt += nwaves) {
         const int64_t t = tt >> 3;
-        const int start = NN_START_PART0 - (int)(tt & 7);   // eighth (tt & 7) of the start cells
+        const int o = (int)(tt & 7);
         const int64_t k = A.list_in[t];
         const double4 own = A.sxm[k];
         const double hi = A.h[k];
         const double hi2 = hi * hi;
         const double grow = hi * (1.0 + 1e-9);
+        const int start = NN_START_PART0 - o;   // eighth o of the start cells
         int count = 0;
         const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
                                     own.z + grow, start, KeepAll(), [&](int m) {
@@ -717,31 +731,38 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
 }
 
 // the h update of the listed particles from their counts (after the counting pass, so every
-// count saw the h of this pass); clears cnt_slot for the next pass
+// count saw the h of this pass; each thread writes only its own particle's h): clears
+// cnt_slot for the next pass and appends the unconverged particles to list_out
 __global__ __launch_bounds__(256) void k_h_finish(NNArgs A) {
     const uint32_t cnt = *A.count_in;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t k = A.list_in[t];
-        const int N = (int)A.cnt_slot[t];
-        A.cnt_slot[t] = 0u;
-        const double hi = A.h[k];
-        double hn;
-        const bool fin = h_update(A, N, hi, hn);
-        A.iters[k] = A.pass + 1;
-        A.snoi[k] = N;
-        A.h_slot[t] = hn;
-        if (fin) A.done[k] = 1;
-        else A.list_out[atomicAdd(A.count_out, 1u)] = (int32_t)k;
-        atomicAdd(&A.counters[CNT_H_COUNTS], 1ull);
+    const int lane = wave_lane();
+    const uint64_t lt = lanemask_lt();
+    for (int64_t t0 = blockIdx.x * (int64_t)blockDim.x; t0 < cnt; t0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = t0 + threadIdx.x;
+        const bool valid = t < cnt;
+        bool stay = false;
+        int64_t k = 0;
+        if (valid) {
+            k = A.list_in[t];
+            const int N = (int)A.cnt_slot[t];
+            A.cnt_slot[t] = 0u;
+            double hn;
+            const bool fin = h_update(A, N, A.h[k], hn);
+            A.iters[k] = A.pass + 1;
+            A.snoi[k] = N;
+            A.h_next[k] = hn;
+            if (fin) A.done[k] = 1;
+            stay = !fin;
+        }
+        // wave-aggregated list append and counter (list order is free)
+        const uint64_t mS = wave_ballot(stay);
+        uint32_t b = 0;
+        if (lane == 0 && mS != 0) b = atomicAdd(A.count_out, (uint32_t)popc64(mS));
+        b = __shfl(b, 0);
+        if (stay) A.list_out[b + popc64(mS & lt)] = (int32_t)k;
+        const uint64_t mV = wave_ballot(valid);
+        if (lane == 0 && mV != 0) atomicAdd(&A.counters[CNT_H_COUNTS], (unsigned long long)popc64(mV));
     }
-}
-
-// h[list[t]] = h_slot[t] after a per-particle pass
-__global__ __launch_bounds__(256) void k_h_apply(const int32_t *__restrict__ list, const uint32_t *__restrict__ count,
-                                                 const double *__restrict__ h_slot, double *__restrict__ h) {
-    const uint32_t cnt = *count;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x)
-        h[list[t]] = h_slot[t];
 }
 
 // Start table of the current tree: a particle alone in its level-l prefix for every
@@ -856,7 +877,6 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
         o.out_list = nullptr;
         o.list_in = c->olist;
         o.count_in = c->ocount;
-        o.h_slot = nullptr;
         if (speculative) hipLaunchKernelGGL(k_nn_one<2>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, o);
         else hipLaunchKernelGGL(k_nn_one<1>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, o);
     }
@@ -871,11 +891,10 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
         a.count_in = c->hlist_count + (pass - 1);
         a.list_out = c->hlist[pass & 1];
         a.count_out = c->hlist_count + pass;
-        a.h_slot = c->hslot;
+        a.h_next = c->sh;   // k_h_finish updates h in place once every count is done
         a.cnt_slot = c->hcnt;
         hipLaunchKernelGGL(k_nn_count_split, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
         hipLaunchKernelGGL(k_h_finish, dim3(256), dim3(256), 0, c->stream, a);
-        hipLaunchKernelGGL(k_h_apply, dim3(256), dim3(256), 0, c->stream, a.list_in, a.count_in, c->hslot, c->sh);
     }
 }
 

@@ -116,7 +116,6 @@ struct mhpc_ctx {
     uint32_t *ocount = nullptr;
     unsigned long long *hmax_bits = nullptr;     // largest h (bit pattern)
     uint32_t *hlist_count = nullptr;             // [MHPC_MAX_PASSES + 1]
-    double *hslot = nullptr;                     // new h per list slot
     uint32_t *hcnt = nullptr;                    // neighbour count per list slot
 
     // learned smoothing length
