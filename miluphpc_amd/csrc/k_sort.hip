@@ -3,10 +3,10 @@
 // Replaces TreeNS::prepareSorting + HelperNS::sortArray = cub::DeviceRadixSort::SortPairs
 // (tree.cu:1404-1413, helper.cu:150-167): same result, a stable sort by key in which
 // ties keep ascending index order.  Eight passes of 8-bit digits.  Per pass:
-//   count   -- digit histogram of each 4096-key tile (LDS atomics);
+//   count   -- digit histogram of each 2048-key tile (LDS atomics);
 //   scan    -- one workgroup per digit scans the tile histograms (digit-major), adding
 //              the digit's global offset from the one-read histogram of all passes;
-//   scatter -- each wave ranks its 16 slots of 64 keys with a ballot multisplit (peers
+//   scatter -- each wave ranks its 8 slots of 64 keys with a ballot multisplit (peers
 //              = lanes holding the same digit), the block stages the tile in LDS in
 //              digit order and writes each digit run contiguously.
 // HBM bytes per pass: count reads 8 B/key, scatter reads 12 and writes 12.

@@ -28,7 +28,7 @@
 #define MHPC_SORT_BITS 8
 #define MHPC_SORT_BUCKETS 256
 #define MHPC_SORT_PASSES 8
-#define MHPC_SORT_TILE 4096     // keys per scatter block (256 threads x 16)
+#define MHPC_SORT_TILE 2048    // keys per scatter block (256 threads x 8)
 #define MHPC_TREE_TILE 1024     // positions per tree block (256 threads x 4)
 #define MHPC_HTABLE_MAX 4096    // h-iteration factor table size
 #define MHPC_MAX_PASSES 64       // upper bound of max_counts

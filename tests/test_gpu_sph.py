@@ -77,6 +77,50 @@ def test_h_iteration_bitexact(gpu_lib, orc, sedov24, cap):
     assert np.array_equal(h, oh)
 
 
+def clustered(n=16384, target=40.0):
+    """Plummer sphere with h from the analytic number density for about `target`
+    neighbours: h spans two orders of magnitude, so walks start anywhere from the deepest
+    start-table level to the whole tree."""
+    s = ics.plummer(n, seed=5)
+    r2 = s["x"] ** 2 + s["y"] ** 2 + s["z"] ** 2
+    s["sml"] = (target / (n * (1.0 + r2) ** -2.5)) ** (1.0 / 3.0)
+    return s
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_neighbour_sets_clustered(gpu_lib, orc, curve):
+    print_case(f"neighbours plummer 16384 clustered h, curve {curve}")
+    s = clustered()
+    p = gpu_lib.default_params(curve=curve)
+    with make_ctx(gpu_lib, s, p) as ctx:
+        ctx.bbox(); ctx.keys(); ctx.sort(); ctx.build_tree()
+        ctx.neighbours()
+        nnl, noi = ctx.neighbour_lists()
+    st, onnl, onoi = orc.nn_brute(s["x"], s["y"], s["z"], s["sml"], 180)
+    assert st == 0 and onoi.max() < 180
+    assert np.array_equal(noi, onoi)
+    assert sets(nnl, noi) == sets(onnl, onoi)
+
+
+def test_h_iteration_clustered(gpu_lib, orc):
+    print_case("h-iteration plummer 16384 clustered h")
+    s = clustered()
+    h0 = 0.7 * s["sml"]
+    p = gpu_lib.default_params(curve=1)
+    with make_ctx(gpu_lib, {**s, "sml": h0}, p) as ctx:
+        ctx.bbox(); ctx.keys(); ctx.sort(); ctx.build_tree()
+        ctx.h_iterate()
+        h = ctx.download("sml")
+        iters = ctx.download("h_iters")
+        noi = ctx.download("noi")
+    st, oh, oiters, onoi = orc.h_iterate(s["x"], s["y"], s["z"], h0)
+    assert st == 0
+    assert iters.max() > 1
+    assert np.array_equal(iters, oiters)
+    assert np.array_equal(noi, onoi)
+    assert np.array_equal(h, oh)
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_sph_rhs_parity(gpu_lib, orc, sedov24, precision):
     print_case(f"sph rhs sedov 24^3 precision={precision}")
