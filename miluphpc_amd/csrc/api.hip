@@ -346,6 +346,7 @@ static int finish(mhpc_ctx *c) {
                         hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
     const uint32_t e = *c->h_err;
+    if (c->h_counters[CNT_TREE_DEPTH] > 0) c->com_depth_hint = (int)c->h_counters[CNT_TREE_DEPTH];
     if (e == 0) return MHPC_OK;
     int st = MHPC_OK;
     if (e & ERRF_DEPTH) st = MHPC_E_DEPTH;

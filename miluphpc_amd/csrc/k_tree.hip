@@ -216,100 +216,146 @@ void launch_tree(mhpc_ctx *c) {
 // centre of mass over the children in slot order (tree.cu:1416-1497), traceless
 // quadrupole about the COM with the parallel-axis shift, and the particles' AABB.
 // Each launch reads its level's range from cell_base on the device.
-__global__ __launch_bounds__(256) void k_bottom_up(
-    int L, int64_t n, const int32_t *__restrict__ cell_base, const int32_t *__restrict__ cchild,
+// Eight lanes per cell, lane s loading child s (all children's loads in flight at once);
+// the children's terms go through LDS and the cell's first lane sums them in slot order, so
+// every sum has the order and rounding of a sequential loop over the children.  An empty
+// slot contributes exact zeros.
+#define BU_BLOCK 256
+// Levels L_hi down to L_lo; more than one level only with a single workgroup (the barrier
+// between levels orders them).  With depth_out set, thread 0 also records the deepest level
+// that has cells.
+__global__ __launch_bounds__(BU_BLOCK) void k_bottom_up(
+    int L_hi, int L_lo, int64_t n, const int32_t *__restrict__ cell_base, const int32_t *__restrict__ cchild,
     const double4 *__restrict__ sxm, double *__restrict__ cmass, double *__restrict__ cx,
     double *__restrict__ cy, double *__restrict__ cz, double *__restrict__ cquad,
     float4 *__restrict__ cquadf, double *__restrict__ cbmin, double *__restrict__ cbmax,
-    int64_t cap_cells, int gravity, int quad) {
+    int64_t cap_cells, int gravity, int quad, unsigned long long *depth_out) {
+    __shared__ double term[10][BU_BLOCK];
+    __shared__ double com[3][BU_BLOCK / 8];
+    const int tid = threadIdx.x, sub = tid & 7, g = tid >> 3;
+    if (depth_out && blockIdx.x == 0 && tid == 0) {
+        int d = 0;
+        for (int l = 1; l <= MHPC_LEVELS; l++)
+            if (cell_base[l + 1] > cell_base[l]) d = l;
+        *depth_out = (unsigned long long)d;
+    }
+    for (int L = L_hi; L >= L_lo; L--) {
     const int64_t c0 = cell_base[L], c1a = cell_base[L + 1];
     const int64_t c1 = c1a < cap_cells ? c1a : cap_cells;
-    for (int64_t c = c0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < c1;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        int32_t ch[8];
-        const int4 *cp = reinterpret_cast<const int4 *>(cchild + 8 * c);
-        const int4 q0 = cp[0], q1 = cp[1];
-        ch[0] = q0.x; ch[1] = q0.y; ch[2] = q0.z; ch[3] = q0.w;
-        ch[4] = q1.x; ch[5] = q1.y; ch[6] = q1.z; ch[7] = q1.w;
-        double bmin[3] = {1e300, 1e300, 1e300}, bmax[3] = {-1e300, -1e300, -1e300};
-        double m = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
-#pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int32_t k = ch[s];
-            if (k == -1) continue;
-            double px, py, pz, pm, lo[3], hi[3];
-            if (k < n) {
-                const double4 r = sxm[k];
-                px = r.x; py = r.y; pz = r.z; pm = r.w;
-                lo[0] = hi[0] = px; lo[1] = hi[1] = py; lo[2] = hi[2] = pz;
-            } else {
-                const int64_t cc = k - n;
-                px = cx[cc]; py = cy[cc]; pz = cz[cc]; pm = cmass[cc];
-                lo[0] = cbmin[cc]; lo[1] = cbmin[cap_cells + cc]; lo[2] = cbmin[2 * cap_cells + cc];
-                hi[0] = cbmax[cc]; hi[1] = cbmax[cap_cells + cc]; hi[2] = cbmax[2 * cap_cells + cc];
-            }
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                bmin[a] = fmin(bmin[a], lo[a]);
-                bmax[a] = fmax(bmax[a], hi[a]);
-            }
-            if (gravity) {
-                mx += pm * px;
-                my += pm * py;
-                mz += pm * pz;
-                m += pm;
-            }
-        }
-        cbmin[c] = bmin[0]; cbmin[cap_cells + c] = bmin[1]; cbmin[2 * cap_cells + c] = bmin[2];
-        cbmax[c] = bmax[0]; cbmax[cap_cells + c] = bmax[1]; cbmax[2 * cap_cells + c] = bmax[2];
-        if (!gravity) continue;
-        if (m > 0.0) { mx /= m; my /= m; mz /= m; }
-        cmass[c] = m;
-        cx[c] = mx;
-        cy[c] = my;
-        cz[c] = mz;
-        if (!quad) continue;
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
-#pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int32_t k = ch[s];
-            if (k == -1) continue;
-            double px, py, pz, pm;
-            double q[6] = {0, 0, 0, 0, 0, 0};
-            if (k < n) {
-                const double4 r = sxm[k];
-                px = r.x; py = r.y; pz = r.z; pm = r.w;
-            } else {
-                const int64_t cc = k - n;
-                px = cx[cc]; py = cy[cc]; pz = cz[cc]; pm = cmass[cc];
+    const int64_t nthreads = (c1 - c0) * 8;
+    for (int64_t b = (int64_t)blockIdx.x * BU_BLOCK; b < nthreads; b += (int64_t)gridDim.x * BU_BLOCK) {
+        const int64_t t = b + tid;
+        const bool valid = t < nthreads;
+        const int64_t c = c0 + (t >> 3);
+        const int32_t k = valid ? cchild[8 * c + sub] : -1;
+        double px = 0.0, py = 0.0, pz = 0.0, pm = 0.0;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (k >= 0 && k < n) {
+            const double4 r = sxm[k];
+            px = r.x; py = r.y; pz = r.z; pm = r.w;
+            lo[0] = hi[0] = px; lo[1] = hi[1] = py; lo[2] = hi[2] = pz;
+        } else if (k >= n) {
+            const int64_t cc = k - n;
+            px = cx[cc]; py = cy[cc]; pz = cz[cc]; pm = cmass[cc];
+            lo[0] = cbmin[cc]; lo[1] = cbmin[cap_cells + cc]; lo[2] = cbmin[2 * cap_cells + cc];
+            hi[0] = cbmax[cc]; hi[1] = cbmax[cap_cells + cc]; hi[2] = cbmax[2 * cap_cells + cc];
+            if (quad) {
 #pragma unroll
                 for (int e = 0; e < 6; e++) q[e] = cquad[e * cap_cells + cc];
             }
-            const double dx = px - mx, dy = py - my, dz = pz - mz;
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            a0 += q[0] + pm * (3.0 * dx * dx - d2);
-            a1 += q[1] + pm * (3.0 * dx * dy);
-            a2 += q[2] + pm * (3.0 * dx * dz);
-            a3 += q[3] + pm * (3.0 * dy * dy - d2);
-            a4 += q[4] + pm * (3.0 * dy * dz);
-            a5 += q[5] + pm * (3.0 * dz * dz - d2);
         }
-        cquad[c] = a0; cquad[cap_cells + c] = a1; cquad[2 * cap_cells + c] = a2;
-        cquad[3 * cap_cells + c] = a3; cquad[4 * cap_cells + c] = a4; cquad[5 * cap_cells + c] = a5;
-        cquadf[2 * c] = make_float4((float)a0, (float)a1, (float)a2, (float)a3);
-        cquadf[2 * c + 1] = make_float4((float)a4, (float)a5, 0.f, 0.f);
+        term[0][tid] = pm;
+        term[1][tid] = pm * px;
+        term[2][tid] = pm * py;
+        term[3][tid] = pm * pz;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            term[4 + a][tid] = lo[a];
+            term[7 + a][tid] = hi[a];
+        }
+        __syncthreads();
+        if (sub == 0 && valid) {
+            double bmin[3] = {1e300, 1e300, 1e300}, bmax[3] = {-1e300, -1e300, -1e300};
+            double m = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    bmin[a] = fmin(bmin[a], term[4 + a][tid + s]);
+                    bmax[a] = fmax(bmax[a], term[7 + a][tid + s]);
+                }
+                mx += term[1][tid + s];
+                my += term[2][tid + s];
+                mz += term[3][tid + s];
+                m += term[0][tid + s];
+            }
+            cbmin[c] = bmin[0]; cbmin[cap_cells + c] = bmin[1]; cbmin[2 * cap_cells + c] = bmin[2];
+            cbmax[c] = bmax[0]; cbmax[cap_cells + c] = bmax[1]; cbmax[2 * cap_cells + c] = bmax[2];
+            if (gravity) {
+                if (m > 0.0) { mx /= m; my /= m; mz /= m; }
+                cmass[c] = m;
+                cx[c] = mx;
+                cy[c] = my;
+                cz[c] = mz;
+                com[0][g] = mx;
+                com[1][g] = my;
+                com[2][g] = mz;
+            }
+        }
+        __syncthreads();
+        if (gravity && quad) {
+            double T[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if (k != -1) {
+                const double dx = px - com[0][g], dy = py - com[1][g], dz = pz - com[2][g];
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                T[0] = q[0] + pm * (3.0 * dx * dx - d2);
+                T[1] = q[1] + pm * (3.0 * dx * dy);
+                T[2] = q[2] + pm * (3.0 * dx * dz);
+                T[3] = q[3] + pm * (3.0 * dy * dy - d2);
+                T[4] = q[4] + pm * (3.0 * dy * dz);
+                T[5] = q[5] + pm * (3.0 * dz * dz - d2);
+            }
+#pragma unroll
+            for (int e = 0; e < 6; e++) term[e][tid] = T[e];
+            __syncthreads();
+            if (sub == 0 && valid) {
+                double A[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s < 8; s++) {
+#pragma unroll
+                    for (int e = 0; e < 6; e++) A[e] += term[e][tid + s];
+                }
+#pragma unroll
+                for (int e = 0; e < 6; e++) cquad[e * cap_cells + c] = A[e];
+                cquadf[2 * c] = make_float4((float)A[0], (float)A[1], (float)A[2], (float)A[3]);
+                cquadf[2 * c + 1] = make_float4((float)A[4], (float)A[5], 0.f, 0.f);
+            }
+            __syncthreads();
+        }
+    }
+    if (L_hi != L_lo) __syncthreads();
     }
 }
 
 void launch_bottom_up(mhpc_ctx *c, bool gravity) {
     if (c->n <= 0) return;
-    int grid = (int)ceil_div(c->n, 256 * 4);
+    // a level holds fewer cells than particles; eight threads per cell, grid-stride beyond
+    int grid = (int)ceil_div(c->n * 8, BU_BLOCK * 4);
     if (grid < 1) grid = 1;
-    if (grid > 2048) grid = 2048;
-    for (int L = MHPC_LEVELS; L >= 0; L--) {
-        hipLaunchKernelGGL(k_bottom_up, dim3(L <= 3 ? 1 : grid), dim3(256), 0, c->stream, L, c->n,
-                           c->cell_base, c->cchild, c->sxm, c->cmass, c->cx, c->cy,
-                           c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax, c->cap_cells,
-                           gravity ? 1 : 0, (gravity && c->p.quadrupole) ? 1 : 0);
-    }
+    if (grid > 4096) grid = 4096;
+    const int quad = (gravity && c->p.quadrupole) ? 1 : 0;
+    auto run = [&](int hi, int lo, int g, unsigned long long *depth_out) {
+        hipLaunchKernelGGL(k_bottom_up, dim3(g), dim3(BU_BLOCK), 0, c->stream, hi, lo, c->n, c->cell_base,
+                           c->cchild, c->sxm, c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin,
+                           c->cbmax, c->cap_cells, gravity ? 1 : 0, quad, depth_out);
+    };
+    // levels below the last call's deepest (normally empty) in one single-workgroup sweep,
+    // then one launch per level down to 4, then levels 3..0 (<= 585 cells) in one sweep
+    int deep = c->com_depth_hint;
+    if (deep < 4) deep = 4;
+    if (deep > MHPC_LEVELS) deep = MHPC_LEVELS;
+    if (deep < MHPC_LEVELS) run(MHPC_LEVELS, deep + 1, 1, nullptr);
+    for (int L = deep; L >= 4; L--) run(L, L, grid, nullptr);
+    run(3, 0, 1, c->counters + CNT_TREE_DEPTH);
 }

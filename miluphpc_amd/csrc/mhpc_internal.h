@@ -54,6 +54,7 @@ enum {
     CNT_NEIGHBOURS = 2,
     CNT_MAX_NOI = 3,
     CNT_H_COUNTS = 4,
+    CNT_TREE_DEPTH = 5,     // deepest level with cells (written by the bottom-up pass)
     CNT_COUNT_ = 8
 };
 
@@ -144,6 +145,9 @@ struct mhpc_ctx {
     int32_t *nn_start = nullptr;
     int nn_kmax = 0;
     bool nn_start_valid = false;
+    // deepest level that had cells in the last finished call: the bottom-up pass launches
+    // the levels below it as one single-workgroup sweep
+    int com_depth_hint = MHPC_LEVELS;
     bool nn_valid = false, rho_valid = false, eos_valid = false;
     std::string last_error;
 };
