@@ -23,6 +23,12 @@ extern "C" {
 
 #define MHPC_API_VERSION 1
 
+/* Barnes-Hut particle groups: the acceptance test and the interaction list are shared by
+   this many consecutive key-sorted particles, one wavefront (the reference's
+   computeForces_v2 shares them per warp of 32, gravity.cu:726-903; computeForces_v1 per
+   particle).  The accelerations are those of the oracle's group walk with this size. */
+#define MHPC_BH_GROUP 64
+
 typedef struct mhpc_ctx mhpc_ctx;
 
 /* Error codes.  Replace gpuErrorcheck's exit (cuda_utilities.cuh:41-46), cudaTerminate

@@ -18,6 +18,7 @@
 #include "mhpc_internal.h"
 
 #define BH_WAVES 2                       // waves (groups) per block
+static_assert(MHPC_BH_GROUP == 64, "one Barnes-Hut group per wavefront");
 #define BH_STACK 768                     // soft stack capacity per wave
 #define BH_STACK_PHYS (BH_STACK + 192)   // hard capacity
 #define BH_LIST 128

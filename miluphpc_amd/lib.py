@@ -40,6 +40,7 @@ RHS_SPECULATIVE_NN = 1 << 6
 RHS_FEATURES = 1 << 7
 
 MLP_NPARAM = 4801   # W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3
+BH_GROUP = 64       # particles sharing one Barnes-Hut acceptance test and list (MHPC_BH_GROUP)
 MLP_HIDDEN = 64
 MLP_IN = 8          # 7 features + 1 zero pad
 

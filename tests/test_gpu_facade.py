@@ -24,7 +24,7 @@ def read_fields(path, names):
     return {k: data[i].copy() for i, k in enumerate(names)}
 
 
-def test_facade_leapfrog_matches_oracle(orc, tmp_path):
+def test_facade_leapfrog_matches_oracle(gpu_lib, orc, tmp_path):
     print_case("c++ facade: Leapfrog 2 steps, 6000 particles")
     out = tmp_path / "grav.bin"
     res = subprocess.run([str(DRIVER), "gravity", "6000", "2", str(out)], capture_output=True, text=True,
@@ -36,7 +36,7 @@ def test_facade_leapfrog_matches_oracle(orc, tmp_path):
     m = f["m"]
 
     def acc(st):
-        r = orc.gravity_rhs({**st, "mass": m}, 0.5, 1e-6, curve=1, mac=0)
+        r = orc.gravity_rhs({**st, "mass": m}, 0.5, 1e-6, curve=1, mac=0, group=gpu_lib.BH_GROUP)
         return r["ax"], r["ay"], r["az"]
 
     st = {k: v.copy() for k, v in s.items()}

@@ -194,7 +194,7 @@ def test_barnes_hut_fp64_parity(gpu_lib, orc, plummer64k, mac):
     box = orc.bbox(s["x"], s["y"], s["z"])
     t = orc.Tree(s["x"], s["y"], s["z"], box)
     t.com(s["mass"])
-    o = orc.bh_group(t, g["sorted"], box, 0.5, 1e-6, mac=mac, group=64, potential=True)
+    o = orc.bh_group(t, g["sorted"], box, 0.5, 1e-6, mac=mac, group=gpu_lib.BH_GROUP, potential=True)
     assert g["stats"]["n_cell_interactions"] == o["n_cell"]
     assert g["stats"]["n_part_interactions"] == o["n_part"]
     err = vec_rel_err(g["g_ax"], g["g_ay"], g["g_az"], o["ax"], o["ay"], o["az"])
@@ -212,7 +212,7 @@ def test_barnes_hut_fp32_quadrupole_parity(gpu_lib, orc, plummer64k):
     box = orc.bbox(s["x"], s["y"], s["z"])
     t = orc.Tree(s["x"], s["y"], s["z"], box)
     t.com(s["mass"])
-    o = orc.bh_group(t, g["sorted"], box, 0.6, 4e-8, mac=0, group=64, quad=True, potential=True)
+    o = orc.bh_group(t, g["sorted"], box, 0.6, 4e-8, mac=0, group=gpu_lib.BH_GROUP, quad=True, potential=True)
     assert g["stats"]["n_cell_interactions"] == o["n_cell"]
     assert g["stats"]["n_part_interactions"] == o["n_part"]
     err = vec_rel_err(g["g_ax"], g["g_ay"], g["g_az"], o["ax"], o["ay"], o["az"])
@@ -257,7 +257,7 @@ def test_leapfrog_matches_oracle(gpu_lib, orc):
     st = {k: s[k].copy() for k in ("x", "y", "z", "vx", "vy", "vz")}
 
     def acc(st):
-        r = orc.gravity_rhs({**st, "mass": s["mass"]}, 0.5, 1e-6, curve=1)
+        r = orc.gravity_rhs({**st, "mass": s["mass"]}, 0.5, 1e-6, curve=1, group=gpu_lib.BH_GROUP)
         return r["ax"], r["ay"], r["az"]
 
     st["g_ax"], st["g_ay"], st["g_az"] = acc(st)
