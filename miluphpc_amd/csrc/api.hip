@@ -107,7 +107,7 @@ static void free_all(mhpc_ctx *c) {
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
                     c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->sdirty,
-                    c->hmax_bits, c->olist, c->ocount, c->hcnt, c->nn_start};
+                    c->hmax_bits, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
@@ -170,6 +170,7 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     c->nn_kmax = 1;
     while (c->nn_kmax < MHPC_NN_KMAX && (1ll << (3 * c->nn_kmax)) < cap) c->nn_kmax++;
     HIPC(dalloc(&c->nn_start, nn_tab_off(c->nn_kmax + 1)));
+    HIPC(dalloc(&c->nn_start_rng, nn_tab_off(c->nn_kmax + 1)));
     HIPC(dalloc(&c->lcp, cap));
     c->tree_tiles_cap = ceil_div(cap, MHPC_TREE_TILE);
     HIPC(dalloc(&c->tcnt, (MHPC_LEVELS + 1) * c->tree_tiles_cap));

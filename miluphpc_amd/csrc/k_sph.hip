@@ -50,6 +50,7 @@ struct NNArgs {
     int64_t cap_cells;
     // start table (levels 1..kmax) and the root box the keys were computed in
     const int32_t *stab;
+    const int2 *srng;         // {first, count} of each table entry's node
     int kmax;
     const double *box;
     // list mode: entry s of particle k is nbr4[(s / 4) * stride + k].{x,y,z,w}[s % 4]
@@ -106,12 +107,7 @@ __device__ __forceinline__ int nn_axis_index(double v, double mn, double mx, int
 #define NN_START_ALL (-1)     // walk start: table cells covering the box
 #define NN_START_PART0 (-2)   // NN_START_PART0 - o: table cells of index = o (mod 8)
 
-// Push the start nodes of a walk for the box [lo, hi] onto L.stack: the deepest table level
-// K <= kmax at which the box spans at most 4 cells per axis, and the table's node for every
-// such cell (a cell of level K, a particle alone in it, or none).  Every particle inside
-// the box lies in one of these cells (the keys use the same bisection), and the nodes'
-// subtrees are disjoint, so the walk sees the same candidates as one from the root, in
-// another order.  With nparts = 8, only cells of index = part (mod 8).  Returns the count.
+// The start level of a walk for the box [lo, hi] (see nn_start_push).
 struct NNStart {
     int K, s;                          // start level K = kmax - s (< 1: whole tree)
     int ilx, ihx, ily, ihy, ilz, ihz;  // the box corners' cell indices at level kmax
@@ -140,11 +136,20 @@ __device__ __forceinline__ NNStart nn_start_level(const NNArgs &A, double lox, d
     return S;
 }
 
+// Start a walk for the box [lo, hi]: the deepest table level K <= kmax at which the box
+// spans at most 4 cells per axis, and the table's node for every such cell (a cell of level
+// K, a particle alone in it, or none).  Every particle inside the box lies in one of these
+// cells (the keys use the same bisection), and the nodes' subtrees are disjoint, so the walk
+// sees the same candidates as one from the root, in another order.  A node of at most
+// NN_LEAF particles goes straight to the range list (its particles are candidates, as for a
+// leaf the walk would take), any other onto the stack.  With nparts = 8, only cells of
+// index = part (mod 8).  Returns the stack count; nr receives the range count.
 template <typename LdsT>
 __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
-                                             double hiy, double loz, double hiz, int part, int nparts) {
+                                             double hiy, double loz, double hiz, int part, int nparts, int &nr) {
     const int lane = wave_lane();
     int node = -1;
+    int2 rng = make_int2(0, 0x7fffffff);
     const NNStart S = nn_start_level(A, lox, hix, loy, hiy, loz, hiz);
     const int K = S.K, s = S.s;
     const int ilx = S.ilx, ihx = S.ihx, ily = S.ily, ihy = S.ihy, ilz = S.ilz, ihz = S.ihz;
@@ -163,11 +168,21 @@ __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lo
                 P = (P << 3) | d;
             }
             node = A.stab[nn_tab_off(K) + (int64_t)P];
+            if (node >= 0) rng = A.srng[nn_tab_off(K) + (int64_t)P];
         }
     }
-    const uint64_t m = wave_ballot(node >= 0);
-    if (node >= 0) L.stack[popc64(m & lanemask_lt())] = node;
-    return popc64(m);
+    const uint64_t lt = lanemask_lt();
+    const bool asRange = node >= 0 && rng.y <= NN_LEAF;
+    const bool asNode = node >= 0 && !asRange;
+    const uint64_t mR = wave_ballot(asRange), mN = wave_ballot(asNode);
+    if (asRange) {
+        const int pos = popc64(mR & lt);
+        L.rstart[pos] = rng.x;
+        L.rcnt[pos] = rng.y;
+    }
+    if (asNode) L.stack[popc64(mN & lt)] = node;
+    nr = popc64(mR);
+    return popc64(mN);
 }
 
 // Walk the (sub)tree of node `start` (the root is node n) for the axis-aligned box [lo, hi]
@@ -188,7 +203,7 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
         if (lane == 0) L.stack[0] = start;
     } else {
         top = nn_start_push(L, A, lox, hix, loy, hiy, loz, hiz, start == NN_START_ALL ? 0 : NN_START_PART0 - start,
-                            start == NN_START_ALL ? 1 : 8);
+                            start == NN_START_ALL ? 1 : 8, nr);
     }
     wave_sync();
     const uint64_t lt = lanemask_lt();
@@ -771,23 +786,30 @@ __global__ __launch_bounds__(256) void k_nn_start_table(int64_t n, int kmax, con
                                                         const uint64_t *__restrict__ slkey,
                                                         const int32_t *__restrict__ cell_base,
                                                         const int32_t *__restrict__ cstart,
+                                                        const int32_t *__restrict__ cend,
                                                         const int32_t *__restrict__ clevel, int64_t cap_cells,
-                                                        int32_t *__restrict__ tab) {
+                                                        int32_t *__restrict__ tab, int2 *__restrict__ rng) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t < n) {
         const int a = t > 0 ? (int)lcp[t - 1] : 0;
         const int b = (int)lcp[t];
         const int lp = (a > b ? a : b) + 1;
         const uint64_t key = slkey[t];
-        for (int l = lp; l <= kmax; l++)
-            tab[nn_tab_off(l) + (int64_t)(key >> (3 * (MHPC_LEVELS - l)))] = (int32_t)t;
+        for (int l = lp; l <= kmax; l++) {
+            const int64_t e = nn_tab_off(l) + (int64_t)(key >> (3 * (MHPC_LEVELS - l)));
+            tab[e] = (int32_t)t;
+            rng[e] = make_int2((int)t, 1);
+        }
     } else {
         const int64_t ncells = cell_base[MHPC_LEVELS + 1];
         const int64_t c = t - n + 1;   // cells 1 .. ncells-1 (cell 0 is the root)
         if (ncells > cap_cells || c >= ncells) return;
         const int L = clevel[c];
         if (L < 1 || L > kmax) return;
-        tab[nn_tab_off(L) + (int64_t)(slkey[cstart[c]] >> (3 * (MHPC_LEVELS - L)))] = (int32_t)(n + c);
+        const int32_t s0 = cstart[c];
+        const int64_t e = nn_tab_off(L) + (int64_t)(slkey[s0] >> (3 * (MHPC_LEVELS - L)));
+        tab[e] = (int32_t)(n + c);
+        rng[e] = make_int2((int)s0, (int)(cend[c] - s0 + 1));
     }
 }
 
@@ -797,7 +819,8 @@ static void ensure_nn_start(mhpc_ctx *c) {
     const uint64_t *slkey = (c->p.curve == MHPC_CURVE_HILBERT) ? c->slkey : c->skey;
     const int64_t items = c->n + c->cap_cells;
     hipLaunchKernelGGL(k_nn_start_table, dim3((unsigned)ceil_div(items, 256)), dim3(256), 0, c->stream, c->n,
-                       c->nn_kmax, c->lcp, slkey, c->cell_base, c->cstart, c->clevel, c->cap_cells, c->nn_start);
+                       c->nn_kmax, c->lcp, slkey, c->cell_base, c->cstart, c->cend, c->clevel, c->cap_cells,
+                       c->nn_start, c->nn_start_rng);
     c->nn_start_valid = true;
 }
 
@@ -805,6 +828,7 @@ static NNArgs nn_common(mhpc_ctx *c) {
     ensure_nn_start(c);
     NNArgs a = {};
     a.stab = c->nn_start;
+    a.srng = c->nn_start_rng;
     a.kmax = c->nn_kmax;
     a.box = c->box;
     a.n = c->n;

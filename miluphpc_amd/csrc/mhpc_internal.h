@@ -143,6 +143,7 @@ struct mhpc_ctx {
     // neighbour-walk start table: for levels 1..nn_kmax, the tree node covering each
     // geometric cell of that level (Lebesgue prefix order), -1 where there is no particle
     int32_t *nn_start = nullptr;
+    int2 *nn_start_rng = nullptr;    // {first sorted particle, count} of each entry's node
     int nn_kmax = 0;
     bool nn_start_valid = false;
     // deepest level that had cells in the last finished call: the bottom-up pass launches
