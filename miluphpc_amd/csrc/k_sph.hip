@@ -79,8 +79,8 @@ struct NNArgs {
     uint32_t *cnt_slot;       // split counting: neighbour count of list slot t
     const uint8_t *only;      // list mode: when set, only particles with only[k] != 0
     uint8_t *dirty;           // dirty marking: particles whose list must be rebuilt
-    const double *hmax_p;     // dirty marking: largest pass-0 h
-    unsigned long long *hmax0_bits;   // pass 0 with lists: largest h (bit pattern, atomicMax)
+    double *h0_slot;          // pass 0 with lists: the pass-0 h of each unconverged particle
+                              // (slot of list_out); read by the dirty marking
     int32_t *out_list;        // group mode: outlier particles deferred to the per-particle mode
     uint32_t *out_count;
     uint32_t *err;
@@ -534,12 +534,6 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
                 A.h_next[k] = hi;
             }
         }
-        if (MODE == 2) {
-            // largest pass-0 h: the search radius of the dirty marking (a particle whose list
-            // from this pass stays in use keeps this h)
-            const double wmax = wave_max(valid ? hi : 0.0);
-            if (lane == 0) atomicMax(A.hmax0_bits, (unsigned long long)__double_as_longlong(wmax));
-        }
         if (A.list_out) {
             // append the unconverged particles (wave-aggregated; list order is free, the
             // next pass's results do not depend on it)
@@ -547,7 +541,11 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             uint32_t b = 0;
             if (lane == 0 && mS) b = atomicAdd(A.count_out, (uint32_t)popc64(mS));
             b = __shfl(b, 0);
-            if (stay) A.list_out[b + popc64(mS & lanemask_lt())] = (int32_t)k;
+            if (stay) {
+                const uint32_t pos = b + popc64(mS & lanemask_lt());
+                A.list_out[pos] = (int32_t)k;
+                if (A.h0_slot) A.h0_slot[pos] = hi;   // the h this pass's list was built with
+            }
         }
         if (lane == 0) atomicAdd(&A.counters[CNT_H_COUNTS], (unsigned long long)popc64(mActive));
     }
@@ -616,7 +614,11 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
                 A.snoi[k] = count;
                 A.h_next[k] = hn;
                 if (fin) A.done[k] = 1;
-                else A.list_out[atomicAdd(A.count_out, 1u)] = (int32_t)k;
+                else {
+                    const uint32_t pos = atomicAdd(A.count_out, 1u);
+                    A.list_out[pos] = (int32_t)k;
+                    if (A.h0_slot) A.h0_slot[pos] = hi;
+                }
                 atomicAdd(&A.counters[CNT_H_COUNTS], 1ull);
             }
         }
@@ -625,30 +627,33 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
 }
 
 // Dirty marking after a speculative pass 0: for every particle j whose h changed after
-// pass 0 (list_in), flag each particle i with d_ij < h_i (the only pairs whose membership
-// d < h_i and d < h_j can differ from pass 0), and j itself.  An unflagged particle keeps
-// its pass-0 h, so searching x_j +- (largest pass-0 h) finds every such i.  One wavefront
-// per changed particle; lanes are candidates.
+// pass 0 (list_in, with its pass-0 h in h0_slot), flag j and every particle i whose pair
+// membership with j changed, (d < min(h_i, h_j0)) != (d < min(h_i, h_j1)) with h_j1 the final
+// h: the only pairs whose membership d < h_i and d < h_j can differ from pass 0 (an unflagged
+// particle keeps its pass-0 h).  Such an i lies within max(h_j0, h_j1) of x_j.  One wavefront
+// per changed particle; lanes are candidates; d is the pair test's (symmetric) value.
 __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
     NNWaveLds &L = lds_all[w];
     const uint32_t cnt = *A.count_in;
     const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
-    const double H = *A.hmax_p;
-    const double grow = H * (1.0 + 1e-9);
     for (int64_t t = (int64_t)blockIdx.x * NN1_WAVES + w; t < cnt; t += nwaves) {
         const int64_t jc = A.list_in[t];
         const double4 own = A.sxm[jc];
+        const double h0 = A.h0_slot[t], h1 = A.h[jc];
+        const double h02 = h0 * h0, h12 = h1 * h1;
+        const double grow = (h0 > h1 ? h0 : h1) * (1.0 + 1e-9);
         if (lane == 0) A.dirty[jc] = 1;
         const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, NN_START_ALL, KeepAll(), [&](int m) {
+                                           own.z + grow, NN_START_ALL, KeepAll(), [&](int m) {
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
                 const double dy = own.y - L.ty[lane];
                 const double dz = own.z - L.tz[lane];
                 const double d = dx * dx + dy * dy + dz * dz;
-                if (d < L.th2[lane]) A.dirty[L.tj[lane]] = 1;
+                const double hi2 = L.th2[lane];
+                if (((d < hi2) & (d < h02)) != ((d < hi2) & (d < h12))) A.dirty[L.tj[lane]] = 1;
             }
         });
         if (!ok) {
@@ -847,7 +852,7 @@ static unsigned nn_grid(mhpc_ctx *c) { return (unsigned)ceil_div(ceil_div(c->n, 
 
 // done flags and iteration counts to zero; the per-pass list counters to zero
 __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *list_counts, int nlc,
-                         unsigned long long *hmax0_bits, uint32_t *cnt_slot) {
+                         uint32_t *cnt_slot) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k < n) {
         done[k] = 0;
@@ -855,7 +860,6 @@ __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *lis
         cnt_slot[k] = 0u;
     }
     if (k < nlc) list_counts[k] = 0;
-    if (k == 0) *hmax0_bits = 0ull;
 }
 
 // a9: the plain h-iteration as Jacobi passes; sh holds the final h afterwards.  Pass 0
@@ -864,8 +868,7 @@ __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *lis
 void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_h_init, dim3((unsigned)ceil_div(c->n > 64 ? c->n : 64, 256)), dim3(256), 0, c->stream,
-                       c->n, c->sdone, c->siters, c->hlist_count, MHPC_MAX_PASSES + 1, c->hmax_bits,
-                       c->hcnt);
+                       c->n, c->sdone, c->siters, c->hlist_count, MHPC_MAX_PASSES + 1, c->hcnt);
     NNArgs a = nn_common(c);
     a.done = c->sdone;
     a.iters = c->siters;
@@ -883,6 +886,7 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     a.h_next = c->sh2;
     a.list_out = c->hlist[2];
     a.count_out = c->hlist_count + 0;
+    a.h0_slot = c->h0slot;
     const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), NN1_MAX_BLOCKS);
     launch_fill_u32(c, c->ocount, 0u, 1);
     a.out_list = c->olist;
@@ -890,7 +894,6 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     if (speculative) {
         a.nbr4 = reinterpret_cast<int4 *>(c->nbr);
         a.stride = c->cap_pad;
-        a.hmax0_bits = c->hmax_bits;
         hipLaunchKernelGGL(k_nn_walk<2>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
     } else {
         hipLaunchKernelGGL(k_nn_walk<1>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
@@ -906,6 +909,7 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     }
     a.nbr4 = nullptr;
     a.out_list = nullptr;
+    a.h0_slot = nullptr;   // the pass-0 slots stay for the dirty marking
     std::swap(c->sh, c->sh2);
     // passes 1.., per-particle mode over the list of the previous pass
     for (int pass = 1; pass < passes; pass++) {
@@ -934,7 +938,7 @@ void launch_neighbours_fixup(mhpc_ctx *c) {
     a.list_in = c->hlist[2];
     a.count_in = c->hlist_count + 0;
     a.dirty = c->sdirty;
-    a.hmax_p = reinterpret_cast<const double *>(c->hmax_bits);
+    a.h0_slot = c->h0slot;
     const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), NN1_MAX_BLOCKS);
     hipLaunchKernelGGL(k_nn_mark_dirty, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
     NNArgs b = nn_common(c);

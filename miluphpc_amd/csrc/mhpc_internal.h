@@ -115,7 +115,7 @@ struct mhpc_ctx {
     uint8_t *sdirty = nullptr;                   // speculative lists: particles to re-list
     int32_t *olist = nullptr;                    // neighbour search: group-mode outliers
     uint32_t *ocount = nullptr;
-    unsigned long long *hmax_bits = nullptr;     // largest h (bit pattern)
+    double *h0slot = nullptr;                    // pass-0 h of the unconverged particles (list slot)
     uint32_t *hlist_count = nullptr;             // [MHPC_MAX_PASSES + 1]
     uint32_t *hcnt = nullptr;                    // neighbour count per list slot
 
