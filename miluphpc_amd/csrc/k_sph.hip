@@ -20,6 +20,7 @@
 #define NN_LEAF 16
 #define NN_OUTLIER 2.0       // group-mode outlier: h above this factor x the group's smallest
 #define NN_OUTLIER_MAX 16    // at most this many outliers per group are split off
+#define NN_SPARSE 8          // re-lists: groups with at most this many flagged particles go per particle
 
 // per-wave LDS of the walker; TCAP = 128 gives the filtered walk room to compact the
 // candidates of two 64-particle loads before a full tile of 64 is handed on
@@ -405,6 +406,15 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     uint64_t mActive = wave_ballot(active);
     if (mActive == 0) {
         if (MODE >= 1 && valid) A.h_next[k] = A.h[k];
+        return;
+    }
+    if (MODE == 0 && A.only && A.out_list && popc64(mActive) <= NN_SPARSE) {
+        // a re-list of a few flagged particles: 64 lanes testing candidates for one or two of
+        // them is wasted, so they go to the per-particle mode (lanes = candidates)
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(A.out_count, (uint32_t)popc64(mActive));
+        b = __shfl(b, 0);
+        if (active) A.out_list[b + popc64(mActive & lanemask_lt())] = (int32_t)k;
         return;
     }
     const double4 own = valid ? A.sxm[k] : make_double4(0.0, 0.0, 0.0, 0.0);
