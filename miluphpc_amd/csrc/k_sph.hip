@@ -78,6 +78,8 @@ struct NNArgs {
     int32_t *list_out;
     uint32_t *count_out;
     uint32_t *cnt_slot;       // split counting: neighbour count of list slot t
+    const int32_t *slot_list; // split counting: the list slots it takes
+    const uint32_t *slot_count;
     const uint8_t *only;      // list mode: when set, only particles with only[k] != 0
     uint8_t *dirty;           // dirty marking: particles whose list must be rebuilt
     double *h0_slot;          // pass 0 with lists: the pass-0 h of each unconverged particle
@@ -716,21 +718,123 @@ __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, int64_t n) {
     if (k < n) p[k] = 0;
 }
 
-// Counting pass of the h-iteration for the listed particles, split eight ways: wavefront
-// (t, o) counts particle list_in[t]'s neighbours under the start cells of index = o
-// (mod 8) (the root's child o when the box is too large for the start table) and adds them
-// to cnt_slot[t] (integer sums: exact and order-free).  Eight short walks per particle keep
-// more wavefronts in flight than one long one (measured faster even for boxes of a few
-// start cells), and a particle with a large h (a free-boundary particle whose h keeps
-// growing) is shared by eight wavefronts; k_h_finish then applies the update.
+// Counting pass of the h-iteration for the listed particles, direct form: one wavefront per
+// particle takes the start cells of its box x +- h (see nn_start_push) as particle ranges
+// and tests every particle of them, the lanes each taking candidates 64 apart with all their
+// loads issued before the tests (no tree walk, no LDS tiles).  A box with a start node too
+// large for that (a particle with a large h) is appended to hard_list for k_nn_count_split.
+// The count goes to cnt_slot[t] (integer, exact).
+#define NN_DIRECT_MAX 1024    // candidates a wavefront takes in the direct form
+__global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_direct(NNArgs A) {
+    __shared__ NNWaveLds lds_all[NN1_WAVES];
+    const int w = threadIdx.x / 64, lane = wave_lane();
+    NNWaveLds &L = lds_all[w];
+    const uint32_t cnt = *A.count_in;
+    const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
+    const uint64_t lt = lanemask_lt();
+    for (int64_t t = (int64_t)blockIdx.x * NN1_WAVES + w; t < cnt; t += nwaves) {
+        const int64_t k = A.list_in[t];
+        const double4 own = A.sxm[k];
+        const double hi = A.h[k];
+        const double hi2 = hi * hi;
+        const double grow = hi * (1.0 + 1e-9);
+        const NNStart S = nn_start_level(A, own.x - grow, own.x + grow, own.y - grow, own.y + grow,
+                                         own.z - grow, own.z + grow);
+        bool hard = S.K < 1;
+        int first = 0, num = 0;
+        if (!hard) {
+            const int s = S.s, K = S.K;
+            const int bx = S.ilx >> s, by = S.ily >> s, bz = S.ilz >> s;
+            const int nx = (S.ihx >> s) - bx + 1, ny = (S.ihy >> s) - by + 1, nz = (S.ihz >> s) - bz + 1;
+            if (lane < nx * ny * nz) {
+                const int ix = bx + lane % nx, iy = by + (lane / nx) % ny, iz = bz + lane / (nx * ny);
+                uint64_t P = 0;
+                for (int l = K - 1; l >= 0; l--) {
+                    const uint64_t d = (uint64_t)(((ix >> l) & 1) ^ 1) | ((uint64_t)(((iy >> l) & 1) ^ 1) << 1) |
+                                       ((uint64_t)(((iz >> l) & 1) ^ 1) << 2);
+                    P = (P << 3) | d;
+                }
+                const int64_t e = nn_tab_off(K) + (int64_t)P;
+                if (A.stab[e] >= 0) {
+                    const int2 r = A.srng[e];
+                    first = r.x;
+                    num = r.y;
+                }
+            }
+        }
+        // the cells' ranges, compacted, with their exclusive prefix
+        const uint64_t mR = wave_ballot(num > 0);
+        const int incl = wave_incl_scan(num);
+        const int total = __shfl(incl, 63);
+        hard = hard || total > NN_DIRECT_MAX;
+        if (hard) {
+            if (lane == 0) A.list_out[atomicAdd(A.count_out, 1u)] = (int32_t)t;   // hard_list (slots)
+            continue;
+        }
+        const int nr = popc64(mR);
+        if (num > 0) {
+            const int pos = popc64(mR & lt);
+            L.rstart[pos] = first;
+            L.rcnt[pos] = num;
+            L.rpre[pos] = incl - num;
+        }
+        wave_sync();
+        int count = 0;
+        for (int base = 0; base < total; base += 256) {
+            int jj[4];
+            double4 pj[4];
+            double hj[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int c = base + 64 * u + lane;
+                jj[u] = -1;
+                if (c < total) {
+                    int lo = 0, hi_ = nr - 1;
+                    while (lo < hi_) {
+                        const int mid = (lo + hi_ + 1) >> 1;
+                        if (L.rpre[mid] <= c) lo = mid; else hi_ = mid - 1;
+                    }
+                    jj[u] = L.rstart[lo] + (c - L.rpre[lo]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (jj[u] >= 0) {
+                    pj[u] = A.sxm[jj[u]];
+                    hj[u] = A.h[jj[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (jj[u] >= 0) {
+                    const double dx = own.x - pj[u].x;
+                    const double dy = own.y - pj[u].y;
+                    const double dz = own.z - pj[u].z;
+                    const double d = dx * dx + dy * dy + dz * dz;
+                    count += (d < hi2 && d < hj[u] * hj[u] && jj[u] != (int)k) ? 1 : 0;
+                }
+            }
+        }
+        count = wave_sum(count);
+        if (lane == 0) A.cnt_slot[t] = (uint32_t)count;
+        wave_sync();
+    }
+}
+
+// Counting pass of the h-iteration for the particles the direct form left (slots in
+// slot_list), split eight ways: wavefront (t, o) counts particle list_in[t]'s neighbours
+// under the start cells of index = o (mod 8) (the root's child o when the box is too large
+// for the start table) and adds them to cnt_slot[t] (integer sums: exact and order-free).
+// A particle with a large h (a free-boundary particle whose h keeps growing) is so shared
+// by eight wavefronts; k_h_finish then applies the update.
 __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
     NNWaveLds &L = lds_all[w];
-    const int64_t work = 8 * (int64_t)*A.count_in;
+    const int64_t work = 8 * (int64_t)*A.slot_count;
     const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
     for (int64_t tt = (int64_t)blockIdx.x * NN1_WAVES + w; tt < work; tt += nwaves) {
-        const int64_t t = tt >> 3;
+        const int64_t t = A.slot_list[tt >> 3];
         const int o = (int)(tt & 7);
         const int64_t k = A.list_in[t];
         const double4 own = A.sxm[k];
@@ -931,6 +1035,14 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
         a.count_out = c->hlist_count + pass;
         a.h_next = c->sh;   // k_h_finish updates h in place once every count is done
         a.cnt_slot = c->hcnt;
+        // direct counts; the particles they cannot take go to the split walk
+        NNArgs d = a;
+        d.list_out = c->hhard;
+        d.count_out = c->hhard_count;
+        launch_fill_u32(c, c->hhard_count, 0u, 1);
+        hipLaunchKernelGGL(k_nn_count_direct, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, d);
+        a.slot_list = c->hhard;
+        a.slot_count = c->hhard_count;
         hipLaunchKernelGGL(k_nn_count_split, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
         hipLaunchKernelGGL(k_h_finish, dim3(256), dim3(256), 0, c->stream, a);
     }
