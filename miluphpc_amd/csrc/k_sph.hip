@@ -21,10 +21,14 @@
 #define NN_OUTLIER 2.0       // group-mode outlier: h above this factor x the group's smallest
 #define NN_OUTLIER_MAX 16    // at most this many outliers per group are split off
 #define NN_SPARSE 8          // re-lists: groups with at most this many flagged particles go per particle
+#define NN_DRANGES 256       // direct form: candidate ranges per wavefront
+#define NN_DSPAN 5           // direct form: start cells per axis - 1 (<= 216 cells)
+#define NN_DIRECT_MAX 32768  // direct form: candidates per wavefront
+#define NN_DU 8              // direct form: candidates per lane and round (loads in flight)
 
 // per-wave LDS of the walker; TCAP = 128 gives the filtered walk room to compact the
 // candidates of two 64-particle loads before a full tile of 64 is handed on
-template <int TCAP>
+template <int TCAP, int DR>
 struct NNWaveLdsT {
     int stack[NN_STACK_PHYS];
     int rstart[NN_RANGES];
@@ -32,9 +36,10 @@ struct NNWaveLdsT {
     int rpre[64];
     double tx[TCAP], ty[TCAP], tz[TCAP], th2[TCAP];
     int tj[TCAP];
+    int drs[DR], drp[DR];   // direct form: candidate ranges (first particle, exclusive prefix)
 };
-using NNWaveLds = NNWaveLdsT<64>;
-using NNWaveLdsF = NNWaveLdsT<128>;
+using NNWaveLds = NNWaveLdsT<64, NN_DRANGES>;
+using NNWaveLdsF = NNWaveLdsT<128, 1>;
 
 // candidate filter that keeps everything (per-particle walks test each candidate once)
 struct KeepAll {
@@ -116,6 +121,7 @@ struct NNStart {
     int ilx, ihx, ily, ihy, ilz, ihz;  // the box corners' cell indices at level kmax
 };
 
+template <int SPAN = 3>
 __device__ __forceinline__ NNStart nn_start_level(const NNArgs &A, double lox, double hix, double loy,
                                                   double hiy, double loz, double hiz) {
     NNStart S = {};
@@ -130,8 +136,8 @@ __device__ __forceinline__ NNStart nn_start_level(const NNArgs &A, double lox, d
         S.ilz = nn_axis_index(loz, b4, b5, K0);
         S.ihz = nn_axis_index(hiz, b4, b5, K0);
         int s = 0;
-        while (s < K0 && ((S.ihx >> s) - (S.ilx >> s) > 3 || (S.ihy >> s) - (S.ily >> s) > 3 ||
-                          (S.ihz >> s) - (S.ilz >> s) > 3))
+        while (s < K0 && ((S.ihx >> s) - (S.ilx >> s) > SPAN || (S.ihy >> s) - (S.ily >> s) > SPAN ||
+                          (S.ihz >> s) - (S.ilz >> s) > SPAN))
             s++;
         S.s = s;
     }
@@ -186,6 +192,91 @@ __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lo
     if (asNode) L.stack[popc64(mN & lt)] = node;
     nr = popc64(mR);
     return popc64(mN);
+}
+
+// Direct form of a per-particle search for the box [lo, hi]: the start cells of the box (up
+// to NN_DSPAN + 1 per axis, at the deepest table level where they cover it) are taken whole as
+// particle ranges -- no tree walk -- and every particle of them is handed to
+// visit(valid, j, {x,y,z,m}_j, h_j), NN_DU candidates per lane and round with all their loads
+// issued first; candidates come in ascending order of (round, slot, lane), and visit is called
+// by all lanes together (it may use ballots).  With nparts > 1 only the start cells of index =
+// part (mod nparts).  Returns false, before any visit, when the box is too large for the
+// table, has more than NN_DRANGES non-empty cells or more than NN_DIRECT_MAX candidates.
+template <typename LdsT, typename VisitFn>
+__device__ __forceinline__ bool nn_direct(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
+                                          double hiy, double loz, double hiz, int part, int nparts,
+                                          VisitFn &&visit) {
+    const int lane = wave_lane();
+    const uint64_t lt = lanemask_lt();
+    const NNStart S = nn_start_level<NN_DSPAN>(A, lox, hix, loy, hiy, loz, hiz);
+    if (S.K < 1) return false;
+    const int K = S.K, s = S.s;
+    const int bx = S.ilx >> s, by = S.ily >> s, bz = S.ilz >> s;
+    const int nx = (S.ihx >> s) - bx + 1, ny = (S.ihy >> s) - by + 1, nz = (S.ihz >> s) - bz + 1;
+    const int ncell = nx * ny * nz;
+    int nr = 0, total = 0;
+    for (int c0 = 0; c0 < ncell; c0 += 64) {
+        const int idx = c0 + lane;
+        int first = 0, num = 0;
+        if (idx < ncell && idx % nparts == part) {
+            const int ix = bx + idx % nx, iy = by + (idx / nx) % ny, iz = bz + idx / (nx * ny);
+            uint64_t P = 0;
+            for (int l = K - 1; l >= 0; l--) {
+                const uint64_t d = (uint64_t)(((ix >> l) & 1) ^ 1) | ((uint64_t)(((iy >> l) & 1) ^ 1) << 1) |
+                                   ((uint64_t)(((iz >> l) & 1) ^ 1) << 2);
+                P = (P << 3) | d;
+            }
+            const int64_t e = nn_tab_off(K) + (int64_t)P;
+            if (A.stab[e] >= 0) {
+                const int2 r = A.srng[e];
+                first = r.x;
+                num = r.y;
+            }
+        }
+        const uint64_t m = wave_ballot(num > 0);
+        const int incl = wave_incl_scan(num);
+        const int tot = __shfl(incl, 63);
+        if (nr + popc64(m) > NN_DRANGES || total + tot > NN_DIRECT_MAX) return false;
+        if (num > 0) {
+            const int pos = nr + popc64(m & lt);
+            L.drs[pos] = first;
+            L.drp[pos] = total + incl - num;
+        }
+        nr += popc64(m);
+        total += tot;
+    }
+    wave_sync();
+    for (int base = 0; base < total; base += 64 * NN_DU) {
+        int jj[NN_DU];
+#pragma unroll
+        for (int u = 0; u < NN_DU; u++) {
+            const int c = base + 64 * u + lane;
+            jj[u] = -1;
+            if (c < total) {
+                int lo = 0, hi_ = nr - 1;
+                while (lo < hi_) {
+                    const int mid = (lo + hi_ + 1) >> 1;
+                    if (L.drp[mid] <= c) lo = mid; else hi_ = mid - 1;
+                }
+                jj[u] = L.drs[lo] + (c - L.drp[lo]);
+            }
+        }
+        double4 pj[NN_DU];
+        double hj[NN_DU];
+#pragma unroll
+        for (int u = 0; u < NN_DU; u++) {
+            pj[u] = make_double4(0.0, 0.0, 0.0, 0.0);
+            hj[u] = 0.0;
+            if (jj[u] >= 0) {
+                pj[u] = A.sxm[jj[u]];
+                hj[u] = A.h[jj[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NN_DU; u++) visit(jj[u] >= 0, jj[u], pj[u], hj[u]);
+    }
+    wave_sync();
+    return true;
 }
 
 // Walk the (sub)tree of node `start` (the root is node n) for the axis-aligned box [lo, hi]
@@ -587,26 +678,37 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
         const double hi2 = hi * hi;
         const double grow = hi * (1.0 + 1e-9);
         int count = 0;
-        const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, NN_START_ALL, KeepAll(), [&](int m) {
+        // one candidate per lane: pair test, then the hits appended in lane order
+        auto take = [&](bool v, int j, double x, double y, double z, double h2) {
             bool hit = false;
-            if (lane < m) {
-                const double dx = own.x - L.tx[lane];
-                const double dy = own.y - L.ty[lane];
-                const double dz = own.z - L.tz[lane];
+            if (v) {
+                const double dx = own.x - x;
+                const double dy = own.y - y;
+                const double dz = own.z - z;
                 const double d = dx * dx + dy * dy + dz * dz;
-                hit = d < hi2 && d < L.th2[lane] && L.tj[lane] != (int)k;
+                hit = d < hi2 && d < h2 && j != (int)k;
             }
             const uint64_t mh = wave_ballot(hit);
             if (MODE != 1 && hit) {
                 const int s = count + popc64(mh & lt);
-                if (s < A.max_nb) nbr32[((int64_t)(s >> 2) * A.stride + k) * 4 + (s & 3)] = L.tj[lane];
+                if (s < A.max_nb) nbr32[((int64_t)(s >> 2) * A.stride + k) * 4 + (s & 3)] = j;
             }
             count += popc64(mh);
-        });
-        if (!ok) {
-            if (lane == 0) set_err(A.err, ERRF_STACK);
-            return;
+        };
+        const double lox = own.x - grow, hix = own.x + grow, loy = own.y - grow, hiy = own.y + grow;
+        const double loz = own.z - grow, hiz = own.z + grow;
+        const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, 0, 1,
+                                      [&](bool v, int j, const double4 &p, double h) { take(v, j, p.x, p.y, p.z, h * h); });
+        if (!direct) {
+            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, KeepAll(), [&](int m) {
+                const bool v = lane < m;
+                take(v, v ? L.tj[lane] : -1, v ? L.tx[lane] : 0.0, v ? L.ty[lane] : 0.0, v ? L.tz[lane] : 0.0,
+                     v ? L.th2[lane] : 0.0);
+            });
+            if (!ok) {
+                if (lane == 0) set_err(A.err, ERRF_STACK);
+                return;
+            }
         }
         if (MODE == 0) {
             if (lane == 0) {
@@ -657,20 +759,27 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
         const double h02 = h0 * h0, h12 = h1 * h1;
         const double grow = (h0 > h1 ? h0 : h1) * (1.0 + 1e-9);
         if (lane == 0) A.dirty[jc] = 1;
-        const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                           own.z + grow, NN_START_ALL, KeepAll(), [&](int m) {
-            if (lane < m) {
-                const double dx = own.x - L.tx[lane];
-                const double dy = own.y - L.ty[lane];
-                const double dz = own.z - L.tz[lane];
+        auto mark = [&](bool v, int i, double x, double y, double z, double hi2) {
+            if (v) {
+                const double dx = own.x - x;
+                const double dy = own.y - y;
+                const double dz = own.z - z;
                 const double d = dx * dx + dy * dy + dz * dz;
-                const double hi2 = L.th2[lane];
-                if (((d < hi2) & (d < h02)) != ((d < hi2) & (d < h12))) A.dirty[L.tj[lane]] = 1;
+                if (((d < hi2) & (d < h02)) != ((d < hi2) & (d < h12))) A.dirty[i] = 1;
             }
-        });
-        if (!ok) {
-            if (lane == 0) set_err(A.err, ERRF_STACK);
-            return;
+        };
+        const double lox = own.x - grow, hix = own.x + grow, loy = own.y - grow, hiy = own.y + grow;
+        const double loz = own.z - grow, hiz = own.z + grow;
+        const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, 0, 1,
+                                      [&](bool v, int i, const double4 &p, double h) { mark(v, i, p.x, p.y, p.z, h * h); });
+        if (!direct) {
+            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, KeepAll(), [&](int m) {
+                if (lane < m) mark(true, L.tj[lane], L.tx[lane], L.ty[lane], L.tz[lane], L.th2[lane]);
+            });
+            if (!ok) {
+                if (lane == 0) set_err(A.err, ERRF_STACK);
+                return;
+            }
         }
         wave_sync();
     }
@@ -718,102 +827,37 @@ __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, int64_t n) {
     if (k < n) p[k] = 0;
 }
 
-// Counting pass of the h-iteration for the listed particles, direct form: one wavefront per
-// particle takes the start cells of its box x +- h (see nn_start_push) as particle ranges
-// and tests every particle of them, the lanes each taking candidates 64 apart with all their
-// loads issued before the tests (no tree walk, no LDS tiles).  A box with a start node too
-// large for that (a particle with a large h) is appended to hard_list for k_nn_count_split.
-// The count goes to cnt_slot[t] (integer, exact).
-#define NN_DIRECT_MAX 1024    // candidates a wavefront takes in the direct form
+// Counting pass of the h-iteration for the listed particles, direct form (nn_direct): one
+// wavefront per particle tests every particle of the start cells of its box x +- h, with no
+// tree walk.  A box the direct form declines (too large for the table, too many cells or
+// candidates) is appended to hard_list for k_nn_count_split.  The count goes to cnt_slot[t]
+// (integer, exact).
 __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_direct(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
     NNWaveLds &L = lds_all[w];
     const uint32_t cnt = *A.count_in;
     const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
-    const uint64_t lt = lanemask_lt();
     for (int64_t t = (int64_t)blockIdx.x * NN1_WAVES + w; t < cnt; t += nwaves) {
         const int64_t k = A.list_in[t];
         const double4 own = A.sxm[k];
         const double hi = A.h[k];
         const double hi2 = hi * hi;
         const double grow = hi * (1.0 + 1e-9);
-        const NNStart S = nn_start_level(A, own.x - grow, own.x + grow, own.y - grow, own.y + grow,
-                                         own.z - grow, own.z + grow);
-        bool hard = S.K < 1;
-        int first = 0, num = 0;
-        if (!hard) {
-            const int s = S.s, K = S.K;
-            const int bx = S.ilx >> s, by = S.ily >> s, bz = S.ilz >> s;
-            const int nx = (S.ihx >> s) - bx + 1, ny = (S.ihy >> s) - by + 1, nz = (S.ihz >> s) - bz + 1;
-            if (lane < nx * ny * nz) {
-                const int ix = bx + lane % nx, iy = by + (lane / nx) % ny, iz = bz + lane / (nx * ny);
-                uint64_t P = 0;
-                for (int l = K - 1; l >= 0; l--) {
-                    const uint64_t d = (uint64_t)(((ix >> l) & 1) ^ 1) | ((uint64_t)(((iy >> l) & 1) ^ 1) << 1) |
-                                       ((uint64_t)(((iz >> l) & 1) ^ 1) << 2);
-                    P = (P << 3) | d;
-                }
-                const int64_t e = nn_tab_off(K) + (int64_t)P;
-                if (A.stab[e] >= 0) {
-                    const int2 r = A.srng[e];
-                    first = r.x;
-                    num = r.y;
-                }
+        int count = 0;
+        const bool direct = nn_direct(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
+                                      own.z + grow, 0, 1, [&](bool v, int j, const double4 &p, double h) {
+            if (v) {
+                const double dx = own.x - p.x;
+                const double dy = own.y - p.y;
+                const double dz = own.z - p.z;
+                const double d = dx * dx + dy * dy + dz * dz;
+                count += (d < hi2 && d < h * h && j != (int)k) ? 1 : 0;
             }
-        }
-        // the cells' ranges, compacted, with their exclusive prefix
-        const uint64_t mR = wave_ballot(num > 0);
-        const int incl = wave_incl_scan(num);
-        const int total = __shfl(incl, 63);
-        hard = hard || total > NN_DIRECT_MAX;
-        if (hard) {
+        });
+        if (!direct) {
             if (lane == 0) A.list_out[atomicAdd(A.count_out, 1u)] = (int32_t)t;   // hard_list (slots)
             continue;
-        }
-        const int nr = popc64(mR);
-        if (num > 0) {
-            const int pos = popc64(mR & lt);
-            L.rstart[pos] = first;
-            L.rcnt[pos] = num;
-            L.rpre[pos] = incl - num;
-        }
-        wave_sync();
-        int count = 0;
-        for (int base = 0; base < total; base += 256) {
-            int jj[4];
-            double4 pj[4];
-            double hj[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int c = base + 64 * u + lane;
-                jj[u] = -1;
-                if (c < total) {
-                    int lo = 0, hi_ = nr - 1;
-                    while (lo < hi_) {
-                        const int mid = (lo + hi_ + 1) >> 1;
-                        if (L.rpre[mid] <= c) lo = mid; else hi_ = mid - 1;
-                    }
-                    jj[u] = L.rstart[lo] + (c - L.rpre[lo]);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                if (jj[u] >= 0) {
-                    pj[u] = A.sxm[jj[u]];
-                    hj[u] = A.h[jj[u]];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                if (jj[u] >= 0) {
-                    const double dx = own.x - pj[u].x;
-                    const double dy = own.y - pj[u].y;
-                    const double dz = own.z - pj[u].z;
-                    const double d = dx * dx + dy * dy + dz * dz;
-                    count += (d < hi2 && d < hj[u] * hj[u] && jj[u] != (int)k) ? 1 : 0;
-                }
-            }
         }
         count = wave_sum(count);
         if (lane == 0) A.cnt_slot[t] = (uint32_t)count;
