@@ -201,7 +201,8 @@ __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lo
 // issued first; candidates come in ascending order of (round, slot, lane), and visit is called
 // by all lanes together (it may use ballots).  With nparts > 1 only the start cells of index =
 // part (mod nparts).  Returns false, before any visit, when the box is too large for the
-// table, has more than NN_DRANGES non-empty cells or more than NN_DIRECT_MAX candidates.
+// table, has more than NN_DRANGES non-empty cells or more than NN_DIRECT_MAX candidates
+// (counted over the whole box, so every part of a split search decides alike).
 template <typename LdsT, typename VisitFn>
 __device__ __forceinline__ bool nn_direct(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
                                           double hiy, double loz, double hiz, int part, int nparts,
@@ -214,11 +215,11 @@ __device__ __forceinline__ bool nn_direct(LdsT &L, const NNArgs &A, double lox, 
     const int bx = S.ilx >> s, by = S.ily >> s, bz = S.ilz >> s;
     const int nx = (S.ihx >> s) - bx + 1, ny = (S.ihy >> s) - by + 1, nz = (S.ihz >> s) - bz + 1;
     const int ncell = nx * ny * nz;
-    int nr = 0, total = 0;
+    int nr = 0, total = 0, nr_all = 0, total_all = 0;
     for (int c0 = 0; c0 < ncell; c0 += 64) {
         const int idx = c0 + lane;
         int first = 0, num = 0;
-        if (idx < ncell && idx % nparts == part) {
+        if (idx < ncell) {
             const int ix = bx + idx % nx, iy = by + (idx / nx) % ny, iz = bz + idx / (nx * ny);
             uint64_t P = 0;
             for (int l = K - 1; l >= 0; l--) {
@@ -233,10 +234,14 @@ __device__ __forceinline__ bool nn_direct(LdsT &L, const NNArgs &A, double lox, 
                 num = r.y;
             }
         }
+        // the decision to decline is taken on the whole box, so all parts agree
+        nr_all += popc64(wave_ballot(num > 0));
+        total_all += wave_sum(num);
+        if (nr_all > NN_DRANGES || total_all > NN_DIRECT_MAX) return false;
+        if (idx % nparts != part) num = 0;
         const uint64_t m = wave_ballot(num > 0);
         const int incl = wave_incl_scan(num);
         const int tot = __shfl(incl, 63);
-        if (nr + popc64(m) > NN_DRANGES || total + tot > NN_DIRECT_MAX) return false;
         if (num > 0) {
             const int pos = nr + popc64(m & lt);
             L.drs[pos] = first;
@@ -829,7 +834,8 @@ __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, int64_t n) {
 
 // Counting pass of the h-iteration for the listed particles, direct form (nn_direct): one
 // wavefront per particle tests every particle of the start cells of its box x +- h, with no
-// tree walk.  A box the direct form declines (too large for the table, too many cells or
+// tree walk.  (Sharing large boxes among several wavefronts gained 0.07 ms on an evolved state
+// and cost more than that on the others: the extra wavefronts of the small boxes.)  A box the direct form declines (too large for the table, too many cells or
 // candidates) is appended to hard_list for k_nn_count_split.  The count goes to cnt_slot[t]
 // (integer, exact).
 __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_direct(NNArgs A) {
