@@ -10,10 +10,12 @@
 // tile against its own particle with the reference criterion d^2 < h_i^2 and
 // d^2 < h_j^2 (FP64, exact), so the neighbour sets equal a brute-force search.
 #include <algorithm>
+#include <type_traits>
 
 #include "mhpc_internal.h"
 
 #define NN_WAVES 2
+#define NN_PB 4      // group walk, list modes: candidates read from LDS per batch
 #define NN_STACK 512
 #define NN_STACK_PHYS (NN_STACK + 192)
 #define NN_RANGES 128
@@ -560,8 +562,16 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     };
 
     int count = 0;
-    int b0 = 0, b1 = 0, b2 = 0;   // pending list entries, flushed four at a time as one int4
+    // the last three hits, oldest first (a shift register): on every fourth hit they and the
+    // new entry go out as one int4; the stale values left after a store are shifted out by
+    // the next three hits before they are stored again
+    int p0 = 0, p1 = 0, p2 = 0;
     int4 *wp = A.nbr4 + k;        // where the next four go
+    // the count & 3 pending entries, in order, for the final partial int4
+    auto tail4 = [&]() {
+        const int r = count & 3;
+        return r == 1 ? make_int4(p2, 0, 0, 0) : r == 2 ? make_int4(p1, p2, 0, 0) : make_int4(p0, p1, p2, 0);
+    };
     const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, keep, [&](int m) {
         if (!active) return;
         if (MODE == 1) {
@@ -574,45 +584,67 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
             }
             return;
         }
-        // list modes: the append is predicated (selects into the three pending entries);
-        // only the 16-byte store of every fourth entry is a branch
-        for (int e = 0; e < m; e++) {
-            const double dx = xi - L.tx[e];
-            const double dy = yi - L.ty[e];
-            const double dz = zi - L.tz[e];
-            const double d = dx * dx + dy * dy + dz * dz;
-            const int j = L.tj[e];
-            const bool hit = (d < hi2) & (d < L.th2[e]) & (j != (int)k);
-            const int slot = count & 3;
-            b0 = (hit && slot == 0) ? j : b0;
-            b1 = (hit && slot == 1) ? j : b1;
-            b2 = (hit && slot == 2) ? j : b2;
-            if (hit && slot == 3) {
-                if (count < A.max_nb) *wp = make_int4(b0, b1, b2, j);
-                wp += A.stride;
+        // list modes: candidates come as LDS broadcasts, NN_PB per batch with all their reads
+        // issued first (the appends below are branches, across which the compiler does not
+        // move loads).  The append is predicated (three selects shift the pending entries);
+        // only the 16-byte store of every fourth entry is a branch.  The capacity check of
+        // the stores is needed only when some lane could reach max_nb within this tile.
+        auto pairs = [&](auto check_cap) {
+            for (int e = 0; e < m; e += NN_PB) {
+                int js[NN_PB];
+                double xs[NN_PB], ys[NN_PB], zs[NN_PB], h2s[NN_PB];
+#pragma unroll
+                for (int u = 0; u < NN_PB; u++) {
+                    const int eu = e + u < m ? e + u : m - 1;   // m >= 1: a tile is never empty
+                    xs[u] = L.tx[eu];
+                    ys[u] = L.ty[eu];
+                    zs[u] = L.tz[eu];
+                    h2s[u] = L.th2[eu];
+                    js[u] = L.tj[eu];
+                }
+#pragma unroll
+                for (int u = 0; u < NN_PB; u++) {
+                    if (e + u < m) {
+                        const int j = js[u];
+                        const double dx = xi - xs[u];
+                        const double dy = yi - ys[u];
+                        const double dz = zi - zs[u];
+                        const double d = dx * dx + dy * dy + dz * dz;
+                        const bool hit = (d < hi2) & (d < h2s[u]) & (j != (int)k);
+                        if (hit && (count & 3) == 3) {
+                            if (!decltype(check_cap)::value || count < A.max_nb) *wp = make_int4(p0, p1, p2, j);
+                            wp += A.stride;
+                        }
+                        p0 = hit ? p1 : p0;
+                        p1 = hit ? p2 : p1;
+                        p2 = hit ? j : p2;
+                        count += hit ? 1 : 0;
+                    }
+                }
             }
-            count += hit ? 1 : 0;
-        }
+        };
+        if (wave_ballot(count + m > A.max_nb) == 0) pairs(std::false_type{});
+        else pairs(std::true_type{});
     });
     if (!ok) {
         if (lane == 0) set_err(A.err, ERRF_STACK);
         return;
     }
     if (MODE == 2 && active && (count & 3) != 0 && count < A.max_nb)
-        A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, 0);
+        A.nbr4[(int64_t)(count >> 2) * A.stride + k] = tail4();
     if (MODE == 0 && A.only) {
         // re-list of flagged particles: counts and caller-order outputs are published for
         // all particles by k_publish_noi afterwards
         if (active) {
             if ((count & 3) != 0 && count < A.max_nb)
-                A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, 0);
+                A.nbr4[(int64_t)(count >> 2) * A.stride + k] = tail4();
             A.snoi[k] = count;
             if (count > A.max_nb) set_err(A.err, ERRF_CAP_NEIGHBOURS);
         }
     } else if (MODE == 0) {
         const bool own_list = valid && !outlier;
         if (own_list && (count & 3) != 0 && count < A.max_nb)
-            A.nbr4[(int64_t)(count >> 2) * A.stride + k] = make_int4(b0, b1, b2, 0);
+            A.nbr4[(int64_t)(count >> 2) * A.stride + k] = tail4();
         if (own_list) {
             A.snoi[k] = count;
             const int32_t i = A.perm[k];
