@@ -15,6 +15,7 @@
 #include "mhpc_internal.h"
 
 #define NN_WAVES 2
+#define NN_CWIN 512   // group walk, list modes: the consumers' LDS window (in-window neighbours first)
 #define NN_PB 4      // group walk, list modes: candidates read from LDS per batch
 #define NN_STACK 512
 #define NN_STACK_PHYS (NN_STACK + 192)
@@ -147,6 +148,23 @@ __device__ __forceinline__ NNStart nn_start_level(const NNArgs &A, double lox, d
     return S;
 }
 
+// Consumer window of a list walk: the particles [w0, w1) of the density and force kernels'
+// LDS window that holds the walking group (w1 - w0 >= NN_LEAF; w0 == w1: none).  Their
+// candidates are streamed first as one range, so every list holds its in-window neighbours
+// first, and the tree walk skips them: a node or range inside the window is dropped (false)
+// and a range of at most NN_LEAF particles is clipped to its part outside (it crosses at most
+// one end of the window).  Larger nodes that cross an end are opened as usual.
+__device__ __forceinline__ bool nn_win_clip(int w0, int w1, int &rs, int &rc) {
+    if (w0 == w1) return true;
+    const int64_t e = (int64_t)rs + rc;
+    if (rs >= w0 && e <= w1) return false;
+    if (rc <= NN_LEAF) {
+        if (rs < w0 && e > w0) rc = w0 - rs;
+        else if (rs < w1 && e > w1) { rc = (int)(e - w1); rs = w1; }
+    }
+    return true;
+}
+
 // Start a walk for the box [lo, hi]: the deepest table level K <= kmax at which the box
 // spans at most 4 cells per axis, and the table's node for every such cell (a cell of level
 // K, a particle alone in it, or none).  Every particle inside the box lies in one of these
@@ -154,10 +172,12 @@ __device__ __forceinline__ NNStart nn_start_level(const NNArgs &A, double lox, d
 // sees the same candidates as one from the root, in another order.  A node of at most
 // NN_LEAF particles goes straight to the range list (its particles are candidates, as for a
 // leaf the walk would take), any other onto the stack.  With nparts = 8, only cells of
-// index = part (mod 8).  Returns the stack count; nr receives the range count.
+// index = part (mod 8).  With a consumer window [w0, w1), its range comes first and the
+// nodes are clipped against it.  Returns the stack count; nr receives the range count.
 template <typename LdsT>
 __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
-                                             double hiy, double loz, double hiz, int part, int nparts, int &nr) {
+                                             double hiy, double loz, double hiz, int part, int nparts, int w0,
+                                             int w1, int &nr) {
     const int lane = wave_lane();
     int node = -1;
     int2 rng = make_int2(0, 0x7fffffff);
@@ -182,17 +202,23 @@ __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lo
             if (node >= 0) rng = A.srng[nn_tab_off(K) + (int64_t)P];
         }
     }
+    if (node >= 0 && !nn_win_clip(w0, w1, rng.x, rng.y)) node = -1;
+    const int r0 = w1 > w0 ? 1 : 0;   // range 0: the window
+    if (r0 && lane == 0) {
+        L.rstart[0] = w0;
+        L.rcnt[0] = w1 - w0;
+    }
     const uint64_t lt = lanemask_lt();
     const bool asRange = node >= 0 && rng.y <= NN_LEAF;
     const bool asNode = node >= 0 && !asRange;
     const uint64_t mR = wave_ballot(asRange), mN = wave_ballot(asNode);
     if (asRange) {
-        const int pos = popc64(mR & lt);
+        const int pos = r0 + popc64(mR & lt);
         L.rstart[pos] = rng.x;
         L.rcnt[pos] = rng.y;
     }
     if (asNode) L.stack[popc64(mN & lt)] = node;
-    nr = popc64(mR);
+    nr = r0 + popc64(mR);
     return popc64(mN);
 }
 
@@ -295,8 +321,8 @@ __device__ __forceinline__ bool nn_direct(LdsT &L, const NNArgs &A, double lox, 
 // dropped candidates).  Returns false on stack overflow.  Wave-uniform control flow.
 template <bool FILTER, typename LdsT, typename KeepFn, typename TileFn>
 __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
-                                            double hiy, double loz, double hiz, int start, KeepFn &&keep,
-                                            TileFn &&tile) {
+                                            double hiy, double loz, double hiz, int start, int w0, int w1,
+                                            KeepFn &&keep, TileFn &&tile) {
     const int lane = wave_lane();
     const int64_t n = A.n;
     int top = 1, nr = 0;
@@ -304,7 +330,7 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
         if (lane == 0) L.stack[0] = start;
     } else {
         top = nn_start_push(L, A, lox, hix, loy, hiy, loz, hiz, start == NN_START_ALL ? 0 : NN_START_PART0 - start,
-                            start == NN_START_ALL ? 1 : 8, nr);
+                            start == NN_START_ALL ? 1 : 8, w0, w1, nr);
     }
     wave_sync();
     const uint64_t lt = lanemask_lt();
@@ -328,6 +354,7 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
                 take = px >= lox && px <= hix && py >= loy && py <= hiy && pz >= loz && pz <= hiz;
                 rs = node;
                 rc = 1;
+                take = take && nn_win_clip(w0, w1, rs, rc);
             } else if (node >= n) {
                 c = node - n;
                 const int64_t cc = A.cap_cells;
@@ -337,7 +364,9 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
                 if (overlap) {
                     rs = A.cstart[c];
                     rc = A.cend[c] - rs + 1;
-                    if (rc <= NN_LEAF) take = true; else open = true;
+                    if (nn_win_clip(w0, w1, rs, rc)) {
+                        if (rc <= NN_LEAF) take = true; else open = true;
+                    }
                 }
             }
             const uint64_t mT = wave_ballot(take);
@@ -572,7 +601,14 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
         const int r = count & 3;
         return r == 1 ? make_int4(p2, 0, 0, 0) : r == 2 ? make_int4(p1, p2, 0, 0) : make_int4(p0, p1, p2, 0);
     };
-    const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, keep, [&](int m) {
+    // list modes: the group's consumer window (NN_CWIN-aligned) is streamed first
+    int w0 = 0, w1 = 0;
+    if (MODE != 1) {
+        w0 = (int)(group * 64 / NN_CWIN * NN_CWIN);
+        w1 = (int)(w0 + NN_CWIN < n ? w0 + NN_CWIN : n);
+        if (w1 - w0 < NN_LEAF) w0 = w1 = 0;
+    }
+    const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, w0, w1, keep, [&](int m) {
         if (!active) return;
         if (MODE == 1) {
             for (int e = 0; e < m; e++) {
@@ -737,7 +773,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
         const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, 0, 1,
                                       [&](bool v, int j, const double4 &p, double h) { take(v, j, p.x, p.y, p.z, h * h); });
         if (!direct) {
-            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, KeepAll(), [&](int m) {
+            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, 0, 0, KeepAll(), [&](int m) {
                 const bool v = lane < m;
                 take(v, v ? L.tj[lane] : -1, v ? L.tx[lane] : 0.0, v ? L.ty[lane] : 0.0, v ? L.tz[lane] : 0.0,
                      v ? L.th2[lane] : 0.0);
@@ -810,7 +846,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
         const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, 0, 1,
                                       [&](bool v, int i, const double4 &p, double h) { mark(v, i, p.x, p.y, p.z, h * h); });
         if (!direct) {
-            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, KeepAll(), [&](int m) {
+            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, 0, 0, KeepAll(), [&](int m) {
                 if (lane < m) mark(true, L.tj[lane], L.tx[lane], L.ty[lane], L.tz[lane], L.th2[lane]);
             });
             if (!ok) {
@@ -926,7 +962,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
         const int start = NN_START_PART0 - o;   // eighth o of the start cells
         int count = 0;
         const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, start, KeepAll(), [&](int m) {
+                                    own.z + grow, start, 0, 0, KeepAll(), [&](int m) {
             bool hit = false;
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
@@ -1582,6 +1618,7 @@ __device__ __forceinline__ void force_pair(const ForceI &I, const double4 &pj, c
 // One block per window of FORCE_WIN consecutive sorted particles, their records staged in
 // LDS (see the density kernel): in-window neighbours are read there, the others gathered.
 #define FORCE_WIN 512
+static_assert(FORCE_WIN == NN_CWIN && DENS_WIN == NN_CWIN, "lists hold the neighbours of the consumers' window first");
 
 template <bool FP64, bool FEAT>
 __global__ __launch_bounds__(FORCE_WIN) void k_forces(ForceArgs A) {
