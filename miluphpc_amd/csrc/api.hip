@@ -107,8 +107,7 @@ static void free_all(mhpc_ctx *c) {
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
                     c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->sdirty,
-                    c->h0slot, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng, c->hhard,
-                    c->hhard_count};
+                    c->h0slot, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng, c->hhard};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
@@ -163,10 +162,9 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->sdirty, cap));
     HIPC(dalloc(&c->h0slot, cap));
     HIPC(dalloc(&c->hhard, cap));
-    HIPC(dalloc(&c->hhard_count, 1));
     HIPC(dalloc(&c->olist, cap));
     HIPC(dalloc(&c->ocount, 4));
-    HIPC(dalloc(&c->hlist_count, MHPC_MAX_PASSES + 1));
+    HIPC(dalloc(&c->hlist_count, 2 * (MHPC_MAX_PASSES + 1)));
     HIPC(dalloc(&c->hcnt, cap));
     // start table levels: the smallest K with 8^K >= cap (one geometric cell per particle on
     // average at the deepest level), 1 <= K <= MHPC_NN_KMAX

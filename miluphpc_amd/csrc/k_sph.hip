@@ -734,7 +734,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
 // MODE 2: MODE 1 plus the list; MODE 0: the list only.  Lists are written in candidate
 // order, one 4-byte entry per lane.
 #define NN1_WAVES 4
-#define NN1_MAX_BLOCKS 8192   // one wavefront per listed particle up to 32768 particles
+#define NN1_MAX_BLOCKS 1024   // about one resident round: 4096 wavefronts loop over the list
 template <int MODE>
 __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
@@ -1100,7 +1100,7 @@ __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *lis
 void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_h_init, dim3((unsigned)ceil_div(c->n > 64 ? c->n : 64, 256)), dim3(256), 0, c->stream,
-                       c->n, c->sdone, c->siters, c->hlist_count, MHPC_MAX_PASSES + 1, c->hcnt);
+                       c->n, c->sdone, c->siters, c->hlist_count, 2 * (MHPC_MAX_PASSES + 1), c->hcnt);
     NNArgs a = nn_common(c);
     a.done = c->sdone;
     a.iters = c->siters;
@@ -1154,13 +1154,13 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
         a.h_next = c->sh;   // k_h_finish updates h in place once every count is done
         a.cnt_slot = c->hcnt;
         // direct counts; the particles they cannot take go to the split walk
+        uint32_t *hard_count = c->hlist_count + MHPC_MAX_PASSES + 1 + pass;   // zeroed by k_h_init
         NNArgs d = a;
         d.list_out = c->hhard;
-        d.count_out = c->hhard_count;
-        launch_fill_u32(c, c->hhard_count, 0u, 1);
+        d.count_out = hard_count;
         hipLaunchKernelGGL(k_nn_count_direct, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, d);
         a.slot_list = c->hhard;
-        a.slot_count = c->hhard_count;
+        a.slot_count = hard_count;
         hipLaunchKernelGGL(k_nn_count_split, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
         hipLaunchKernelGGL(k_h_finish, dim3(256), dim3(256), 0, c->stream, a);
     }

@@ -117,8 +117,8 @@ struct mhpc_ctx {
     uint32_t *ocount = nullptr;
     double *h0slot = nullptr;                    // pass-0 h of the unconverged particles (list slot)
     int32_t *hhard = nullptr;                    // tail passes: slots left to the split counts
-    uint32_t *hhard_count = nullptr;
-    uint32_t *hlist_count = nullptr;             // [MHPC_MAX_PASSES + 1]
+    uint32_t *hlist_count = nullptr;             // [2 (MHPC_MAX_PASSES + 1)]: list lengths of the
+                                                 // passes, then the hard-list lengths of the passes
     uint32_t *hcnt = nullptr;                    // neighbour count per list slot
 
     // learned smoothing length
