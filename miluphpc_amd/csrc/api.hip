@@ -107,7 +107,7 @@ static void free_all(mhpc_ctx *c) {
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
                     c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->sdirty,
-                    c->h0slot, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng, c->hhard};
+                    c->h0slot, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng, c->hhard, c->xyzm};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
@@ -143,6 +143,7 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->slkey, cap));
     HIPC(dalloc(&c->perm, cap));
     HIPC(dalloc(&c->valB, cap));
+    HIPC(dalloc(&c->xyzm, cap));
     c->sort_tiles_cap = ceil_div(cap, MHPC_SORT_TILE);
     HIPC(dalloc(&c->tile_hist, MHPC_SORT_BUCKETS * c->sort_tiles_cap));
     HIPC(dalloc(&c->tile_off, MHPC_SORT_BUCKETS * c->sort_tiles_cap));

@@ -139,12 +139,16 @@ __device__ __forceinline__ uint64_t lebesgue_key(double x, double y, double z,
     return key;
 }
 
+// Also writes each particle's {x,y,z,m} as one 32-byte record in caller order, so that the
+// gather into key order reads one record per particle instead of four scattered doubles.
 __global__ __launch_bounds__(256) void k_keys(const double *__restrict__ x,
                                               const double *__restrict__ y,
-                                              const double *__restrict__ z, int64_t n,
+                                              const double *__restrict__ z,
+                                              const double *__restrict__ m, int64_t n,
                                               const double *__restrict__ box, int curve,
                                               uint64_t *__restrict__ key,
-                                              uint64_t *__restrict__ lkey) {
+                                              uint64_t *__restrict__ lkey,
+                                              double4 *__restrict__ xyzm) {
     __shared__ unsigned char fsm[96];
     if (threadIdx.x < 96)
         fsm[threadIdx.x] = (unsigned char)((&c_hilbert_table[0][0])[threadIdx.x] |
@@ -152,7 +156,9 @@ __global__ __launch_bounds__(256) void k_keys(const double *__restrict__ x,
     __syncthreads();
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint64_t leb = lebesgue_key(x[i], y[i], z[i], box);
+    const double xi = x[i], yi = y[i], zi = z[i];
+    xyzm[i] = make_double4(xi, yi, zi, m[i]);
+    uint64_t leb = lebesgue_key(xi, yi, zi, box);
     if (curve == MHPC_CURVE_HILBERT) {
         uint64_t h = 0;
         int dir = 0;
@@ -173,17 +179,18 @@ __global__ __launch_bounds__(256) void k_keys(const double *__restrict__ x,
 void launch_keys(mhpc_ctx *c) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_keys, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream,
-                       c->f[MHPC_F_X], c->f[MHPC_F_Y], c->f[MHPC_F_Z], c->n, c->box, c->p.curve,
-                       c->key, c->lkey);
+                       c->f[MHPC_F_X], c->f[MHPC_F_Y], c->f[MHPC_F_Z], c->f[MHPC_F_MASS], c->n, c->box,
+                       c->p.curve, c->key, c->lkey, c->xyzm);
 }
 
 // ------------------------------------------------------------------------------------
-// gather into key order: the packed records {x,y,z,m} (and for SPH {vx,vy,vz,0}) plus
-// the scalar fields a phase reads.  Writes are coalesced; reads follow perm.
+// gather into key order: the packed records {x,y,z,m} (from the caller-order copy the key
+// kernel wrote) and for SPH {vx,vy,vz,0}, plus the scalar fields a phase reads.  Writes are
+// coalesced; reads follow perm.
 struct GatherArgs {
     const int32_t *perm;
     int64_t n;
-    const double *x, *y, *z, *m;
+    const double4 *xyzm;
     double4 *sxm;
     const double *vx, *vy, *vz;
     double4 *sv4;               // null when SPH is off
@@ -199,7 +206,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     if (k >= a.n) return;
     const int32_t i = a.perm[k];
     if (i < 0 || (int64_t)i >= a.n) return;
-    a.sxm[k] = make_double4(a.x[i], a.y[i], a.z[i], a.m[i]);
+    a.sxm[k] = a.xyzm[i];
     if (a.sv4) a.sv4[k] = make_double4(a.vx[i], a.vy[i], a.vz[i], 0.0);
 #pragma unroll
     for (int f = 0; f < 6; f++)
@@ -212,7 +219,7 @@ void launch_gather(mhpc_ctx *c, bool gravity, bool sph) {
     GatherArgs a = {};
     a.perm = c->perm;
     a.n = c->n;
-    a.x = c->f[MHPC_F_X]; a.y = c->f[MHPC_F_Y]; a.z = c->f[MHPC_F_Z]; a.m = c->f[MHPC_F_MASS];
+    a.xyzm = c->xyzm;
     a.sxm = c->sxm;
     int nf = 0;
     auto add = [&](int field, double *dst) {

@@ -73,6 +73,7 @@ struct mhpc_ctx {
     // sort
     uint64_t *skey = nullptr, *keyB = nullptr;   // sorted keys / ping-pong buffer
     int32_t *perm = nullptr, *valB = nullptr;    // sorted position -> particle index
+    double4 *xyzm = nullptr;                     // {x,y,z,m} in caller order (written with the keys)
     uint32_t *tile_hist = nullptr, *tile_off = nullptr;
     uint32_t *digit_hist = nullptr;              // [passes][256]
     int64_t sort_tiles_cap = 0;
