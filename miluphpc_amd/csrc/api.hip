@@ -107,7 +107,7 @@ static void free_all(mhpc_ctx *c) {
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
                     c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->sdirty,
-                    c->h0slot, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng, c->hhard, c->xyzm};
+                    c->h0slot, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng, c->hhard, c->xyzm, c->ppar, c->cpar};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
@@ -184,6 +184,8 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->cstart, cap_cells));
     HIPC(dalloc(&c->cend, cap_cells));
     HIPC(dalloc(&c->clevel, cap_cells));
+    HIPC(dalloc(&c->cpar, cap_cells));
+    HIPC(dalloc(&c->ppar, cap));
     HIPC(dalloc(&c->cchild, 8 * cap_cells));
     HIPC(dalloc(&c->cmass, cap_cells));
     HIPC(dalloc(&c->cx, cap_cells));

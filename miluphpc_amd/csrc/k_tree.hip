@@ -11,9 +11,11 @@
 //   scan   -- per level, exclusive scan over tiles (one workgroup per level);
 //   cells  -- every position ranks its start / end events among the level's starts
 //             (ballot prefix sums) and writes cell start, end, level: cells come out in
-//             level-major order, sorted by start within a level;
-//   link   -- every cell and particle finds its parent by binary search over the
-//             parent level's starts and fills the parent's child slot (octant = Lebesgue
+//             level-major order, sorted by start within a level.  The same prefix sums
+//             give, for every level P, the level-P cell containing the position (index
+//             cell_base[P] + level-P starts at or before it - 1), hence each cell's and
+//             each particle's parent;
+//   link   -- every cell and particle fills its parent's child slot (octant = Lebesgue
 //             digit, the reference's slot convention).
 // No locks, no retries, and the same input always gives the same numbering.
 // The bottom-up pass then runs level 21..0: mass, centre of mass (children in slot
@@ -70,7 +72,8 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_cells(
     const uint8_t *__restrict__ lcp, int64_t n, const uint32_t *__restrict__ toff, int64_t ntiles,
     const int32_t *__restrict__ level_count, int32_t *__restrict__ cell_base_out,
     int32_t *__restrict__ cstart, int32_t *__restrict__ cend, int32_t *__restrict__ clevel,
-    int32_t *__restrict__ cchild, int64_t cap_cells, uint32_t *err) {
+    int32_t *__restrict__ cchild, int32_t *__restrict__ ppar, int32_t *__restrict__ cpar, int64_t cap_cells,
+    uint32_t *err) {
     __shared__ int32_t cb[MHPC_LEVELS + 3];
     __shared__ uint32_t pre[TREE_ITEMS][4][MHPC_LEVELS + 1];
     const int tid = threadIdx.x, w = tid / 64, lane = wave_lane();
@@ -123,46 +126,43 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_cells(
 #pragma unroll
     for (int s = 0; s < TREE_ITEMS; s++) {
         const int64_t k = base + s * TREE_BLOCK + tid;
+        // the particle's parent is the deepest cell containing it, of level P = max(a, b)
+        const int P = max(a[s], b[s]);
+        int64_t cont_prev = 0;   // the level-(L-1) cell containing k (level 0: the root)
         for (int L = 1; L <= lmax; L++) {
             const bool st = a[s] < L && L <= b[s];
             const bool en = b[s] < L && L <= a[s];
             const uint64_t m = wave_ballot(st);
-            if (st || en) {
-                const int64_t before =
-                    (int64_t)toff[(int64_t)L * ntiles + blockIdx.x] + pre[s][w][L] + popc64(m & lt);
-                if (st) {
-                    const int64_t c = cb[L] + before;
-                    if (c < cap_cells) {
-                        cstart[c] = (int32_t)k;
-                        clevel[c] = L;
+            // level-L starts before k (global); with k's own start, the containing cell's rank
+            const int64_t before =
+                (int64_t)toff[(int64_t)L * ntiles + blockIdx.x] + pre[s][w][L] + popc64(m & lt);
+            const int64_t cont = cb[L] + before + (st ? 1 : 0) - 1;   // valid for L <= P
+            if (st) {
+                const int64_t c = cb[L] + before;
+                if (c < cap_cells) {
+                    cstart[c] = (int32_t)k;
+                    clevel[c] = L;
+                    cpar[c] = (int32_t)cont_prev;
 #pragma unroll
-                        for (int q = 0; q < 8; q++) cchild[8 * c + q] = -1;
-                    }
-                } else {
-                    const int64_t c = cb[L] + before - 1;
-                    if (c < cap_cells) cend[c] = (int32_t)k;
+                    for (int q = 0; q < 8; q++) cchild[8 * c + q] = -1;
                 }
+            } else if (en) {
+                const int64_t c = cb[L] + before - 1;
+                if (c < cap_cells) cend[c] = (int32_t)k;
             }
+            if (L == P && k < n) ppar[k] = (int32_t)(cont < cap_cells ? cont : cap_cells);
+            cont_prev = cont;
         }
+        if (P == 0 && k < n) ppar[k] = 0;
     }
-}
-
-// last cell of level P whose start is <= k (cells of a level are sorted by start)
-__device__ __forceinline__ int32_t find_cell(const int32_t *__restrict__ cstart,
-                                             const int32_t *__restrict__ cb, int P, int32_t k) {
-    int32_t lo = cb[P], hi = cb[P + 1] - 1;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (cstart[mid] <= k) lo = mid; else hi = mid - 1;
-    }
-    return lo;
 }
 
 __global__ __launch_bounds__(TREE_BLOCK) void k_tree_link(
     const uint8_t *__restrict__ lcp, const uint64_t *__restrict__ slkey,
     const int32_t *__restrict__ perm, int64_t n, const int32_t *__restrict__ cell_base,
-    const int32_t *__restrict__ cstart, const int32_t *__restrict__ clevel,
-    int32_t *__restrict__ cchild, int32_t *__restrict__ plevel, int64_t cap_cells) {
+    const int32_t *__restrict__ cstart, const int32_t *__restrict__ clevel, const int32_t *__restrict__ ppar,
+    const int32_t *__restrict__ cpar, int32_t *__restrict__ cchild, int32_t *__restrict__ plevel,
+    int64_t cap_cells) {
     __shared__ int32_t cb[MHPC_LEVELS + 3];
     if (threadIdx.x < MHPC_LEVELS + 3) cb[threadIdx.x] = cell_base[threadIdx.x];
     __syncthreads();
@@ -176,8 +176,7 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_link(
         const int Lp = max(a, b) + 1;
         plevel[perm[k]] = Lp;
         if (Lp > MHPC_LEVELS) return;   // duplicate key, flagged by k_tree_count
-        const int P = Lp - 1;
-        const int32_t parent = (P == 0) ? 0 : find_cell(cstart, cb, P, k);
+        const int32_t parent = ppar[k];
         if (parent >= cap_cells) return;
         const int digit = (int)((slkey[k] >> (3 * (MHPC_LEVELS - Lp))) & 7u);
         cchild[8 * (int64_t)parent + digit] = k;
@@ -186,7 +185,7 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_link(
         if (c >= ncells) return;
         const int L = clevel[c];
         const int32_t s = cstart[c];
-        const int32_t parent = (L == 1) ? 0 : find_cell(cstart, cb, L - 1, s);
+        const int32_t parent = cpar[c];
         const int digit = (int)((slkey[s] >> (3 * (MHPC_LEVELS - L))) & 7u);
         cchild[8 * (int64_t)parent + digit] = (int32_t)(n + c);
     }
@@ -204,11 +203,11 @@ void launch_tree(mhpc_ctx *c) {
                        ntiles, c->level_count);
     hipLaunchKernelGGL(k_tree_cells, dim3((unsigned)ntiles), dim3(TREE_BLOCK), 0, c->stream, c->lcp, n,
                        c->toff, ntiles, c->level_count, c->cell_base, c->cstart, c->cend, c->clevel,
-                       c->cchild, c->cap_cells, c->err);
+                       c->cchild, c->ppar, c->cpar, c->cap_cells, c->err);
     const int64_t items = n + c->cap_cells;
     hipLaunchKernelGGL(k_tree_link, dim3((unsigned)ceil_div(items, TREE_BLOCK)), dim3(TREE_BLOCK), 0,
-                       c->stream, c->lcp, slkey, c->perm, n, c->cell_base, c->cstart, c->clevel,
-                       c->cchild, c->level, c->cap_cells);
+                       c->stream, c->lcp, slkey, c->perm, n, c->cell_base, c->cstart, c->clevel, c->ppar,
+                       c->cpar, c->cchild, c->level, c->cap_cells);
 }
 
 // ------------------------------------------------------------------------------------

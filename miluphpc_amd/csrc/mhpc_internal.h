@@ -74,6 +74,8 @@ struct mhpc_ctx {
     uint64_t *skey = nullptr, *keyB = nullptr;   // sorted keys / ping-pong buffer
     int32_t *perm = nullptr, *valB = nullptr;    // sorted position -> particle index
     double4 *xyzm = nullptr;                     // {x,y,z,m} in caller order (written with the keys)
+    int32_t *ppar = nullptr;                     // sorted particle -> parent cell (tree build)
+    int32_t *cpar = nullptr;                     // cell -> parent cell (tree build)
     uint32_t *tile_hist = nullptr, *tile_off = nullptr;
     uint32_t *digit_hist = nullptr;              // [passes][256]
     int64_t sort_tiles_cap = 0;
