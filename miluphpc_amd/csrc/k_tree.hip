@@ -229,9 +229,13 @@ __global__ __launch_bounds__(BU_BLOCK) void k_bottom_up(
     double *__restrict__ cy, double *__restrict__ cz, double *__restrict__ cquad,
     float4 *__restrict__ cquadf, double *__restrict__ cbmin, double *__restrict__ cbmax,
     int64_t cap_cells, int gravity, int quad, unsigned long long *depth_out) {
+    // a cell's eight lanes are consecutive lanes of one wavefront, so they exchange their
+    // terms through LDS with wavefront barriers only; lane s of the cell then sums one
+    // quantity over the eight children in slot order (the sequential loop's rounding)
     __shared__ double term[10][BU_BLOCK];
-    __shared__ double com[3][BU_BLOCK / 8];
-    const int tid = threadIdx.x, sub = tid & 7, g = tid >> 3;
+    __shared__ double red[4][BU_BLOCK / 8];
+    const int tid = threadIdx.x, sub = tid & 7, g = tid >> 3, c8 = tid & ~7;
+    float *quadf = reinterpret_cast<float *>(cquadf);   // 8 floats per cell
     if (depth_out && blockIdx.x == 0 && tid == 0) {
         int d = 0;
         for (int l = 1; l <= MHPC_LEVELS; l++)
@@ -244,7 +248,7 @@ __global__ __launch_bounds__(BU_BLOCK) void k_bottom_up(
     const int64_t nthreads = (c1 - c0) * 8;
     for (int64_t b = (int64_t)blockIdx.x * BU_BLOCK; b < nthreads; b += (int64_t)gridDim.x * BU_BLOCK) {
         const int64_t t = b + tid;
-        const bool valid = t < nthreads;
+        const bool valid = t < nthreads;   // the eight lanes of a cell agree
         const int64_t c = c0 + (t >> 3);
         const int32_t k = valid ? cchild[8 * c + sub] : -1;
         double px = 0.0, py = 0.0, pz = 0.0, pm = 0.0;
@@ -273,70 +277,74 @@ __global__ __launch_bounds__(BU_BLOCK) void k_bottom_up(
             term[4 + a][tid] = lo[a];
             term[7 + a][tid] = hi[a];
         }
-        __syncthreads();
-        if (sub == 0 && valid) {
-            double bmin[3] = {1e300, 1e300, 1e300}, bmax[3] = {-1e300, -1e300, -1e300};
-            double m = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+        wave_sync();
+        if (valid) {
+            if (sub < 4) {
+                // lane 0: mass; lanes 1-3: mass-weighted positions
+                double v = 0.0;
 #pragma unroll
-            for (int s = 0; s < 8; s++) {
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    bmin[a] = fmin(bmin[a], term[4 + a][tid + s]);
-                    bmax[a] = fmax(bmax[a], term[7 + a][tid + s]);
-                }
-                mx += term[1][tid + s];
-                my += term[2][tid + s];
-                mz += term[3][tid + s];
-                m += term[0][tid + s];
-            }
-            cbmin[c] = bmin[0]; cbmin[cap_cells + c] = bmin[1]; cbmin[2 * cap_cells + c] = bmin[2];
-            cbmax[c] = bmax[0]; cbmax[cap_cells + c] = bmax[1]; cbmax[2 * cap_cells + c] = bmax[2];
-            if (gravity) {
-                if (m > 0.0) { mx /= m; my /= m; mz /= m; }
-                cmass[c] = m;
-                cx[c] = mx;
-                cy[c] = my;
-                cz[c] = mz;
-                com[0][g] = mx;
-                com[1][g] = my;
-                com[2][g] = mz;
-            }
-        }
-        __syncthreads();
-        if (gravity && quad) {
-            double T[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            if (k != -1) {
-                const double dx = px - com[0][g], dy = py - com[1][g], dz = pz - com[2][g];
-                const double d2 = dx * dx + dy * dy + dz * dz;
-                T[0] = q[0] + pm * (3.0 * dx * dx - d2);
-                T[1] = q[1] + pm * (3.0 * dx * dy);
-                T[2] = q[2] + pm * (3.0 * dx * dz);
-                T[3] = q[3] + pm * (3.0 * dy * dy - d2);
-                T[4] = q[4] + pm * (3.0 * dy * dz);
-                T[5] = q[5] + pm * (3.0 * dz * dz - d2);
-            }
-#pragma unroll
-            for (int e = 0; e < 6; e++) term[e][tid] = T[e];
-            __syncthreads();
-            if (sub == 0 && valid) {
-                double A[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                for (int s = 0; s < 8; s++) v += term[sub][c8 + s];
+                red[sub][g] = v;
+            } else if (sub < 7) {
+                const int a = sub - 4;
+                double bmin = 1e300, bmax = -1e300;
 #pragma unroll
                 for (int s = 0; s < 8; s++) {
-#pragma unroll
-                    for (int e = 0; e < 6; e++) A[e] += term[e][tid + s];
+                    bmin = fmin(bmin, term[4 + a][c8 + s]);
+                    bmax = fmax(bmax, term[7 + a][c8 + s]);
+                }
+                cbmin[a * cap_cells + c] = bmin;
+                cbmax[a * cap_cells + c] = bmax;
+            }
+        }
+        wave_sync();
+        if (gravity) {
+            if (valid && sub < 4) {
+                const double m = red[0][g];
+                double v = red[sub][g];
+                if (sub == 0) {
+                    cmass[c] = m;
+                } else {
+                    if (m > 0.0) v /= m;
+                    (sub == 1 ? cx : sub == 2 ? cy : cz)[c] = v;
+                    red[sub][g] = v;   // the centre of mass for the quadrupole terms
+                }
+            }
+            wave_sync();
+            if (quad) {
+                double T[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                if (k != -1) {
+                    const double dx = px - red[1][g], dy = py - red[2][g], dz = pz - red[3][g];
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    T[0] = q[0] + pm * (3.0 * dx * dx - d2);
+                    T[1] = q[1] + pm * (3.0 * dx * dy);
+                    T[2] = q[2] + pm * (3.0 * dx * dz);
+                    T[3] = q[3] + pm * (3.0 * dy * dy - d2);
+                    T[4] = q[4] + pm * (3.0 * dy * dz);
+                    T[5] = q[5] + pm * (3.0 * dz * dz - d2);
                 }
 #pragma unroll
-                for (int e = 0; e < 6; e++) cquad[e * cap_cells + c] = A[e];
-                cquadf[2 * c] = make_float4((float)A[0], (float)A[1], (float)A[2], (float)A[3]);
-                cquadf[2 * c + 1] = make_float4((float)A[4], (float)A[5], 0.f, 0.f);
+                for (int e = 0; e < 6; e++) term[e][tid] = T[e];
+                wave_sync();
+                if (valid) {
+                    // lane e < 6: quadrupole component e; lanes 6, 7: the FP32 copy's padding
+                    float f = 0.f;
+                    if (sub < 6) {
+                        double A = 0.0;
+#pragma unroll
+                        for (int s = 0; s < 8; s++) A += term[sub][c8 + s];
+                        cquad[sub * cap_cells + c] = A;
+                        f = (float)A;
+                    }
+                    quadf[8 * c + sub] = f;
+                }
             }
-            __syncthreads();
         }
+        wave_sync();   // the terms are read before the next iteration overwrites them
     }
     if (L_hi != L_lo) __syncthreads();
     }
 }
-
 void launch_bottom_up(mhpc_ctx *c, bool gravity) {
     if (c->n <= 0) return;
     // a level holds fewer cells than particles; eight threads per cell, grid-stride beyond
