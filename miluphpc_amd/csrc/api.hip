@@ -923,7 +923,7 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     mark(7);
     if (hiter) {
         launch_h_iterate(c, c->p.max_counts, spec);
-        launch_scatter_h(c);
+        if (!spec) launch_scatter_h(c);   // with lists from pass 0, the list fix-up publishes h
     }
     mark(8);
     if (spec) launch_neighbours_fixup(c);

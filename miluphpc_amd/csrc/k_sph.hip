@@ -862,16 +862,26 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
 // one block-level sum / max and one pair of atomics per block (a pair per wave would
 // serialise tens of thousands of atomics on one address)
 #define PUBLISH_BLOCKS 512
+// counts (and with sh set, h and iteration counts) back to caller order; neighbour sum and
+// maximum into the counters
 __global__ __launch_bounds__(256) void k_publish_noi(int64_t n, const int32_t *__restrict__ perm,
                                                      const int32_t *__restrict__ snoi, int32_t *__restrict__ noi,
-                                                     unsigned long long *__restrict__ counters) {
+                                                     unsigned long long *__restrict__ counters,
+                                                     const double *__restrict__ sh,
+                                                     const int32_t *__restrict__ siters, double *__restrict__ sml,
+                                                     int32_t *__restrict__ iters) {
     __shared__ unsigned long long ssum[4];
     __shared__ int smax[4];
     unsigned long long sum = 0;
     int mx = 0;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         const int v = snoi[k];
-        noi[perm[k]] = v;
+        const int32_t i = perm[k];
+        noi[i] = v;
+        if (sh) {
+            sml[i] = sh[k];
+            iters[i] = siters[k];
+        }
         sum += (unsigned long long)v;
         mx = v > mx ? v : mx;
     }
@@ -1082,16 +1092,19 @@ static NNArgs nn_common(mhpc_ctx *c) {
 
 static unsigned nn_grid(mhpc_ctx *c) { return (unsigned)ceil_div(ceil_div(c->n, 64), NN_WAVES); }
 
-// done flags and iteration counts to zero; the per-pass list counters to zero
-__global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *list_counts, int nlc,
-                         uint32_t *cnt_slot) {
+// done flags, iteration counts and dirty flags to zero; the per-pass list counters and the
+// outlier counters (pass 0, list fix-up) to zero
+__global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint8_t *dirty, uint32_t *list_counts,
+                         int nlc, uint32_t *ocount, uint32_t *cnt_slot) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k < n) {
         done[k] = 0;
         iters[k] = 0;
+        dirty[k] = 0;
         cnt_slot[k] = 0u;
     }
     if (k < nlc) list_counts[k] = 0;
+    if (k < 2) ocount[k] = 0;
 }
 
 // a9: the plain h-iteration as Jacobi passes; sh holds the final h afterwards.  Pass 0
@@ -1100,7 +1113,8 @@ __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint32_t *lis
 void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_h_init, dim3((unsigned)ceil_div(c->n > 64 ? c->n : 64, 256)), dim3(256), 0, c->stream,
-                       c->n, c->sdone, c->siters, c->hlist_count, 2 * (MHPC_MAX_PASSES + 1), c->hcnt);
+                       c->n, c->sdone, c->siters, c->sdirty, c->hlist_count, 2 * (MHPC_MAX_PASSES + 1), c->ocount,
+                       c->hcnt);
     NNArgs a = nn_common(c);
     a.done = c->sdone;
     a.iters = c->siters;
@@ -1120,9 +1134,8 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     a.count_out = c->hlist_count + 0;
     a.h0_slot = c->h0slot;
     const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), NN1_MAX_BLOCKS);
-    launch_fill_u32(c, c->ocount, 0u, 1);
     a.out_list = c->olist;
-    a.out_count = c->ocount;
+    a.out_count = c->ocount;   // zeroed by k_h_init
     if (speculative) {
         a.nbr4 = reinterpret_cast<int4 *>(c->nbr);
         a.stride = c->cap_pad;
@@ -1170,9 +1183,10 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
 // differ from the list with the final h (the changed particles and every particle within
 // its own h of one), then publish counts.  The lists then equal those of a full list walk
 // with the final h as sets (order may differ).
+// after launch_h_iterate with lists from pass 0 (its k_h_init zeroes the dirty flags and the
+// second outlier counter)
 void launch_neighbours_fixup(mhpc_ctx *c) {
     if (c->n <= 0) return;
-    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream, c->sdirty, c->n);
     NNArgs a = nn_common(c);
     a.h = c->sh;
     a.list_in = c->hlist[2];
@@ -1187,18 +1201,18 @@ void launch_neighbours_fixup(mhpc_ctx *c) {
     b.stride = c->cap_pad;
     b.snoi = c->snoi;
     b.only = c->sdirty;
-    launch_fill_u32(c, c->ocount, 0u, 1);
     b.out_list = c->olist;
-    b.out_count = c->ocount;
+    b.out_count = c->ocount + 1;
     hipLaunchKernelGGL(k_nn_walk<0>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, b);
     b.out_list = nullptr;
     b.list_in = c->olist;
-    b.count_in = c->ocount;
+    b.count_in = c->ocount + 1;
     b.counters = nullptr;   // k_publish_noi counts every particle
     b.noi_out = nullptr;
     hipLaunchKernelGGL(k_nn_one<0>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, b);
     hipLaunchKernelGGL(k_publish_noi, dim3((unsigned)std::min<int64_t>(ceil_div(c->n, 256), PUBLISH_BLOCKS)),
-                       dim3(256), 0, c->stream, c->n, c->perm, c->snoi, c->noi, c->counters);
+                       dim3(256), 0, c->stream, c->n, c->perm, c->snoi, c->noi, c->counters, c->sh, c->siters,
+                       c->f[MHPC_F_SML], c->h_iters);
 }
 
 void launch_neighbours(mhpc_ctx *c) {
