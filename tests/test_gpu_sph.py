@@ -77,6 +77,43 @@ def test_h_iteration_bitexact(gpu_lib, orc, sedov24, cap):
     assert np.array_equal(h, oh)
 
 
+@pytest.mark.parametrize("drop", [7, 507])
+def test_lists_ragged_last_window(gpu_lib, orc, sedov24, drop):
+    """List walks stream each group's 512-particle window of the key order first and clip it
+    out of the tree walk.  With n = 27 * 512 - drop the last window holds 505 particles
+    (partial, crossing ranges clipped) or 5 (fewer than a leaf: streamed by the tree walk
+    alone).  Plain lists and the speculative RHS's lists match the oracle's sets."""
+    print_case(f"ragged last window: sedov 24^3 minus {drop} particles")
+    rng = np.random.default_rng(drop)
+    keep = np.sort(rng.choice(len(sedov24["x"]), len(sedov24["x"]) - drop, replace=False))
+    s = {k: v[keep] for k, v in sedov24.items()}
+    p = gpu_lib.default_params(curve=0)
+    with make_ctx(gpu_lib, s, p) as ctx:
+        ctx.bbox(); ctx.keys(); ctx.sort(); ctx.build_tree()
+        ctx.neighbours()
+        nnl, noi = ctx.neighbour_lists()
+    st, onnl, onoi = orc.nn_grid(s["x"], s["y"], s["z"], s["sml"], 180)
+    assert st == 0
+    assert np.array_equal(noi, onoi)
+    assert sets(nnl, noi) == sets(onnl, onoi)
+    h0 = 1.25 * s["sml"]
+    flags = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML | gpu_lib.RHS_SPECULATIVE_NN
+    with make_ctx(gpu_lib, {**s, "sml": h0}, p) as ctx:
+        ctx.rhs(flags)
+        h = ctx.download("sml")
+        iters = ctx.download("h_iters")
+        lists = ctx.neighbour_lists()
+    st, oh, oiters, onoi = orc.h_iterate(s["x"], s["y"], s["z"], h0)
+    assert st == 0
+    assert np.array_equal(h, oh)
+    assert np.array_equal(iters, oiters)
+    assert (iters > 1).sum() > 10   # re-listing is exercised
+    st, onnl, onoi = orc.nn_grid(s["x"], s["y"], s["z"], h, 180)
+    assert st == 0
+    assert np.array_equal(lists[1], onoi)
+    assert sets(*lists) == sets(onnl, onoi)
+
+
 def clustered(n=16384, target=40.0):
     """Plummer sphere with h from the analytic number density for about `target`
     neighbours: h spans two orders of magnitude, so walks start anywhere from the deepest
