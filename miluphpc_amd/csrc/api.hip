@@ -99,7 +99,8 @@ static int ensure_nbr(mhpc_ctx *c) {
 static void free_all(mhpc_ctx *c) {
     void *ptrs[] = {c->noi, c->level, c->h_iters, c->key, c->lkey, c->skey, c->keyB, c->perm,
                     c->valB, c->tile_hist, c->tile_off, c->digit_hist, c->slkey, c->sxm, c->sv4,
-                    c->sr4, c->sh, c->sh2, c->se, c->srho, c->sp, c->scs, c->snoi, c->siters, c->sdone, c->lcp, c->tcnt, c->toff,
+                    c->sr4, c->sh, c->sh2, c->se, c->srho, c->sp, c->scs, c->snoi, c->siters, c->sdone,
+                    c->lcp, c->tcnt, c->toff,
                     c->level_count, c->cell_base, c->cstart, c->cend, c->clevel, c->cchild,
                     c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax,
                     c->nbr, c->htable, c->bbox_partial, c->box, c->err, c->counters,
@@ -107,7 +108,8 @@ static void free_all(mhpc_ctx *c) {
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
                     c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->sdirty,
-                    c->h0slot, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng, c->hhard, c->xyzm, c->ppar, c->cpar};
+                    c->h0slot, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng, c->hhard,
+                    c->xyzm, c->ppar, c->cpar};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)

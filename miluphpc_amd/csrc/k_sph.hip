@@ -905,17 +905,12 @@ __global__ __launch_bounds__(256) void k_publish_noi(int64_t n, const int32_t *_
     }
 }
 
-__global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, int64_t n) {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k < n) p[k] = 0;
-}
-
 // Counting pass of the h-iteration for the listed particles, direct form (nn_direct): one
 // wavefront per particle tests every particle of the start cells of its box x +- h, with no
 // tree walk.  (Sharing large boxes among several wavefronts gained 0.07 ms on an evolved state
-// and cost more than that on the others: the extra wavefronts of the small boxes.)  A box the direct form declines (too large for the table, too many cells or
-// candidates) is appended to hard_list for k_nn_count_split.  The count goes to cnt_slot[t]
-// (integer, exact).
+// and cost more than that on the others: the extra wavefronts of the small boxes.)  A box the
+// direct form declines (too large for the table, too many cells or candidates) is appended to
+// hard_list for k_nn_count_split.  The count goes to cnt_slot[t] (integer, exact).
 __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_direct(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
