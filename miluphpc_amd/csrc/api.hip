@@ -109,13 +109,14 @@ static void free_all(mhpc_ctx *c) {
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
                     c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->sdirty,
                     c->h0slot, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng, c->hhard,
-                    c->xyzm, c->ppar, c->cpar};
+                    c->xyzm, c->ppar, c->cpar, c->sort_check};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
         if (c->f[f]) (void)hipFree(c->f[f]);
     if (c->h_err) (void)hipHostFree(c->h_err);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->h_sort_check) (void)hipHostFree(c->h_sort_check);
     for (int e = 0; e < MHPC_NUM_EVENTS; e++)
         if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -145,6 +146,7 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->slkey, cap));
     HIPC(dalloc(&c->perm, cap));
     HIPC(dalloc(&c->valB, cap));
+    HIPC(dalloc(&c->sort_check, 1));
     HIPC(dalloc(&c->xyzm, cap));
     c->sort_tiles_cap = ceil_div(cap, MHPC_SORT_TILE);
     HIPC(dalloc(&c->tile_hist, MHPC_SORT_BUCKETS * c->sort_tiles_cap));
@@ -209,6 +211,7 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->counters, CNT_COUNT_));
     HIPC(hipMemset(c->counters, 0, sizeof(unsigned long long) * CNT_COUNT_));
     HIPC(hipHostMalloc((void **)&c->h_err, sizeof(uint32_t), hipHostMallocDefault));
+    HIPC(hipHostMalloc((void **)&c->h_sort_check, sizeof(uint32_t), hipHostMallocDefault));
     HIPC(hipHostMalloc((void **)&c->h_counters, sizeof(unsigned long long) * CNT_COUNT_,
                        hipHostMallocDefault));
     return MHPC_OK;

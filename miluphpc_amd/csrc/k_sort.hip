@@ -18,8 +18,10 @@
 
 __global__ __launch_bounds__(SORT_BLOCK) void k_sort_hist_all(const uint64_t *__restrict__ keys,
                                                               int64_t n,
-                                                              uint32_t *__restrict__ hist) {
+                                                              uint32_t *__restrict__ hist,
+                                                              uint32_t *__restrict__ check_flag) {
     __shared__ uint32_t h[MHPC_SORT_PASSES * MHPC_SORT_BUCKETS];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *check_flag = 0u;
     for (int t = threadIdx.x; t < MHPC_SORT_PASSES * MHPC_SORT_BUCKETS; t += SORT_BLOCK) h[t] = 0;
     __syncthreads();
     int64_t i = blockIdx.x * (int64_t)SORT_BLOCK + threadIdx.x;
@@ -153,6 +155,22 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_scatter(const uint64_t *__r
     }
 }
 
+static void sort_passes(mhpc_ctx *c, int first, int64_t ntiles);
+
+// Adjacent sorted keys equal in the bits the passes sorted (>= shift): their order in the low
+// bits is not established.  Sets *flag.
+__global__ __launch_bounds__(256) void k_sort_check(const uint64_t *__restrict__ skey, int64_t n, int shift,
+                                                    uint32_t *__restrict__ flag) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= 1 && k < n && (skey[k] >> shift) == (skey[k - 1] >> shift)) *flag = 1u;
+}
+
+// The sort only needs the key levels at which particles are distinct.  The previous tree of
+// this context had leaves no deeper than its deepest cell level + 1; with one level of margin
+// the passes covering levels 1 .. depth + 2 usually suffice (4 of 8 on a 128^3 lattice).  Those
+// passes run first; a check over the result finds any adjacent keys still equal in the sorted
+// bits, the host reads it, and only then the full sort runs (from the untouched input keys).
+// The result is the full sort's either way: distinct sorted prefixes fix the whole order.
 void launch_sort(mhpc_ctx *c) {
     const int64_t n = c->n;
     if (n <= 0) return;
@@ -160,10 +178,27 @@ void launch_sort(mhpc_ctx *c) {
     launch_fill_u32(c, c->digit_hist, 0u, MHPC_SORT_PASSES * MHPC_SORT_BUCKETS);
     int hb = (int)(ceil_div(n, SORT_BLOCK * 16) < 1024 ? ceil_div(n, SORT_BLOCK * 16) : 1024);
     hipLaunchKernelGGL(k_sort_hist_all, dim3(hb), dim3(SORT_BLOCK), 0, c->stream, c->key, n,
-                       c->digit_hist);
+                       c->digit_hist, c->sort_check);
+    const int levels = c->com_depth_hint + 2;
+    const int first = levels < MHPC_LEVELS ? 3 * (MHPC_LEVELS - levels) / MHPC_SORT_BITS : 0;
+    if (first > 0) {
+        sort_passes(c, first, ntiles);
+        hipLaunchKernelGGL(k_sort_check, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, c->skey, n,
+                           MHPC_SORT_BITS * first, c->sort_check);
+        if (hipMemcpyAsync(c->h_sort_check, c->sort_check, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream) ==
+                hipSuccess &&
+            hipStreamSynchronize(c->stream) == hipSuccess && *c->h_sort_check == 0u)
+            return;
+    }
+    sort_passes(c, 0, ntiles);
+}
+
+// LSD passes first .. MHPC_SORT_PASSES - 1 from the input keys; the last pass writes skey / perm
+static void sort_passes(mhpc_ctx *c, int first, int64_t ntiles) {
+    const int64_t n = c->n;
     const uint64_t *kin = c->key;
     const int32_t *vin = nullptr;
-    for (int pass = 0; pass < MHPC_SORT_PASSES; pass++) {
+    for (int pass = first; pass < MHPC_SORT_PASSES; pass++) {
         uint64_t *kout = (pass & 1) ? c->skey : c->keyB;
         int32_t *vout = (pass & 1) ? c->perm : c->valB;
         const int shift = MHPC_SORT_BITS * pass;

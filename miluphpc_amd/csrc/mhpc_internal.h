@@ -75,6 +75,8 @@ struct mhpc_ctx {
     int32_t *perm = nullptr, *valB = nullptr;    // sorted position -> particle index
     double4 *xyzm = nullptr;                     // {x,y,z,m} in caller order (written with the keys)
     int32_t *ppar = nullptr;                     // sorted particle -> parent cell (tree build)
+    uint32_t *sort_check = nullptr;              // set when a partial sort left equal sorted prefixes
+    uint32_t *h_sort_check = nullptr;            // pinned host copy
     int32_t *cpar = nullptr;                     // cell -> parent cell (tree build)
     uint32_t *tile_hist = nullptr, *tile_off = nullptr;
     uint32_t *digit_hist = nullptr;              // [passes][256]

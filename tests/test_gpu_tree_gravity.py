@@ -48,6 +48,37 @@ def test_keys_and_sort_bitexact(gpu_lib, orc, plummer64k, curve):
     assert np.array_equal(perm, orc.sort(ko))
 
 
+@pytest.mark.parametrize("curve", [0, 1])
+def test_sort_depth_hint_and_fallback(gpu_lib, orc, curve):
+    """After a tree build the context sorts only the key levels the previous tree needed (plus
+    one), checks that no adjacent keys are left equal in them, and falls back to the full sort
+    when some are.  The order must be the full stable sort's in all three calls: first (no
+    hint), same positions again (partial passes), two particles on one position (fallback)."""
+    print_case(f"sort depth hint and fallback, lattice 16^3, curve {curve}")
+    s = ics.sedov_lattice(16)
+    p = gpu_lib.default_params(curve=curve)
+    with make_ctx(gpu_lib, s, p) as ctx:
+        for step in range(3):
+            if step == 2:
+                s = dict(s)
+                for c in ("x", "y", "z"):
+                    s[c] = s[c].copy()
+                    s[c][1] = s[c][0]
+                ctx.load(x=s["x"], y=s["y"], z=s["z"])
+            box = ctx.bbox()
+            ctx.keys()
+            ctx.sort()
+            k = ctx.download("key")
+            perm = ctx.download("sorted")
+            if step < 2:
+                ctx.build_tree()
+                ctx.centres_of_mass()   # the tree depth the next sort uses
+            ko = orc.keys(s["x"], s["y"], s["z"], box, curve)
+            assert np.array_equal(k, ko), step
+            assert np.array_equal(perm, orc.sort(ko)), step
+        assert k[0] == k[1]
+
+
 def test_keys_golden_fixture(gpu_lib):
     """Lebesgue keys of dyadic points against the committed closed-form vectors."""
     from pathlib import Path
