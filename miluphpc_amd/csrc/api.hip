@@ -422,7 +422,7 @@ extern "C" int mhpc_build_tree(mhpc_ctx *c, float *ms) {
     if (!c) return MHPC_E_ARG;
     if (!c->sorted_valid) return MHPC_E_STATE;
     int st = run_stage(c, ms, [&] {
-        launch_gather(c, true, true);
+        launch_gather(c, true, true, false);
         launch_tree(c);
         launch_bottom_up(c, false);
         return (int)MHPC_OK;
@@ -918,7 +918,7 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     mark(2);
     launch_sort(c);
     mark(3);
-    launch_gather(c, grav, sph);
+    launch_gather(c, grav, sph, true);
     mark(4);
     launch_tree(c);
     mark(5);

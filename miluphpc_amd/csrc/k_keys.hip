@@ -214,7 +214,10 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     if (a.slkey) a.slkey[k] = a.lkey[i];
 }
 
-void launch_gather(mhpc_ctx *c, bool gravity, bool sph) {
+// rhs: the RHS's own gather, after which the density kernel recomputes rho, and p and cs
+// (with the EOS) -- their previous values are gathered only when the EOS reads them (a sound
+// speed from the previous p, or an EOS that keeps the previous sound speed)
+void launch_gather(mhpc_ctx *c, bool gravity, bool sph, bool rhs) {
     if (c->n <= 0) return;
     GatherArgs a = {};
     a.perm = c->perm;
@@ -232,9 +235,13 @@ void launch_gather(mhpc_ctx *c, bool gravity, bool sph) {
         a.sv4 = c->sv4;
         add(MHPC_F_SML, c->sh);
         add(MHPC_F_E, c->se);
-        add(MHPC_F_P, c->sp);
-        add(MHPC_F_CS, c->scs);
-        add(MHPC_F_RHO, c->srho);
+        const bool eos_fresh =
+            !c->p.cs_before_p && (c->p.eos == MHPC_EOS_IDEAL_GAS || c->p.eos == MHPC_EOS_POLYTROPIC);
+        if (!rhs || !eos_fresh) {
+            add(MHPC_F_P, c->sp);
+            add(MHPC_F_CS, c->scs);
+        }
+        if (!rhs) add(MHPC_F_RHO, c->srho);
     }
     (void)gravity;
     a.nf = nf;

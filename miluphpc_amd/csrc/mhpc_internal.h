@@ -167,7 +167,7 @@ void launch_fill_u64(mhpc_ctx *c, unsigned long long *p, unsigned long long v, i
 void launch_bbox(mhpc_ctx *c);
 void launch_keys(mhpc_ctx *c);
 void launch_sort(mhpc_ctx *c);
-void launch_gather(mhpc_ctx *c, bool gravity, bool sph);
+void launch_gather(mhpc_ctx *c, bool gravity, bool sph, bool rhs);
 void launch_tree(mhpc_ctx *c);
 void launch_bottom_up(mhpc_ctx *c, bool gravity);
 void launch_gravity(mhpc_ctx *c, bool potential);
