@@ -172,12 +172,13 @@ __device__ __forceinline__ bool nn_win_clip(int w0, int w1, int &rs, int &rc) {
 // sees the same candidates as one from the root, in another order.  A node of at most
 // NN_LEAF particles goes straight to the range list (its particles are candidates, as for a
 // leaf the walk would take), any other onto the stack.  With nparts = 8, only cells of
-// index = part (mod 8).  With a consumer window [w0, w1), its range comes first and the
-// nodes are clipped against it.  Returns the stack count; nr receives the range count.
+// index = part (mod 8).  With a consumer window [w0, w1), its ranges come first -- the walking
+// group's own particles [g0, g0 + 64) before the rest of the window -- and the nodes are
+// clipped against it.  Returns the stack count; nr receives the range count.
 template <typename LdsT>
 __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
                                              double hiy, double loz, double hiz, int part, int nparts, int w0,
-                                             int w1, int &nr) {
+                                             int w1, int g0, int &nr) {
     const int lane = wave_lane();
     int node = -1;
     int2 rng = make_int2(0, 0x7fffffff);
@@ -203,10 +204,24 @@ __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lo
         }
     }
     if (node >= 0 && !nn_win_clip(w0, w1, rng.x, rng.y)) node = -1;
-    const int r0 = w1 > w0 ? 1 : 0;   // range 0: the window
-    if (r0 && lane == 0) {
-        L.rstart[0] = w0;
-        L.rcnt[0] = w1 - w0;
+    int r0 = 0;   // the window's ranges: own group, then the window before and after it
+    if (w1 > w0) {
+        const int g1 = g0 + 64 < w1 ? g0 + 64 : w1;
+        if (lane == 0) {
+            L.rstart[0] = g0;
+            L.rcnt[0] = g1 - g0;
+            int r = 1;
+            if (g0 > w0) {
+                L.rstart[r] = w0;
+                L.rcnt[r] = g0 - w0;
+                r++;
+            }
+            if (w1 > g1) {
+                L.rstart[r] = g1;
+                L.rcnt[r] = w1 - g1;
+            }
+        }
+        r0 = 1 + (g0 > w0 ? 1 : 0) + (w1 > g1 ? 1 : 0);
     }
     const uint64_t lt = lanemask_lt();
     const bool asRange = node >= 0 && rng.y <= NN_LEAF;
@@ -322,7 +337,7 @@ __device__ __forceinline__ bool nn_direct(LdsT &L, const NNArgs &A, double lox, 
 template <bool FILTER, typename LdsT, typename KeepFn, typename TileFn>
 __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
                                             double hiy, double loz, double hiz, int start, int w0, int w1,
-                                            KeepFn &&keep, TileFn &&tile) {
+                                            int g0, KeepFn &&keep, TileFn &&tile) {
     const int lane = wave_lane();
     const int64_t n = A.n;
     int top = 1, nr = 0;
@@ -330,7 +345,7 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
         if (lane == 0) L.stack[0] = start;
     } else {
         top = nn_start_push(L, A, lox, hix, loy, hiy, loz, hiz, start == NN_START_ALL ? 0 : NN_START_PART0 - start,
-                            start == NN_START_ALL ? 1 : 8, w0, w1, nr);
+                            start == NN_START_ALL ? 1 : 8, w0, w1, g0, nr);
     }
     wave_sync();
     const uint64_t lt = lanemask_lt();
@@ -608,7 +623,11 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
         w1 = (int)(w0 + NN_CWIN < n ? w0 + NN_CWIN : n);
         if (w1 - w0 < NN_LEAF) w0 = w1 = 0;
     }
-    const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, w0, w1, keep, [&](int m) {
+    // with the window on, the group's own particles are the first candidates streamed, so all of
+    // them that pass the filter are in the first tile: only its pair tests must skip j == k
+    bool first_tile = true;
+    const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, w0, w1,
+                                      (int)(group * 64), keep, [&](int m) {
         if (!active) return;
         if (MODE == 1) {
             for (int e = 0; e < m; e++) {
@@ -625,7 +644,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
         // move loads).  The append is predicated (three selects shift the pending entries);
         // only the 16-byte store of every fourth entry is a branch.  The capacity check of
         // the stores is needed only when some lane could reach max_nb within this tile.
-        auto pairs = [&](auto check_cap) {
+        auto pairs = [&](auto check_cap, auto check_self) {
             for (int e = 0; e < m; e += NN_PB) {
                 int js[NN_PB];
                 double xs[NN_PB], ys[NN_PB], zs[NN_PB], h2s[NN_PB];
@@ -646,7 +665,8 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
                         const double dy = yi - ys[u];
                         const double dz = zi - zs[u];
                         const double d = dx * dx + dy * dy + dz * dz;
-                        const bool hit = (d < hi2) & (d < h2s[u]) & (j != (int)k);
+                        const bool hit =
+                            (d < hi2) & (d < h2s[u]) & (!decltype(check_self)::value || j != (int)k);
                         if (hit && (count & 3) == 3) {
                             if (!decltype(check_cap)::value || count < A.max_nb) *wp = make_int4(p0, p1, p2, j);
                             wp += A.stride;
@@ -659,8 +679,15 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
                 }
             }
         };
-        if (wave_ballot(count + m > A.max_nb) == 0) pairs(std::false_type{});
-        else pairs(std::true_type{});
+        const bool self = first_tile || w0 == w1;
+        first_tile = false;
+        if (wave_ballot(count + m > A.max_nb) == 0) {
+            if (self) pairs(std::false_type{}, std::true_type{});
+            else pairs(std::false_type{}, std::false_type{});
+        } else {
+            if (self) pairs(std::true_type{}, std::true_type{});
+            else pairs(std::true_type{}, std::false_type{});
+        }
     });
     if (!ok) {
         if (lane == 0) set_err(A.err, ERRF_STACK);
@@ -773,7 +800,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
         const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, 0, 1,
                                       [&](bool v, int j, const double4 &p, double h) { take(v, j, p.x, p.y, p.z, h * h); });
         if (!direct) {
-            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, 0, 0, KeepAll(), [&](int m) {
+            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, 0, 0, -1, KeepAll(), [&](int m) {
                 const bool v = lane < m;
                 take(v, v ? L.tj[lane] : -1, v ? L.tx[lane] : 0.0, v ? L.ty[lane] : 0.0, v ? L.tz[lane] : 0.0,
                      v ? L.th2[lane] : 0.0);
@@ -846,7 +873,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
         const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, 0, 1,
                                       [&](bool v, int i, const double4 &p, double h) { mark(v, i, p.x, p.y, p.z, h * h); });
         if (!direct) {
-            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, 0, 0, KeepAll(), [&](int m) {
+            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, 0, 0, -1, KeepAll(), [&](int m) {
                 if (lane < m) mark(true, L.tj[lane], L.tx[lane], L.ty[lane], L.tz[lane], L.th2[lane]);
             });
             if (!ok) {
@@ -967,7 +994,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
         const int start = NN_START_PART0 - o;   // eighth o of the start cells
         int count = 0;
         const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, start, 0, 0, KeepAll(), [&](int m) {
+                                    own.z + grow, start, 0, 0, -1, KeepAll(), [&](int m) {
             bool hit = false;
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
