@@ -934,7 +934,9 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     if (spec) launch_neighbours_fixup(c);
     else if (sph) launch_neighbours(c);
     mark(9);
-    if (sph) launch_density_eos(c);   // sound speed and pressure fused (ms_eos stays ~0)
+    // sound speed and pressure fused (ms_eos stays ~0); after the list fix-up it also publishes
+    // h and the neighbour counts
+    if (sph) launch_density_eos(c, spec);
     mark(10);
     mark(11);
     if (sph) launch_forces(c, feats);

@@ -697,7 +697,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
         A.nbr4[(int64_t)(count >> 2) * A.stride + k] = tail4();
     if (MODE == 0 && A.only) {
         // re-list of flagged particles: counts and caller-order outputs are published for
-        // all particles by k_publish_noi afterwards
+        // all particles by the density kernel afterwards
         if (active) {
             if ((count & 3) != 0 && count < A.max_nb)
                 A.nbr4[(int64_t)(count >> 2) * A.stride + k] = tail4();
@@ -882,53 +882,6 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
             }
         }
         wave_sync();
-    }
-}
-
-// caller-order counts and the neighbour statistics from the sorted counts: grid-stride,
-// one block-level sum / max and one pair of atomics per block (a pair per wave would
-// serialise tens of thousands of atomics on one address)
-#define PUBLISH_BLOCKS 512
-// counts (and with sh set, h and iteration counts) back to caller order; neighbour sum and
-// maximum into the counters
-__global__ __launch_bounds__(256) void k_publish_noi(int64_t n, const int32_t *__restrict__ perm,
-                                                     const int32_t *__restrict__ snoi, int32_t *__restrict__ noi,
-                                                     unsigned long long *__restrict__ counters,
-                                                     const double *__restrict__ sh,
-                                                     const int32_t *__restrict__ siters, double *__restrict__ sml,
-                                                     int32_t *__restrict__ iters) {
-    __shared__ unsigned long long ssum[4];
-    __shared__ int smax[4];
-    unsigned long long sum = 0;
-    int mx = 0;
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-        const int v = snoi[k];
-        const int32_t i = perm[k];
-        noi[i] = v;
-        if (sh) {
-            sml[i] = sh[k];
-            iters[i] = siters[k];
-        }
-        sum += (unsigned long long)v;
-        mx = v > mx ? v : mx;
-    }
-    sum = wave_sum(sum);
-    mx = wave_max(mx);
-    const int w = threadIdx.x / 64;
-    if (wave_lane() == 0) {
-        ssum[w] = sum;
-        smax[w] = mx;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        int m = 0;
-        for (int i = 0; i < 4; i++) {
-            t += ssum[i];
-            m = smax[i] > m ? smax[i] : m;
-        }
-        atomicAdd(&counters[CNT_NEIGHBOURS], t);
-        atomicMax(&counters[CNT_MAX_NOI], (unsigned long long)m);
     }
 }
 
@@ -1229,12 +1182,11 @@ void launch_neighbours_fixup(mhpc_ctx *c) {
     b.out_list = nullptr;
     b.list_in = c->olist;
     b.count_in = c->ocount + 1;
-    b.counters = nullptr;   // k_publish_noi counts every particle
+    b.counters = nullptr;   // the density kernel counts every particle
     b.noi_out = nullptr;
     hipLaunchKernelGGL(k_nn_one<0>, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, b);
-    hipLaunchKernelGGL(k_publish_noi, dim3((unsigned)std::min<int64_t>(ceil_div(c->n, 256), PUBLISH_BLOCKS)),
-                       dim3(256), 0, c->stream, c->n, c->perm, c->snoi, c->noi, c->counters, c->sh, c->siters,
-                       c->f[MHPC_F_SML], c->h_iters);
+    // h, counts and the neighbour sums go to caller order with the density kernel
+    // (launch_density_eos(c, true)), which runs next and visits every particle
 }
 
 void launch_neighbours(mhpc_ctx *c) {
@@ -1405,6 +1357,15 @@ __device__ __forceinline__ void density_pair(const double4 &pi, double hi, const
     }
 }
 
+// caller-order h, neighbour and iteration counts and the neighbour sum / maximum, published by
+// the density kernel when noi is set (the RHS with lists from pass 0, after the list fix-up)
+struct DensPub {
+    const int32_t *siters;
+    double *sml;
+    int32_t *noi, *iters;
+    unsigned long long *counters;
+};
+
 // Neighbour window.  Per-lane gathers of neighbour records are bound by the L1 address path
 // (each 16-byte load of a wave touches ~50 distinct cache lines).  A block covers a
 // contiguous range of sorted particles -- a compact region of the curve -- and stages that
@@ -1422,7 +1383,7 @@ __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *
                                                       const int32_t *__restrict__ nbr, int64_t stride,
                                                       int max_nb, int sml_avg, const int32_t *__restrict__ perm,
                                                       double *__restrict__ srho, double *__restrict__ rho,
-                                                      uint32_t *err, EosArgs E) {
+                                                      uint32_t *err, EosArgs E, DensPub P) {
     __shared__ double4 wp[DENS_WIN];
     __shared__ double wh[DENS_WIN];
     const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (int64_t)DENS_WIN;
@@ -1433,72 +1394,105 @@ __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *
         wh[tid] = sh[k];
     }
     __syncthreads();
-    if (k >= n) return;
-    const double4 pi = wp[tid];
-    const double hi = wh[tid];
-    int cnt = snoi[k];
-    cnt = cnt > max_nb ? max_nb : cnt;
-    const int4 *nrow = reinterpret_cast<const int4 *>(nbr) + k;   // row r at nrow[r * stride]
-    double acc = 0.0;
-    Kahan acc32;
-    // two list rows (eight neighbours) per iteration: the eight records are requested
-    // together, then the pairs are evaluated in list order; the next two rows are
-    // requested a step ahead.  The window copy is read unconditionally (clamped index),
-    // the gather only where j is outside the window.
-    int4 qn0 = make_int4(0, 0, 0, 0), qn1 = qn0;
-    if (cnt > 0) qn0 = nrow[0];
-    if (cnt > 4) qn1 = nrow[stride];
-    for (int r = 0; 4 * r < cnt; r += 2) {
-        const int4 qa = qn0, qb = qn1;
-        const int m = cnt - 4 * r;   // entries left (>= 1)
-        if (m > 8) qn0 = nrow[(int64_t)(r + 2) * stride];
-        if (m > 12) qn1 = nrow[(int64_t)(r + 3) * stride];
-        const int32_t jj[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
-        double4 pj[8];
-        double hj[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int32_t j = e < m ? jj[e] : (int32_t)k;
-            const uint64_t o = (uint64_t)((int64_t)j - base);
-            const bool in = o < (uint64_t)DENS_WIN;
-            const int oc = in ? (int)o : 0;
-            pj[e] = wp[oc];
-            hj[e] = wh[oc];
-            if (!in) {
-                pj[e] = sxm[j];
-                hj[e] = sh[j];
+    int pub_cnt = 0;   // the neighbour count this thread publishes (0 beyond n)
+    if (k < n) {
+        const double4 pi = wp[tid];
+        const double hi = wh[tid];
+        int cnt = snoi[k];
+        cnt = cnt > max_nb ? max_nb : cnt;
+        const int4 *nrow = reinterpret_cast<const int4 *>(nbr) + k;   // row r at nrow[r * stride]
+        double acc = 0.0;
+        Kahan acc32;
+        // two list rows (eight neighbours) per iteration: the eight records are requested
+        // together, then the pairs are evaluated in list order; the next two rows are
+        // requested a step ahead.  The window copy is read unconditionally (clamped index),
+        // the gather only where j is outside the window.
+        int4 qn0 = make_int4(0, 0, 0, 0), qn1 = qn0;
+        if (cnt > 0) qn0 = nrow[0];
+        if (cnt > 4) qn1 = nrow[stride];
+        for (int r = 0; 4 * r < cnt; r += 2) {
+            const int4 qa = qn0, qb = qn1;
+            const int m = cnt - 4 * r;   // entries left (>= 1)
+            if (m > 8) qn0 = nrow[(int64_t)(r + 2) * stride];
+            if (m > 12) qn1 = nrow[(int64_t)(r + 3) * stride];
+            const int32_t jj[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+            double4 pj[8];
+            double hj[8];
+    #pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int32_t j = e < m ? jj[e] : (int32_t)k;
+                const uint64_t o = (uint64_t)((int64_t)j - base);
+                const bool in = o < (uint64_t)DENS_WIN;
+                const int oc = in ? (int)o : 0;
+                pj[e] = wp[oc];
+                hj[e] = wh[oc];
+                if (!in) {
+                    pj[e] = sxm[j];
+                    hj[e] = sh[j];
+                }
             }
+    #pragma unroll
+            for (int e = 0; e < 8; e++)
+                if (e < m) density_pair<FP64>(pi, hi, pj[e], hj[e], sml_avg, acc, acc32);
         }
-#pragma unroll
-        for (int e = 0; e < 8; e++)
-            if (e < m) density_pair<FP64>(pi, hi, pj[e], hj[e], sml_avg, acc, acc32);
+        const double r = pi.w * (K8PI / (hi * hi * hi)) + (FP64 ? acc : acc32.value());
+        srho[k] = r;
+        const int32_t i = perm[k];
+        rho[i] = r;
+        if (!(r > 0.0)) set_err(err, ERRF_RHO);
+        if (EOS) eos_particle(E, k, i, r, pi.w, hi);
+        if (P.noi) {
+            pub_cnt = snoi[k];
+            P.noi[i] = pub_cnt;
+            P.sml[i] = hi;
+            P.iters[i] = P.siters[k];
+        }
     }
-    const double r = pi.w * (K8PI / (hi * hi * hi)) + (FP64 ? acc : acc32.value());
-    srho[k] = r;
-    const int32_t i = perm[k];
-    rho[i] = r;
-    if (!(r > 0.0)) set_err(err, ERRF_RHO);
-    if (EOS) eos_particle(E, k, i, r, pi.w, hi);
+    if (P.noi) {
+        // every thread of the block takes part: one sum / max and one pair of atomics per block
+        // (a pair per wavefront would queue tens of thousands of atomics on one address)
+        __shared__ unsigned long long psum[DENS_WIN / 64];
+        __shared__ int pmax[DENS_WIN / 64];
+        const unsigned long long sum = wave_sum((unsigned long long)pub_cnt);
+        const int mx = wave_max(pub_cnt);
+        if (wave_lane() == 0) {
+            psum[tid / 64] = sum;
+            pmax[tid / 64] = mx;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long t = 0;
+            int m = 0;
+            for (int w = 0; w < DENS_WIN / 64; w++) {
+                t += psum[w];
+                m = pmax[w] > m ? pmax[w] : m;
+            }
+            atomicAdd(&P.counters[CNT_NEIGHBOURS], t);
+            atomicMax(&P.counters[CNT_MAX_NOI], (unsigned long long)m);
+        }
+    }
 }
 
 template <bool EOS>
-static void launch_density_t(mhpc_ctx *c) {
+static void launch_density_t(mhpc_ctx *c, bool publish) {
     if (c->n <= 0) return;
     const unsigned g = (unsigned)ceil_div(c->n, DENS_WIN);
     const EosArgs E = eos_args(c);
+    DensPub P = {};
+    if (publish) P = DensPub{c->siters, c->f[MHPC_F_SML], c->noi, c->h_iters, c->counters};
     if (c->p.precision == MHPC_PREC_FP64)
         hipLaunchKernelGGL((k_density<true, EOS>), dim3(g), dim3(DENS_WIN), 0, c->stream, c->n, c->sxm, c->sh,
                            c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
-                           c->f[MHPC_F_RHO], c->err, E);
+                           c->f[MHPC_F_RHO], c->err, E, P);
     else
         hipLaunchKernelGGL((k_density<false, EOS>), dim3(g), dim3(DENS_WIN), 0, c->stream, c->n, c->sxm, c->sh,
                            c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
-                           c->f[MHPC_F_RHO], c->err, E);
+                           c->f[MHPC_F_RHO], c->err, E, P);
 }
 
-void launch_density(mhpc_ctx *c) { launch_density_t<false>(c); }
+void launch_density(mhpc_ctx *c) { launch_density_t<false>(c, false); }
 
-void launch_density_eos(mhpc_ctx *c) { launch_density_t<true>(c); }
+void launch_density_eos(mhpc_ctx *c, bool publish) { launch_density_t<true>(c, publish); }
 
 __global__ __launch_bounds__(256) void k_eos(int64_t n, EosArgs E, const double *__restrict__ srho,
                                              const double4 *__restrict__ sxm, const double *__restrict__ sh,

@@ -176,7 +176,7 @@ void launch_neighbours(mhpc_ctx *c);
 void launch_neighbours_fixup(mhpc_ctx *c);
 void launch_density(mhpc_ctx *c);
 void launch_eos(mhpc_ctx *c);
-void launch_density_eos(mhpc_ctx *c);   // density with the EOS fused (the RHS path)
+void launch_density_eos(mhpc_ctx *c, bool publish);   // density with the EOS fused (the RHS path)
 void launch_forces(mhpc_ctx *c, bool features);
 void launch_scatter_h(mhpc_ctx *c);
 void launch_leapfrog_x(mhpc_ctx *c, double dt);
