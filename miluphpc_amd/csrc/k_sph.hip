@@ -1154,12 +1154,11 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     }
 }
 
-// After a speculative h-iteration: re-list exactly the particles whose pass-0 list may
-// differ from the list with the final h (the changed particles and every particle within
-// its own h of one), then publish counts.  The lists then equal those of a full list walk
-// with the final h as sets (order may differ).
-// after launch_h_iterate with lists from pass 0 (its k_h_init zeroes the dirty flags and the
-// second outlier counter)
+// After a speculative h-iteration (launch_h_iterate with lists from pass 0; its k_h_init
+// zeroed the dirty flags and the second outlier counter): re-list exactly the particles whose
+// pass-0 list may differ from the list with the final h (the changed particles and every
+// particle within its own h of one).  The lists then equal those of a full list walk with the
+// final h as sets (order may differ); the density kernel publishes the counts.
 void launch_neighbours_fixup(mhpc_ctx *c) {
     if (c->n <= 0) return;
     NNArgs a = nn_common(c);
