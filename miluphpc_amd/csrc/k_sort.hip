@@ -16,10 +16,12 @@
 #define SORT_ITEMS (MHPC_SORT_TILE / SORT_BLOCK)   // 16
 #define SORT_WAVES (SORT_BLOCK / 64)
 
+// digit histograms of passes [p_lo, p_hi) over all keys (one read of the keys)
 __global__ __launch_bounds__(SORT_BLOCK) void k_sort_hist_all(const uint64_t *__restrict__ keys,
                                                               int64_t n,
                                                               uint32_t *__restrict__ hist,
-                                                              uint32_t *__restrict__ check_flag) {
+                                                              uint32_t *__restrict__ check_flag, int p_lo,
+                                                              int p_hi) {
     __shared__ uint32_t h[MHPC_SORT_PASSES * MHPC_SORT_BUCKETS];
     if (blockIdx.x == 0 && threadIdx.x == 0) *check_flag = 0u;
     for (int t = threadIdx.x; t < MHPC_SORT_PASSES * MHPC_SORT_BUCKETS; t += SORT_BLOCK) h[t] = 0;
@@ -30,7 +32,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_hist_all(const uint64_t *__
         const uint64_t k = keys[i];
 #pragma unroll
         for (int p = 0; p < MHPC_SORT_PASSES; p++)
-            atomicAdd(&h[p * MHPC_SORT_BUCKETS + (int)((k >> (MHPC_SORT_BITS * p)) & 255u)], 1u);
+            if (p >= p_lo && p < p_hi)
+                atomicAdd(&h[p * MHPC_SORT_BUCKETS + (int)((k >> (MHPC_SORT_BITS * p)) & 255u)], 1u);
     }
     __syncthreads();
     for (int t = threadIdx.x; t < MHPC_SORT_PASSES * MHPC_SORT_BUCKETS; t += SORT_BLOCK)
@@ -177,10 +180,10 @@ void launch_sort(mhpc_ctx *c) {
     const int64_t ntiles = ceil_div(n, MHPC_SORT_TILE);
     launch_fill_u32(c, c->digit_hist, 0u, MHPC_SORT_PASSES * MHPC_SORT_BUCKETS);
     int hb = (int)(ceil_div(n, SORT_BLOCK * 16) < 1024 ? ceil_div(n, SORT_BLOCK * 16) : 1024);
-    hipLaunchKernelGGL(k_sort_hist_all, dim3(hb), dim3(SORT_BLOCK), 0, c->stream, c->key, n,
-                       c->digit_hist, c->sort_check);
     const int levels = c->com_depth_hint + 2;
     const int first = levels < MHPC_LEVELS ? 3 * (MHPC_LEVELS - levels) / MHPC_SORT_BITS : 0;
+    hipLaunchKernelGGL(k_sort_hist_all, dim3(hb), dim3(SORT_BLOCK), 0, c->stream, c->key, n,
+                       c->digit_hist, c->sort_check, first, MHPC_SORT_PASSES);
     if (first > 0) {
         sort_passes(c, first, ntiles);
         hipLaunchKernelGGL(k_sort_check, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, c->skey, n,
@@ -189,6 +192,9 @@ void launch_sort(mhpc_ctx *c) {
                 hipSuccess &&
             hipStreamSynchronize(c->stream) == hipSuccess && *c->h_sort_check == 0u)
             return;
+        // the full sort also needs the low passes' histograms
+        hipLaunchKernelGGL(k_sort_hist_all, dim3(hb), dim3(SORT_BLOCK), 0, c->stream, c->key, n,
+                           c->digit_hist, c->sort_check, 0, first);
     }
     sort_passes(c, 0, ntiles);
 }
