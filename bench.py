@@ -237,7 +237,9 @@ def run_sedov(args, lib, ics, torch, result):
         "dtype": "f64",
         "config": {"workload": f"Sedov-Taylor {args.nside}^3 lattice SPH RHS, plain h-iteration "
                                f"(target 64, band 50-80), cubic spline, ideal gas, FP64",
-                   "n_particles": n, "path": "plain h-iteration, lists from pass 0 (speculative)"},
+                   "n_particles": n, "path": "plain h-iteration, lists from pass 0 (speculative)",
+                   "sort": "key levels of the previous RHS's tree + 1, checked, full sort when needed "
+                           "(the timed RHS follow the warm-up)"},
         "roofline": {"bound": "hbm", "kernel": ROOFLINE_KERNEL_NAME, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(ROOFLINE_KERNEL_NAME),
