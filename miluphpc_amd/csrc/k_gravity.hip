@@ -68,6 +68,14 @@ struct BHAcc<true> {
     Plain64 x, y, z, phi;
 };
 
+// 1/sqrt(x) for x >= 0, 0 for x = 0, from the hardware reciprocal square root: for normal x
+// the same value rsqrtf gives, without the compiler's scaling for denormal inputs (four
+// instructions per interaction).  x = r^2 + eps^2 is normal unless eps = 0 and two particles
+// are closer than ~1e-19; such x are raised to the smallest normal float.
+__device__ __forceinline__ float rsq_pos(float x) {
+    return x > 0.f ? __builtin_amdgcn_rsqf(fmaxf(x, 1.17549435e-38f)) : 0.f;
+}
+
 // FP32 batches: plain FP32 sums over the (at most 64) entries of one batch, then one
 // compensated (Kahan) addition of each batch sum into the running total
 template <bool POT>
@@ -78,7 +86,7 @@ __device__ __forceinline__ void eval_particles32(const BHWaveLds32 &L, int cnt, 
         const float4 q = L.p[e];
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-        const float rinv = r2 > 0.f ? rsqrtf(r2) : 0.f;
+        const float rinv = rsq_pos(r2);
         const float mr = q.w * rinv;
         const float mr3 = mr * rinv * rinv;
         bx = __builtin_fmaf(mr3, dx, bx);
@@ -100,7 +108,7 @@ __device__ __forceinline__ void eval_cells32(const BHWaveLds32 &L, int cnt, floa
         const float4 q = L.c[e];
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-        const float rinv = r2 > 0.f ? rsqrtf(r2) : 0.f;
+        const float rinv = rsq_pos(r2);
         const float rinv2 = rinv * rinv;
         const float mr = q.w * rinv;
         const float mr3 = mr * rinv2;
@@ -147,7 +155,7 @@ __device__ __forceinline__ void eval_particles(const BHWaveLds64 &L, int cnt, do
             const float dx = (float)(L.px[e] - px), dy = (float)(L.py[e] - py),
                         dz = (float)(L.pz[e] - pz);
             const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, (float)eps2)));
-            const float rinv = r2 > 0.f ? rsqrtf(r2) : 0.f;
+            const float rinv = rsq_pos(r2);
             const float mr = (float)L.pm[e] * rinv;
             const float mr3 = mr * rinv * rinv;
             acc.x.add(mr3 * dx);
@@ -195,7 +203,7 @@ __device__ __forceinline__ void eval_cells(const BHWaveLds64 &L, int cnt, double
             const float dx = (float)(L.cx[e] - px), dy = (float)(L.cy[e] - py),
                         dz = (float)(L.cz[e] - pz);
             const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, (float)eps2)));
-            const float rinv = r2 > 0.f ? rsqrtf(r2) : 0.f;
+            const float rinv = rsq_pos(r2);
             const float rinv2 = rinv * rinv;
             const float m = (float)L.cm[e];
             const float mr = m * rinv;
