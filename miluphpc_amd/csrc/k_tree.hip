@@ -166,28 +166,31 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_link(
     __shared__ int32_t cb[MHPC_LEVELS + 3];
     if (threadIdx.x < MHPC_LEVELS + 3) cb[threadIdx.x] = cell_base[threadIdx.x];
     __syncthreads();
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t ncells = cb[MHPC_LEVELS + 1];
     if (ncells > cap_cells) return;   // capacity error already flagged; cells are incomplete
+    // particles, then cells 1 .. ncells-1 (grid-stride: the cell count is known here only)
+    const int64_t items = n + ncells - 1;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < items;
+         t += (int64_t)gridDim.x * blockDim.x) {
     if (t < n) {
         const int32_t k = (int32_t)t;
         const int a = (k > 0) ? (int)lcp[k - 1] : 0;
         const int b = (int)lcp[k];
         const int Lp = max(a, b) + 1;
         plevel[perm[k]] = Lp;
-        if (Lp > MHPC_LEVELS) return;   // duplicate key, flagged by k_tree_count
+        if (Lp > MHPC_LEVELS) continue;   // duplicate key, flagged by k_tree_count
         const int32_t parent = ppar[k];
-        if (parent >= cap_cells) return;
+        if (parent >= cap_cells) continue;
         const int digit = (int)((slkey[k] >> (3 * (MHPC_LEVELS - Lp))) & 7u);
         cchild[8 * (int64_t)parent + digit] = k;
     } else {
         const int64_t c = t - n + 1;   // cells 1 .. ncells-1
-        if (c >= ncells) return;
         const int L = clevel[c];
         const int32_t s = cstart[c];
         const int32_t parent = cpar[c];
         const int digit = (int)((slkey[s] >> (3 * (MHPC_LEVELS - L))) & 7u);
         cchild[8 * (int64_t)parent + digit] = (int32_t)(n + c);
+    }
     }
 }
 
@@ -204,8 +207,9 @@ void launch_tree(mhpc_ctx *c) {
     hipLaunchKernelGGL(k_tree_cells, dim3((unsigned)ntiles), dim3(TREE_BLOCK), 0, c->stream, c->lcp, n,
                        c->toff, ntiles, c->level_count, c->cell_base, c->cstart, c->cend, c->clevel,
                        c->cchild, c->ppar, c->cpar, c->cap_cells, c->err);
-    const int64_t items = n + c->cap_cells;
-    hipLaunchKernelGGL(k_tree_link, dim3((unsigned)ceil_div(items, TREE_BLOCK)), dim3(TREE_BLOCK), 0,
+    const int64_t items = n + c->cap_cells;   // upper bound; the kernel loops over the actual count
+    const int64_t link_blocks = std::min<int64_t>(ceil_div(items, TREE_BLOCK), 8192);
+    hipLaunchKernelGGL(k_tree_link, dim3((unsigned)link_blocks), dim3(TREE_BLOCK), 0,
                        c->stream, c->lcp, slkey, c->perm, n, c->cell_base, c->cstart, c->clevel, c->ppar,
                        c->cpar, c->cchild, c->level, c->cap_cells);
 }
