@@ -1011,27 +1011,33 @@ __global__ __launch_bounds__(256) void k_nn_start_table(int64_t n, int kmax, con
                                                         const int32_t *__restrict__ cend,
                                                         const int32_t *__restrict__ clevel, int64_t cap_cells,
                                                         int32_t *__restrict__ tab, int2 *__restrict__ rng) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t < n) {
-        const int a = t > 0 ? (int)lcp[t - 1] : 0;
-        const int b = (int)lcp[t];
-        const int lp = (a > b ? a : b) + 1;
-        const uint64_t key = slkey[t];
-        for (int l = lp; l <= kmax; l++) {
-            const int64_t e = nn_tab_off(l) + (int64_t)(key >> (3 * (MHPC_LEVELS - l)));
-            tab[e] = (int32_t)t;
-            rng[e] = make_int2((int)t, 1);
+    // particles, then the cells of levels 1..kmax (cells are numbered level by level, so these
+    // are cells 1 .. cell_base[kmax + 1] - 1); grid-stride over the actual counts
+    const int64_t ncells = cell_base[MHPC_LEVELS + 1];
+    if (ncells > cap_cells) return;   // capacity error already flagged; cells are incomplete
+    const int64_t ctop = cell_base[kmax + 1] < ncells ? cell_base[kmax + 1] : ncells;
+    const int64_t items = n + (ctop > 1 ? ctop - 1 : 0);
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < items;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        if (t < n) {
+            const int a = t > 0 ? (int)lcp[t - 1] : 0;
+            const int b = (int)lcp[t];
+            const int lp = (a > b ? a : b) + 1;
+            const uint64_t key = slkey[t];
+            for (int l = lp; l <= kmax; l++) {
+                const int64_t e = nn_tab_off(l) + (int64_t)(key >> (3 * (MHPC_LEVELS - l)));
+                tab[e] = (int32_t)t;
+                rng[e] = make_int2((int)t, 1);
+            }
+        } else {
+            const int64_t c = t - n + 1;   // cells 1 .. ctop-1 (cell 0 is the root)
+            const int L = clevel[c];
+            if (L < 1 || L > kmax) continue;
+            const int32_t s0 = cstart[c];
+            const int64_t e = nn_tab_off(L) + (int64_t)(slkey[s0] >> (3 * (MHPC_LEVELS - L)));
+            tab[e] = (int32_t)(n + c);
+            rng[e] = make_int2((int)s0, (int)(cend[c] - s0 + 1));
         }
-    } else {
-        const int64_t ncells = cell_base[MHPC_LEVELS + 1];
-        const int64_t c = t - n + 1;   // cells 1 .. ncells-1 (cell 0 is the root)
-        if (ncells > cap_cells || c >= ncells) return;
-        const int L = clevel[c];
-        if (L < 1 || L > kmax) return;
-        const int32_t s0 = cstart[c];
-        const int64_t e = nn_tab_off(L) + (int64_t)(slkey[s0] >> (3 * (MHPC_LEVELS - L)));
-        tab[e] = (int32_t)(n + c);
-        rng[e] = make_int2((int)s0, (int)(cend[c] - s0 + 1));
     }
 }
 
@@ -1040,7 +1046,8 @@ static void ensure_nn_start(mhpc_ctx *c) {
     launch_fill_u32(c, reinterpret_cast<uint32_t *>(c->nn_start), 0xffffffffu, nn_tab_off(c->nn_kmax + 1));
     const uint64_t *slkey = (c->p.curve == MHPC_CURVE_HILBERT) ? c->slkey : c->skey;
     const int64_t items = c->n + c->cap_cells;
-    hipLaunchKernelGGL(k_nn_start_table, dim3((unsigned)ceil_div(items, 256)), dim3(256), 0, c->stream, c->n,
+    hipLaunchKernelGGL(k_nn_start_table, dim3((unsigned)std::min<int64_t>(ceil_div(items, 256), 8192)), dim3(256),
+                       0, c->stream, c->n,
                        c->nn_kmax, c->lcp, slkey, c->cell_base, c->cstart, c->cend, c->clevel, c->cap_cells,
                        c->nn_start, c->nn_start_rng);
     c->nn_start_valid = true;
