@@ -366,23 +366,24 @@ int orc_bh_v1(int64_t n, int64_t ncells, const int32_t *child, const double *nm,
  * would accept (d_box <= d_i), so the group walk is never less accurate per particle.
  * Monopole f = m r^-3 with r^2 = d^2 + eps^2; quadrupole adds -Q.d r^-5 +
  * 5/2 (d.Q.d) d r^-7.  phi_i = -sum m/r over all interactions except the particle itself. */
-int orc_bh_group(int64_t n, const int32_t *perm, int group, int64_t ncells,
-                 const int32_t *child, const int32_t *level, const double *nm,
-                 const double *nx, const double *ny, const double *nz, const double *q,
-                 const double box[6], double theta, double smoothing, int mac,
-                 double *ax, double *ay, double *az, double *phi,
-                 int64_t *n_cell_int, int64_t *n_part_int) {
-    if (group <= 0) return ORC_E_ARG;
+/* the walk over the groups gsel[0..ngsel) (all groups when gsel is NULL) */
+static int bh_group_walk(int64_t n, const int32_t *perm, int group, const int32_t *child,
+                         const int32_t *level, const double *nm, const double *nx, const double *ny,
+                         const double *nz, const double *q, const double box[6], double theta,
+                         double smoothing, int mac, double *ax, double *ay, double *az, double *phi,
+                         int64_t *n_cell_int, int64_t *n_part_int, const int64_t *gsel, int64_t ngsel) {
+    if (group <= 0 || mac < 0 || mac > 3) return ORC_E_ARG;
+    const int form = mac & 1, vote = (mac & 2) != 0;
     const double R = 0.5 * (box[1] - box[0]);
     const double s2_0 = 4.0 * R * R;
     const double dp_1 = R * R / theta;
     const double theta2 = theta * theta;
-    int64_t ngroups = (n + group - 1) / group;
+    int64_t ngroups = gsel ? ngsel : (n + group - 1) / group;
     int64_t cell_int = 0, part_int = 0;
     int err = ORC_OK;
-    (void)ncells;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : cell_int, part_int)
-    for (int64_t g = 0; g < ngroups; g++) {
+    for (int64_t gi = 0; gi < ngroups; gi++) {
+        const int64_t g = gsel ? gsel[gi] : gi;
         int64_t k0 = g * group, k1 = k0 + group;
         if (k1 > n) k1 = n;
         int cnt = (int)(k1 - k0);
@@ -407,21 +408,28 @@ int orc_bh_group(int64_t n, const int32_t *perm, int group, int64_t ncells,
                 accept = 1;
                 part_int += cnt;
             } else {
-                double cc[3] = {nx[c], ny[c], nz[c]};
-                double dd[3];
-                for (int a = 0; a < 3; a++) {
-                    if (cc[a] < lo[a]) dd[a] = lo[a] - cc[a];
-                    else if (cc[a] > hi[a]) dd[a] = cc[a] - hi[a];
-                    else dd[a] = 0.0;
-                }
-                double d2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
                 int L = level[c];
-                if (mac == 0) {
-                    double s2 = s2_0 * ldexp(1.0, -2 * L);
-                    accept = (d2 + smoothing) * theta2 > s2;
+                const double s2 = s2_0 * ldexp(1.0, -2 * L);
+                const double dp = dp_1 * ldexp(1.0, -2 * (L - 1));
+                if (!vote) {
+                    double cc[3] = {nx[c], ny[c], nz[c]};
+                    double dd[3];
+                    for (int a = 0; a < 3; a++) {
+                        if (cc[a] < lo[a]) dd[a] = lo[a] - cc[a];
+                        else if (cc[a] > hi[a]) dd[a] = cc[a] - hi[a];
+                        else dd[a] = 0.0;
+                    }
+                    double d2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+                    accept = form == 0 ? (d2 + smoothing) * theta2 > s2 : dp <= d2 + smoothing;
                 } else {
-                    double dp = dp_1 * ldexp(1.0, -2 * (L - 1));
-                    accept = dp <= d2 + smoothing;
+                    /* every member tests the cell on its own distance; all must accept */
+                    accept = 1;
+                    for (int k = 0; k < cnt && accept; k++) {
+                        int32_t i = perm[k0 + k];
+                        double dx = nx[c] - nx[i], dy = ny[c] - ny[i], dz = nz[c] - nz[i];
+                        double d2 = dx * dx + dy * dy + dz * dz;
+                        accept = form == 0 ? (d2 + smoothing) * theta2 > s2 : dp <= d2 + smoothing;
+                    }
                 }
                 if (accept) cell_int += cnt;
             }
@@ -484,6 +492,29 @@ int orc_bh_group(int64_t n, const int32_t *perm, int group, int64_t ncells,
     if (n_cell_int) *n_cell_int = cell_int;
     if (n_part_int) *n_part_int = part_int;
     return err;
+}
+
+int orc_bh_group(int64_t n, const int32_t *perm, int group, int64_t ncells,
+                 const int32_t *child, const int32_t *level, const double *nm,
+                 const double *nx, const double *ny, const double *nz, const double *q,
+                 const double box[6], double theta, double smoothing, int mac,
+                 double *ax, double *ay, double *az, double *phi,
+                 int64_t *n_cell_int, int64_t *n_part_int) {
+    (void)ncells;
+    return bh_group_walk(n, perm, group, child, level, nm, nx, ny, nz, q, box, theta, smoothing, mac,
+                         ax, ay, az, phi, n_cell_int, n_part_int, NULL, 0);
+}
+
+int orc_bh_group_subset(int64_t n, const int32_t *perm, int group, const int32_t *child,
+                        const int32_t *level, const double *nm, const double *nx, const double *ny,
+                        const double *nz, const double *q, const double box[6], double theta,
+                        double smoothing, int mac, int64_t ngsel, const int64_t *gsel, double *ax,
+                        double *ay, double *az, double *phi, int64_t *n_cell_int, int64_t *n_part_int) {
+    const int64_t ngroups = (n + group - 1) / (group > 0 ? group : 1);
+    for (int64_t t = 0; t < ngsel; t++)
+        if (gsel[t] < 0 || gsel[t] >= ngroups) return ORC_E_ARG;
+    return bh_group_walk(n, perm, group, child, level, nm, nx, ny, nz, q, box, theta, smoothing, mac,
+                         ax, ay, az, phi, n_cell_int, n_part_int, gsel, ngsel);
 }
 
 /* FP64 direct summation, the BH accuracy reference (softened with eps^2 = smoothing). */

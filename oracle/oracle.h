@@ -77,15 +77,25 @@ int orc_bh_v1(int64_t n, int64_t ncells, const int32_t *child, const double *nm,
               const double *nx, const double *ny, const double *nz, double radius,
               double theta, double smoothing, double *ax, double *ay, double *az, double *u);
 /* group MAC over `group` consecutive particles of the key order: the semantics the
- * wavefront-cooperative HIP walk implements. mac: 0 = s^2 < theta^2 (d^2+eps^2) (v1
- * form), 1 = s^2/theta <= d^2+eps^2 (v2 form, gravity.cu:798-841), d = distance from
- * the cell's COM to the group's bounding box.  q may be NULL (monopole only). */
+ * wavefront-cooperative HIP walk implements.  mac bit 0: 0 = s^2 < theta^2 (d^2+eps^2)
+ * (v1 form, gravity.cu:319-330), 1 = s^2/theta <= d^2+eps^2 (v2 form, gravity.cu:798-841).
+ * mac bit 1: 0 = d is the distance from the cell's COM to the group's bounding box (one
+ * test per group); 1 = vote: every member tests the cell with its own distance and the
+ * group accepts only if all members do (the v2 warp vote, gravity.cu:836, over the
+ * group).  q may be NULL (monopole only). */
 int orc_bh_group(int64_t n, const int32_t *perm, int group, int64_t ncells,
                  const int32_t *child, const int32_t *level, const double *nm,
                  const double *nx, const double *ny, const double *nz, const double *q,
                  const double box[6], double theta, double smoothing, int mac,
                  double *ax, double *ay, double *az, double *phi,
                  int64_t *n_cell_int, int64_t *n_part_int);
+/* the group walk over the listed groups gsel[0..ngsel) only (group g = sorted positions
+ * [g*group, (g+1)*group)); the other particles' outputs are left untouched */
+int orc_bh_group_subset(int64_t n, const int32_t *perm, int group, const int32_t *child,
+                        const int32_t *level, const double *nm, const double *nx, const double *ny,
+                        const double *nz, const double *q, const double box[6], double theta,
+                        double smoothing, int mac, int64_t ngsel, const int64_t *gsel, double *ax,
+                        double *ay, double *az, double *phi, int64_t *n_cell_int, int64_t *n_part_int);
 /* direct summation on a subset of targets (FP64, softened) */
 void orc_direct(int64_t n, const double *x, const double *y, const double *z,
                 const double *m, double smoothing, int64_t ntarget, const int32_t *targets,
@@ -162,6 +172,23 @@ double orc_mlp_backward(int64_t nb, int nin, int nh, const double *X, const doub
                         const double *b2, const double *W3, double b3,
                         double *gW1, double *gb1, double *gW2, double *gb2, double *gW3,
                         double *gb3);
+
+/* ---- host-core baseline: the composite passes above timed at C level (no Python in the
+ * timed region), on the caller's arrays ---- */
+/* One full SPH RHS per rep: h-iteration from h0 (h bounded by sml_max when > 0), neighbour
+ * lists, density, sound speed and pressure (cs before p), internal forces, FP64.
+ * seconds[r] = wall time of rep r. */
+int orc_bench_sph_rhs(int64_t n, const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz, const double *m,
+                      const double *h0, const double *e, const double *p0, const double *cs0,
+                      int n_target, int band_lo, int band_hi, int max_counts, int max_nb, double sml_max,
+                      int eos, double gamma, double K, double alpha, double beta, int reps, double *seconds);
+/* Barnes-Hut RHS: bounding box, keys, sort, octree, centres of mass (+ quadrupole) over all
+ * particles (seconds[0]), then the group walk over every `sample_every`-th group (seconds[1],
+ * groups walked in *walked, interactions in counts[2] = {cell, particle}). */
+int orc_bench_gravity(int64_t n, const double *x, const double *y, const double *z, const double *m,
+                      double theta, double smoothing, int curve, int mac, int quad, int group,
+                      int64_t sample_every, double seconds[2], int64_t *walked, int64_t counts[2]);
 
 #ifdef __cplusplus
 }

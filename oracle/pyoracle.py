@@ -41,7 +41,12 @@ def _load() -> C.CDLL:
         "orc_bh_v1": (i32, [i64, i64, vp, vp, vp, vp, vp, d, d, d, vp, vp, vp, vp]),
         "orc_bh_group": (i32, [i64, vp, i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, d, d, i32,
                                vp, vp, vp, vp, vp, vp]),
+        "orc_bh_group_subset": (i32, [i64, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, d, d, i32, i64, vp,
+                                      vp, vp, vp, vp, vp, vp]),
         "orc_direct": (None, [i64, vp, vp, vp, vp, d, i64, vp, vp, vp, vp, vp]),
+        "orc_bench_sph_rhs": (i32, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32,
+                                    d, i32, d, d, d, d, i32, vp]),
+        "orc_bench_gravity": (i32, [i64, vp, vp, vp, vp, d, d, i32, i32, i32, i32, i64, vp, vp, vp]),
         "orc_nn_brute": (i32, [i64, vp, vp, vp, vp, i32, vp, vp]),
         "orc_nn_grid": (i32, [i64, vp, vp, vp, vp, i32, vp, vp]),
         "orc_nn_refwalk": (i32, [i64, vp, vp, vp, vp, vp, vp, vp, vp, d, i32, vp, vp]),
@@ -152,8 +157,13 @@ class Tree:
         return self.q
 
 
-def bh_group(tree: Tree, perm, box, theta, smoothing, mac=0, group=64, quad=False, potential=False):
+def bh_group(tree: Tree, perm, box, theta, smoothing, mac=0, group=64, quad=False, potential=False,
+             vote=True):
+    """Group walk (orc_bh_group).  vote=True: every member tests a cell with its own distance
+    and the group accepts only if all do; vote=False: one test with the distance to the
+    group's bounding box."""
     n = tree.n
+    mac = int(mac) | (2 if vote else 0)
     perm = np.ascontiguousarray(perm, np.int32)
     box = f64(box)
     ax, ay, az = np.empty(n), np.empty(n), np.empty(n)
@@ -384,7 +394,7 @@ def sph_rhs(s: dict, n_target=64, band=(50, 80), max_counts=7, max_nb=180, eos=9
     return out
 
 
-def gravity_rhs(s: dict, theta, smoothing, curve=1, mac=0, group=64, quad=False, potential=False):
+def gravity_rhs(s: dict, theta, smoothing, curve=1, mac=0, group=64, quad=False, potential=False, vote=True):
     """bbox -> keys -> sort -> tree -> COM (+quadrupole) -> group Barnes-Hut, FP64."""
     box = bbox(s["x"], s["y"], s["z"])
     k = keys(s["x"], s["y"], s["z"], box, curve)
@@ -393,6 +403,40 @@ def gravity_rhs(s: dict, theta, smoothing, curve=1, mac=0, group=64, quad=False,
     t.com(s["mass"])
     if quad:
         t.quadrupole()
-    res = bh_group(t, perm, box, theta, smoothing, mac=mac, group=group, quad=quad, potential=potential)
+    res = bh_group(t, perm, box, theta, smoothing, mac=mac, group=group, quad=quad, potential=potential,
+                   vote=vote)
     res.update({"box": box, "keys": k, "perm": perm, "tree": t})
     return res
+
+
+# ---- host-core baseline (timed at C level) ----------------------------------------------
+def bench_sph_rhs(s: dict, reps: int, n_target=64, band=(50, 80), max_counts=7, max_nb=180, eos=9,
+                  gamma=5.0 / 3.0, K=0.0, alpha=1.0, beta=2.0, sml_max=0.0) -> np.ndarray:
+    """Wall seconds of `reps` full SPH RHS (orc_bench_sph_rhs) on the state s."""
+    a = {k: f64(s[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "mass", "sml", "e", "p", "cs")}
+    secs = np.zeros(int(reps))
+    st = LIB.orc_bench_sph_rhs(len(a["x"]), *[P(a[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "mass",
+                                                                 "sml", "e", "p", "cs")],
+                               int(n_target), int(band[0]), int(band[1]), int(max_counts), int(max_nb),
+                               float(sml_max), int(eos), float(gamma), float(K), float(alpha), float(beta),
+                               int(reps), P(secs))
+    if st != OK:
+        raise RuntimeError(f"orc_bench_sph_rhs: status {st}")
+    return secs
+
+
+def bench_gravity(s: dict, theta, smoothing, curve=1, mac=0, quad=False, group=64, sample_every=1,
+                  vote=True) -> dict:
+    """Barnes-Hut RHS timed at C level: tree phases over all particles, walk over every
+    sample_every-th group (orc_bench_gravity)."""
+    a = {k: f64(s[k]) for k in ("x", "y", "z", "mass")}
+    secs = np.zeros(2)
+    walked = C.c_int64(0)
+    counts = np.zeros(2, np.int64)
+    st = LIB.orc_bench_gravity(len(a["x"]), P(a["x"]), P(a["y"]), P(a["z"]), P(a["mass"]), float(theta),
+                               float(smoothing), int(curve), int(mac) | (2 if vote else 0), int(bool(quad)),
+                               int(group), int(sample_every), P(secs), C.byref(walked), P(counts))
+    if st != OK:
+        raise RuntimeError(f"orc_bench_gravity: status {st}")
+    return {"tree_s": float(secs[0]), "walk_s": float(secs[1]), "groups_walked": int(walked.value),
+            "n_cell": int(counts[0]), "n_part": int(counts[1])}

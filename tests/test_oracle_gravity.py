@@ -109,3 +109,70 @@ def test_sqrt_theta_mac_is_coarser_than_classic_at_same_theta(orc):
     c = orc.gravity_rhs(s, theta=0.25, smoothing=1e-6, mac=1)
     assert b["n_part"] + b["n_cell"] < a["n_part"] + a["n_cell"]
     assert (c["n_part"], c["n_cell"]) == (a["n_part"], a["n_cell"])
+
+
+def _vote_cut_counts(t, perm, box, group, theta, eps2):
+    """Independent restatement of the vote form's interaction counts: a recursive descent
+    from the root in Python over the oracle's tree arrays; a cell is taken for a group iff
+    every member's own classic test (d_i^2 + eps^2) theta^2 > s^2 passes."""
+    n = t.n
+    R = 0.5 * (box[1] - box[0])
+    child = t.child.reshape(-1, 8)
+    cells = parts = 0
+    for g0 in range(0, n, group):
+        mem = perm[g0:g0 + group]
+        px, py, pz = t.nx[mem], t.ny[mem], t.nz[mem]
+        stack = [n]
+        while stack:
+            c = stack.pop()
+            if c < n:
+                parts += len(mem)
+                continue
+            d2 = (t.nx[c] - px) ** 2 + (t.ny[c] - py) ** 2 + (t.nz[c] - pz) ** 2
+            s2 = 4.0 * R * R * 2.0 ** (-2 * int(t.level[c]))
+            if c != n and np.all((d2 + eps2) * theta ** 2 > s2):
+                cells += len(mem)
+            else:
+                stack.extend(int(ch) for ch in child[c] if ch != -1)
+    return cells, parts
+
+
+def test_vote_form_counts_match_an_independent_descent(orc):
+    s = ics.plummer(3000, seed=21)
+    eps2, theta = 1e-6, 0.5
+    box = orc.bbox(s["x"], s["y"], s["z"])
+    t = orc.Tree(s["x"], s["y"], s["z"], box)
+    t.com(s["mass"])
+    perm = orc.sort(orc.keys(s["x"], s["y"], s["z"], box, 1))
+    for group in (64, 7):
+        g = orc.bh_group(t, perm, box, theta, eps2, mac=0, group=group, vote=True)
+        assert (g["n_cell"], g["n_part"]) == _vote_cut_counts(t, perm, box, group, theta, eps2)
+
+
+def test_vote_form_sits_between_per_particle_and_box_forms(orc):
+    """Per member, the vote form's cut refines the member's own (G = 1) cut and is refined by
+    the box form's: interaction counts are ordered G=1 <= vote <= box, the vote form with
+    groups of one is the v1 walk, and theta -> 0 is direct summation."""
+    s = ics.plummer(6000, seed=22)
+    eps2 = 1e-6
+    box = orc.bbox(s["x"], s["y"], s["z"])
+    t = orc.Tree(s["x"], s["y"], s["z"], box)
+    t.com(s["mass"])
+    perm = orc.sort(orc.keys(s["x"], s["y"], s["z"], box, 1))
+    for mac in (0, 1):
+        one = orc.bh_group(t, perm, box, 0.5, eps2, mac=mac, group=1, vote=False)
+        one_v = orc.bh_group(t, perm, box, 0.5, eps2, mac=mac, group=1, vote=True)
+        vote = orc.bh_group(t, perm, box, 0.5, eps2, mac=mac, group=64, vote=True)
+        boxf = orc.bh_group(t, perm, box, 0.5, eps2, mac=mac, group=64, vote=False)
+        assert (one_v["n_cell"], one_v["n_part"]) == (one["n_cell"], one["n_part"])
+        assert np.array_equal(one_v["ax"], one["ax"]) and np.array_equal(one_v["az"], one["az"])
+        tot = lambda r: r["n_cell"] + r["n_part"]
+        assert tot(one) < tot(vote) < tot(boxf)
+        assert vote["n_part"] <= boxf["n_part"]
+    d = orc.direct(s["x"], s["y"], s["z"], s["mass"], eps2)
+    z = orc.gravity_rhs(s, theta=1e-9, smoothing=eps2, group=64, vote=True)
+    assert z["n_cell"] == 0
+    assert np.max(vec_rel_err(z["ax"], z["ay"], z["az"], d["ax"], d["ay"], d["az"])) < 1e-11
+    e1 = vec_rel_err(one["ax"], one["ay"], one["az"], d["ax"], d["ay"], d["az"])
+    ev = vec_rel_err(vote["ax"], vote["ay"], vote["az"], d["ax"], d["ay"], d["az"])
+    assert np.percentile(ev, 99) <= np.percentile(e1, 99) * 1.0001
