@@ -2,12 +2,15 @@
 
 Workload (the metric's configuration on one GPU): Sedov-Taylor 128^3 = 2,097,152
 particles, SPH only, plain h-iteration (band 50-80, target 64), cubic spline, ideal gas,
-FP64 -- one step = one full Miluphpc::rhs() (bbox, keys, sort, tree, h-iteration,
-neighbours, density, EOS, forces) with inputs resident in HBM.  The Plummer
-16,777,216-particle Barnes-Hut RHS (theta 0.6, quadrupole, FP32 + Kahan) is timed beside
-it and reported under "plummer".
+FP64, advanced by predictor-corrector Euler steps under a CFL 0.3 time step (two
+Miluphpc::rhs() calls per step: bbox, keys, sort, tree, h-iteration, neighbours, density,
+EOS, forces) with inputs resident in HBM.  One timed step is one such PC-Euler step of the
+evolving run; value = RHS particles/s = 2 N / (median step wall time).  The Plummer
+16,777,216-particle Barnes-Hut RHS (theta 0.6, quadrupole, FP32 + Kahan) is timed beside it
+and reported under "plummer", with its energy drift over five leapfrog steps and the 99th
+percentile force error against FP64 direct summation on 4,096 particles.
 
-Prints ONE JSON line.  value = RHS particles/s of the Sedov workload.
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -30,27 +33,22 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def sph_bytes_per_particle(nnb: float) -> dict:
-    """Algorithmic (compulsory) HBM bytes per particle of each SPH kernel: each particle's
-    own fields once, the neighbour list (4 B per entry + 4 B count) once, neighbour data
-    assumed reused through LDS / L2."""
+# kernels the SPH roofline can be quoted on: stats field -> (kernel name as rocprofv3 lists it,
+# algorithmic HBM bytes per particle at nnb neighbours).  Each particle's own fields count once,
+# the neighbour list (4 B per entry + 4 B count) once; neighbour data is assumed reused through
+# LDS / L2.
+def sph_kernel_table(nnb: float, precision_fp64: bool = True) -> dict:
     return {
-        "ms_nn": 32 + 4 + 4 * nnb,           # read x,y,z,h; write list + count
-        "ms_density": 40 + 4 + 4 * nnb + 8,  # read x,y,z,h,m + list; write rho (the EOS is fused in)
-        "ms_forces": 88 + 4 + 4 * nnb + 48,  # read 11 fields + list; write 6 fields
+        "ms_nn_walk": ("k_nn_walk<2>", 32 + 4 + 4 * nnb),            # read x,y,z,h; write list + count
+        "ms_density": ("k_density<true, true>", 40 + 4 + 4 * nnb + 8),   # read x,y,z,h,m + list; write rho
+        "ms_forces": ("k_forces<true, false>", 88 + 4 + 4 * nnb + 48),   # read 11 fields + list; write 6
     }
 
 
-# the kernel the roofline line is quoted for: the largest total time per RHS by kernel
-# name in the rocprofv3 kernel statistics of this workload (one launch per RHS)
-ROOFLINE_KERNEL = "ms_forces"
-ROOFLINE_KERNEL_NAME = "k_forces<true, false>"
-
-
-def measured_traffic(kernel_name: str):
+def measured_traffic(kernel_name: str, workload: str = "sedov128"):
     """HBM bytes per launch of `kernel_name` from the committed PMC summary (FETCH_SIZE +
     WRITE_SIZE, calibrated against a plain copy), or None if there is none."""
-    path = REPO / "profiles" / "traffic_sedov128.txt"
+    path = REPO / "profiles" / f"traffic_{workload}.txt"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -63,28 +61,10 @@ def measured_traffic(kernel_name: str):
     return None
 
 
-def time_rhs(ctx, lib, flags, steps, warmup, label, restore=None):
-    """restore: {field: device tensor} copied back (device to device, on the context
-    stream) before every RHS, so each step is the same RHS on the same input state."""
+def sync():
     import torch
 
-    def one():
-        for field, t in (restore or {}).items():
-            ctx.upload_device(field, t.data_ptr())
-        return ctx.rhs(flags)
-
-    for i in range(warmup):
-        one()
-        log(f"{label}: warmup {i + 1}/{warmup} done")
     torch.cuda.synchronize()
-    stats = []
-    t0 = time.perf_counter()
-    for i in range(steps):
-        stats.append(one())
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) / steps
-    log(f"{label}: {steps} timed steps, {wall * 1e3:.3f} ms/step")
-    return wall, stats
 
 
 def median_stats(stats):
@@ -96,25 +76,67 @@ def median_stats(stats):
     return out
 
 
-def cpu_baseline_sedov(s) -> dict:
-    """The oracle (FP64, OpenMP) on the same Sedov 128^3 input: one full SPH RHS."""
+def time_steps(step, steps: int, warmup: int, label: str):
+    """W untimed steps, then K steps each bracketed by device synchronisations; returns the
+    per-step wall times (s), the total wall time of the K steps and the steps' results."""
+    for i in range(warmup):
+        step()
+        log(f"{label}: warmup {i + 1}/{warmup} done")
+    sync()
+    walls, res = [], []
+    t_all = time.perf_counter()
+    for i in range(steps):
+        t0 = time.perf_counter()
+        res.append(step())
+        sync()
+        walls.append(time.perf_counter() - t0)
+    total = time.perf_counter() - t_all
+    log(f"{label}: {steps} timed steps, median {np.median(walls) * 1e3:.3f} ms/step")
+    return np.array(walls), total, res
+
+
+def cpu_baseline_sedov(state: dict, sml_max: float) -> dict:
+    """The oracle (FP64, OpenMP, -O3 AVX2) on the same evolved Sedov state: full SPH RHS
+    (h-iteration, neighbour lists, density, EOS, forces), timed at C level."""
     sys.path.insert(0, str(REPO / "oracle"))
     import pyoracle as orc
 
     cores = len(os.sched_getaffinity(0))
     orc.set_threads(cores)
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < 3 or (time.perf_counter() - t_start < 10.0 and len(times) < 12):
-        t0 = time.perf_counter()
-        orc.sph_rhs(s)
-        times.append(time.perf_counter() - t0)
-    dt = float(np.median(times))
-    n = len(s["x"])
-    return {"value": n / dt, "unit": "particles/s", "cores": cores, "kind": "port",
-            "sample": f"full SPH RHS (h-iteration, neighbours, density, EOS, forces) of the Sedov "
-                      f"{round(n ** (1 / 3))}^3 input, median of {len(times)} runs ({sum(times):.1f} s of CPU "
-                      f"work), {dt:.3f} s per RHS on {cores} host threads"}
+    secs = list(orc.bench_sph_rhs(state, 2, sml_max=sml_max))
+    while sum(secs) < 10.0 and len(secs) < 12:
+        secs += list(orc.bench_sph_rhs(state, 1, sml_max=sml_max))
+    dt = float(np.median(secs))
+    n = len(state["x"])
+    return {"value": n / dt, "unit": "particles/s", "cores": cores, "kind": "port", "impl": "oracle",
+            "sample": f"the FP64 oracle (oracle/oracle.c, one CPU restatement of the same algorithm: Jacobi "
+                      f"h-iteration, grid neighbour lists, density, EOS, forces; gcc -O3 -march=x86-64-v3 "
+                      f"-fopenmp) on the evolved Sedov {round(n ** (1 / 3))}^3 state of the timed run, timed at "
+                      f"C level: median of {len(secs)} RHS ({sum(secs):.1f} s of CPU work), {dt:.3f} s per RHS "
+                      f"on {cores} host threads"}
+
+
+def cpu_baseline_plummer(n: int, params, sample_every: int = 16) -> dict:
+    """The oracle's Barnes-Hut RHS on a Plummer sphere of n particles (the per-GPU share of the
+    16.7M sphere on eight GPUs): tree phases over all particles, the group walk over every
+    sample_every-th group, extrapolated to all groups."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import pyoracle as orc
+    from miluphpc_amd import ics
+
+    cores = len(os.sched_getaffinity(0))
+    orc.set_threads(cores)
+    pl = ics.plummer(n)
+    r = orc.bench_gravity(pl, params.theta, params.smoothing, curve=params.curve, mac=params.mac,
+                          quad=bool(params.quadrupole), sample_every=sample_every, vote=params.bh_accept == 1)
+    groups = (n + 63) // 64
+    t = r["tree_s"] + r["walk_s"] * groups / r["groups_walked"]
+    return {"value": n / t, "unit": "particles/s", "cores": cores, "kind": "port", "impl": "oracle",
+            "sample": f"FP64 oracle Barnes-Hut RHS (bbox, keys, sort, insertion octree, COM + quadrupole, "
+                      f"group walk of the same acceptance form) on a {n}-particle Plummer sphere: tree phases "
+                      f"{r['tree_s']:.2f} s over all particles, walk {r['walk_s']:.2f} s over "
+                      f"{r['groups_walked']} of {groups} groups (every {sample_every}th) extrapolated; "
+                      f"{t:.1f} s per RHS on {cores} host threads"}
 
 
 def measured_copy_ceiling(torch, nbytes: int = 1 << 30, reps: int = 5) -> float:
@@ -138,16 +160,18 @@ def measured_copy_ceiling(torch, nbytes: int = 1 << 30, reps: int = 5) -> float:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=18, help="timed PC-Euler steps of the Sedov run")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed PC-Euler steps before them")
     ap.add_argument("--nside", type=int, default=128)
     ap.add_argument("--plummer-n", type=int, default=16777216)
+    ap.add_argument("--plummer-steps", type=int, default=5)
     ap.add_argument("--no-plummer", action="store_true")
     ap.add_argument("--no-sedov", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--no-two-walk", action="store_true")
+    ap.add_argument("--no-initial", action="store_true", help="skip the initial-state RHS line")
     ap.add_argument("--no-shard", action="store_true", help="skip the N/8 Plummer line")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the Plummer direct-sum check and drift")
     ap.add_argument("--learned-nside", type=int, default=128)
     ap.add_argument("--learned-steps", type=int, default=10)
     ap.add_argument("--learned-train-iters", type=int, default=100)
@@ -181,81 +205,127 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
         "hbm_copy_ceiling_GBs": copy_gbs,
     }
-    if args.no_sedov:
-        s = None
-    else:
-        s = run_sedov(args, lib, ics, torch, result)
+    state = None
+    if not args.no_sedov:
+        state = run_sedov(args, lib, ics, result)
+        if not args.no_initial:
+            run_sedov_initial(args, lib, ics, result)
     if not args.no_plummer:
         run_plummer(args, lib, ics, result)
     if not args.no_learned:
         run_learned(args, lib, ics, result)
-    if not args.no_cpu and s is not None:
-        log("cpu baseline (oracle) ...")
-        result["cpu_baseline"] = cpu_baseline_sedov(s)
+    if not args.no_cpu and state is not None:
+        log("cpu baseline (oracle, Sedov) ...")
+        result["cpu_baseline"] = cpu_baseline_sedov(state, result["config"]["sml_max"])
         log(f"cpu baseline {result['cpu_baseline']['value']:.4e} particles/s")
+        if "plummer" in result and not args.no_shard:
+            log("cpu baseline (oracle, Plummer N/8) ...")
+            cb = cpu_baseline_plummer(args.plummer_n // 8, ics.params_plummer_large())
+            result["plummer"]["shard_of_8"]["cpu_baseline"] = cb
+            log(f"cpu baseline plummer {cb['value']:.4e} particles/s")
     print(json.dumps(result), flush=True)
     return 0
 
 
-def run_sedov(args, lib, ics, torch, result):
-    # ---- Sedov 128^3, SPH, plain h-iteration, FP64 ------------------------------------
+STATE_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "mass", "sml", "e", "rho", "p", "cs", "ax", "ay", "az",
+                "g_ax", "g_ay", "g_az", "drhodt", "dedt", "dsmldt", "muijmax", "curlv", "gradrho", "noi")
+
+
+def run_sedov(args, lib, ics, result):
+    """The headline: Sedov 128^3 advanced by PC-Euler steps (CFL 0.3), each step two RHS of the
+    plain h-iteration with its neighbour lists written by pass 0 (only particles whose list can
+    change are re-listed: the same h and neighbour sets as the reference's count-then-list
+    structure)."""
     s = ics.sedov_lattice(args.nside)
     n = len(s["x"])
     log(f"sedov {args.nside}^3 = {n} particles generated")
-    p = ics.params_sedov()
-    # the plain h-iteration with its neighbour lists written by pass 0 (only particles whose
-    # list can change are re-listed): the same h and neighbour sets as the reference's
-    # count-then-list structure, which is timed beside it
+    for k in ("ax", "ay", "az", "dedt", "g_ax", "g_ay", "g_az", "muijmax", "drhodt", "dsmldt"):
+        s.setdefault(k, np.zeros(n))
+    # free boundary: h is bounded at 4 h0 so that the corner and edge particles (which can never
+    # reach the band under d < min(h_i, h_j)) do not grow h without limit over the run
+    sml_max = 4.0 * float(s["sml"].max())
+    p = ics.params_sedov(sml_max=sml_max)
     flags = lib.RHS_SPH | lib.RHS_VARIABLE_SML | lib.RHS_SPECULATIVE_NN
-    two_walk = lib.RHS_SPH | lib.RHS_VARIABLE_SML
-    # the h-iteration rewrites sml; every step restarts from the initial smoothing lengths
-    h0 = torch.tensor(s["sml"], dtype=torch.float64, device="cuda")
     with lib.Context(n, p) as ctx:
         ctx.load(**{k: v for k, v in s.items() if k in lib.FIELD_ID})
-        wall, stats = time_rhs(ctx, lib, flags, args.steps, args.warmup, "sedov", restore={"sml": h0})
-        if not args.no_two_walk:
-            wall_s, stats_s = time_rhs(ctx, lib, two_walk, args.steps, 1, "sedov-two-walk", restore={"sml": h0})
-    if not args.no_two_walk:
-        med_s = median_stats(stats_s)
-        result["sedov_two_walk"] = {
-            "note": "same RHS with the reference's structure: count-only h-iteration, then a full list walk",
-            "value": n / wall_s, "ms_per_step": wall_s * 1e3,
-            "phases_ms": {k[3:]: round(v, 4) for k, v in med_s.items() if k.startswith("ms_")}}
+        walls, total, res = time_steps(lambda: ctx.pc_euler_step(flags, courant_fact=0.3), args.steps,
+                                       args.warmup, "sedov")
+        state = {k: ctx.download(k) for k in STATE_FIELDS}
+    stats = [st for _, s0, s1 in res for st in (s0, s1)]
+    dts = [dt for dt, _, _ in res]
     med = median_stats(stats)
-    nnb = med["n_neighbours"] / n
-    bpp = sph_bytes_per_particle(nnb)
-    dom = ROOFLINE_KERNEL
-    alg_bytes = bpp[dom] * n
+    nnb = float(np.mean([st["n_neighbours"] for st in stats])) / n
+    table = sph_kernel_table(nnb)
+    kernels = {}
+    for k, (name, bpp) in table.items():
+        gbs = bpp * n / (med[k] * 1e-3) / 1e9
+        kernels[k[3:]] = {"kernel": name, "ms": round(med[k], 4), "alg_bytes_per_particle": round(bpp, 1),
+                          "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    kernels["nn_fixup"] = {"kernel": "k_nn_mark_dirty + k_nn_walk<0> + k_nn_one<0> (re-lists)",
+                           "ms": round(med["ms_nn"], 4), "note": "re-lists only the changed neighbourhoods"}
+    kernels["hiter_rest"] = {"kernel": "pass-0 outliers + tail passes (k_nn_one, k_nn_count_direct, k_h_finish)",
+                             "ms": round(med["ms_hiter"] - med["ms_nn_walk"], 4)}
+    # the roofline is quoted on the kernel with the largest time per RHS
+    dom = max(table, key=lambda k: med[k])
+    name, bpp = table[dom]
+    alg_bytes = bpp * n
     achieved = alg_bytes / (med[dom] * 1e-3) / 1e9
-    kernels = {k[3:]: {"ms": round(med[k], 4), "alg_bytes_per_particle": round(bpp[k], 1),
-                       "GB/s": round(bpp[k] * n / (med[k] * 1e-3) / 1e9, 1),
-                       "frac": round(bpp[k] * n / (med[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-               for k in bpp}
+    step_ms = float(np.median(walls)) * 1e3
     result.update({
-        "value": n / wall,
-        "ms_per_step": wall * 1e3,
+        "value": 2 * n / (step_ms * 1e-3),
+        "ms_per_step": step_ms,
+        "ms_per_step_mean": total / args.steps * 1e3,
+        "ms_per_rhs": step_ms / 2,
         "dtype": "f64",
-        "config": {"workload": f"Sedov-Taylor {args.nside}^3 lattice SPH RHS, plain h-iteration "
-                               f"(target 64, band 50-80), cubic spline, ideal gas, FP64",
-                   "n_particles": n, "path": "plain h-iteration, lists from pass 0 (speculative)",
-                   "sort": "key levels of the previous RHS's tree + 1, checked, full sort when needed "
-                           "(the timed RHS follow the warm-up)"},
-        "roofline": {"bound": "hbm", "kernel": ROOFLINE_KERNEL_NAME, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(ROOFLINE_KERNEL_NAME),
-                     "algorithmic_bytes_per_launch": alg_bytes,
-                     "algorithmic_bytes_per_particle": bpp[dom], "kernel_ms": med[dom]},
+        "config": {"workload": f"Sedov-Taylor {args.nside}^3 lattice SPH, PC-Euler steps under CFL 0.3 (2 RHS "
+                               f"per step), plain h-iteration (target 64, band 50-80, h bounded at 4 h0), cubic "
+                               f"spline, ideal gas, FP64",
+                   "n_particles": n, "rhs_per_step": 2, "sml_max": sml_max,
+                   "steps_before_timing": args.warmup,
+                   "path": "plain h-iteration, lists from pass 0 (speculative)",
+                   "sort": "key levels of the previous RHS's tree + 1, checked, full sort when needed"},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name),
+                     "algorithmic_bytes_per_launch": alg_bytes, "algorithmic_bytes_per_particle": bpp,
+                     "kernel_ms": med[dom], "timing": "hipEvents around the kernel on the context stream, "
+                                                      "median over the timed RHS"},
         "sph_kernels": kernels,
         "phases_ms": {k[3:]: round(v, 4) for k, v in med.items() if k.startswith("ms_")},
         "mean_neighbours": nnb,
-        "mean_h_counts": med["n_h_counts"] / n,
+        "mean_h_counts": float(np.mean([st["n_h_counts"] for st in stats])) / n,
+        "h_passes_max": int(max(st["h_passes"] for st in stats)),
+        "sim_time": float(np.sum(dts)),
     })
-    log(f"sedov: {n / wall:.4e} particles/s, dominant {dom} {med[dom]:.3f} ms, {achieved:.1f} GB/s")
-    return s
+    log(f"sedov: {result['value']:.4e} RHS particles/s, {step_ms / 2:.3f} ms/RHS, dominant {name} "
+        f"{med[dom]:.3f} ms, {achieved:.1f} GB/s")
+    return state
 
 
-STATE_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "mass", "sml", "e", "rho", "p", "cs", "ax", "ay", "az",
-                "g_ax", "g_ay", "g_az", "drhodt", "dedt", "dsmldt", "muijmax", "curlv", "gradrho", "noi")
+def run_sedov_initial(args, lib, ics, result):
+    """The same RHS on the Sedov initial state (h restored on the device before each RHS), and
+    the reference's count-then-list structure beside it."""
+    import torch
+
+    s = ics.sedov_lattice(args.nside)
+    n = len(s["x"])
+    p = ics.params_sedov()
+    out = {}
+    h0 = torch.tensor(s["sml"], dtype=torch.float64, device="cuda")
+    with lib.Context(n, p) as ctx:
+        ctx.load(**{k: v for k, v in s.items() if k in lib.FIELD_ID})
+        for name, flags in (("speculative", lib.RHS_SPH | lib.RHS_VARIABLE_SML | lib.RHS_SPECULATIVE_NN),
+                            ("two_walk", lib.RHS_SPH | lib.RHS_VARIABLE_SML)):
+            def one():
+                ctx.upload_device("sml", h0.data_ptr())
+                return ctx.rhs(flags)
+
+            walls, _, res = time_steps(one, 10, 2, f"sedov-initial-{name}")
+            med = median_stats(res)
+            out[name] = {"ms_per_rhs": float(np.median(walls)) * 1e3, "value": n / float(np.median(walls)),
+                         "phases_ms": {k[3:]: round(v, 4) for k, v in med.items() if k.startswith("ms_")}}
+    result["sedov_initial_state"] = {
+        "note": "RHS on the Sedov initial state (h restored before each RHS); two_walk = the reference's "
+                "structure, a count-only h-iteration then a full list walk", **out}
 
 
 def run_learned(args, lib, ics, result):
@@ -272,8 +342,6 @@ def run_learned(args, lib, ics, result):
     n = len(s["x"])
     for k in ("ax", "ay", "az", "dedt", "g_ax", "g_ay", "g_az", "muijmax", "drhodt", "dsmldt"):
         s.setdefault(k, np.zeros(n))
-    # free boundary: bound h at 4 h0 so the corner particles (which can never reach the band)
-    # do not grow h without limit over the run
     p = ics.params_sedov(sml_max=4.0 * float(s["sml"].max()))
     base = lib.RHS_SPH | lib.RHS_VARIABLE_SML
     train = dict(nsamples=65536, iters=args.learned_train_iters, lr=2e-3, seed=5)
@@ -317,43 +385,84 @@ def run_learned(args, lib, ics, result):
     }
 
 
+def plummer_accuracy(ctx, lib, pl, params, nsample: int = 4096):
+    """99th-percentile relative force error of the RHS just computed against FP64 direct
+    summation (the oracle) on a seeded random subsample."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import pyoracle as orc
+
+    orc.set_threads(len(os.sched_getaffinity(0)))
+    sub = np.sort(np.random.default_rng(4096).choice(len(pl["x"]), nsample, replace=False)).astype(np.int32)
+    g = {k: ctx.download(k)[sub] for k in ("g_ax", "g_ay", "g_az")}
+    t0 = time.perf_counter()
+    d = orc.direct(pl["x"], pl["y"], pl["z"], pl["mass"], params.smoothing, targets=sub)
+    num = np.sqrt((g["g_ax"] - d["ax"]) ** 2 + (g["g_ay"] - d["ay"]) ** 2 + (g["g_az"] - d["az"]) ** 2)
+    err = num / np.sqrt(d["ax"] ** 2 + d["ay"] ** 2 + d["az"] ** 2)
+    log(f"plummer: direct-sum check of {nsample} particles {time.perf_counter() - t0:.1f} s, "
+        f"p99 {np.percentile(err, 99):.3e}")
+    return {"subsample": nsample, "seed": 4096, "p50": float(np.median(err)), "p99": float(np.percentile(err, 99)),
+            "max": float(err.max())}
+
+
 def run_plummer(args, lib, ics, result):
     # ---- Plummer 16M, Barnes-Hut, theta 0.6, quadrupole, FP32 + Kahan ----------------
-    if True:
-        pl = ics.plummer(args.plummer_n)
-        log(f"plummer {args.plummer_n} generated")
-        pp = ics.params_plummer_large()
-        with lib.Context(args.plummer_n, pp) as ctx:
-            ctx.load(**pl)
-            pw, pst = time_rhs(ctx, lib, lib.RHS_GRAVITY, max(1, args.steps // 2), 1, "plummer")
+    pl = ics.plummer(args.plummer_n)
+    log(f"plummer {args.plummer_n} generated")
+    pp = ics.params_plummer_large()
+    with lib.Context(args.plummer_n, pp) as ctx:
+        ctx.load(**pl)
+        walls, _, pst = time_steps(lambda: ctx.rhs(lib.RHS_GRAVITY), max(1, args.steps // 2), 1, "plummer")
         pm = median_stats(pst)
-        flops = 23.0 * pm["n_part_interactions"] + 68.0 * pm["n_cell_interactions"]
-        tflops = flops / (pm["ms_gravity"] * 1e-3) / 1e12
-        result["plummer"] = {
-            "workload": f"Plummer {args.plummer_n} Barnes-Hut RHS, theta 0.6, eps 2e-4, monopole + "
-                        "quadrupole, FP32 + Kahan far field, FP64 positions and keys",
-            "value": args.plummer_n / pw, "unit": "particles/s", "ms_per_step": pw * 1e3,
-            "phases_ms": {k[3:]: round(v, 4) for k, v in pm.items() if k.startswith("ms_")},
-            "interactions_per_particle": (pm["n_part_interactions"] + pm["n_cell_interactions"]) / args.plummer_n,
-            "roofline": {"bound": "fp32", "kernel": "gravity", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
-                         "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS,
-                         "flops_per_interaction": {"particle": 23, "cell_quadrupole": 68}},
-        }
-        log(f"plummer: {args.plummer_n / pw:.4e} particles/s, gravity {pm['ms_gravity']:.2f} ms, {tflops:.2f} TFLOP/s")
+        acc = None if args.no_accuracy else plummer_accuracy(ctx, lib, pl, pp)
+        drift = None
+        if not args.no_accuracy:
+            # energy drift: leapfrog (kick-drift-kick) steps at dt = 5e-4 with the potential
+            dt = 5e-4
+            flags = lib.RHS_GRAVITY | lib.RHS_POTENTIAL
+            ctx.rhs(flags)
+            energies = [ctx.diagnostics()["total"]]
+            for _ in range(args.plummer_steps):
+                ctx.leapfrog_update_x(dt)
+                ctx.rhs(flags)
+                ctx.leapfrog_update_v(dt)
+                energies.append(ctx.diagnostics()["total"])
+            e = np.array(energies)
+            rel = (e - e[0]) / abs(e[0])
+            drift = {"steps": args.plummer_steps, "dt": dt, "E0": float(e[0]),
+                     "rel_drift_per_step": float(rel[-1] / args.plummer_steps),
+                     "max_abs_rel_drift": float(np.abs(rel).max()), "rel_history": [float(v) for v in rel]}
+            log(f"plummer: energy drift per step {drift['rel_drift_per_step']:.3e}")
+    flops = 23.0 * pm["n_part_interactions"] + 68.0 * pm["n_cell_interactions"]
+    tflops = flops / (pm["ms_gravity"] * 1e-3) / 1e12
+    step_ms = float(np.median(walls)) * 1e3
+    result["plummer"] = {
+        "workload": f"Plummer {args.plummer_n} Barnes-Hut RHS, theta 0.6, eps 2e-4, monopole + quadrupole, "
+                    "per-member (vote) group acceptance, FP32 + Kahan far field, FP64 positions and keys",
+        "value": args.plummer_n / (step_ms * 1e-3), "unit": "particles/s", "ms_per_step": step_ms,
+        "phases_ms": {k[3:]: round(v, 4) for k, v in pm.items() if k.startswith("ms_")},
+        "interactions_per_particle": (pm["n_part_interactions"] + pm["n_cell_interactions"]) / args.plummer_n,
+        "roofline": {"bound": "fp32", "kernel": "k_bh_walk", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS,
+                     "flops_per_interaction": {"particle": 23, "cell_quadrupole": 68}},
+        "accuracy_vs_direct": acc,
+        "energy_drift": drift,
+    }
+    log(f"plummer: {args.plummer_n / (step_ms * 1e-3):.4e} particles/s, gravity {pm['ms_gravity']:.2f} ms, "
+        f"{tflops:.2f} TFLOP/s")
+    del pl
+    if not args.no_shard:
+        # the per-GPU particle count of the same model on an 8-GPU node: a whole Plummer sphere
+        # of N / 8 particles (a proxy for one shard's kernel shape, not a spatial sub-domain)
+        n8 = args.plummer_n // 8
+        pl = ics.plummer(n8)
+        with lib.Context(n8, pp) as ctx:
+            ctx.load(**pl)
+            w8, _, pst8 = time_steps(lambda: ctx.rhs(lib.RHS_GRAVITY), max(1, args.steps // 2), 1, "plummer-shard")
+        pm8 = median_stats(pst8)
+        result["plummer"]["shard_of_8"] = {
+            "n_particles": n8, "value": n8 / float(np.median(w8)), "ms_per_step": float(np.median(w8)) * 1e3,
+            "gravity_ms": pm8["ms_gravity"]}
         del pl
-        if not args.no_shard:
-            # the per-GPU particle count of the same model on an 8-GPU node: a whole Plummer sphere
-            # of N / 8 particles (a proxy for one shard's kernel shape, not a spatial sub-domain)
-            n8 = args.plummer_n // 8
-            pl = ics.plummer(n8)
-            with lib.Context(n8, pp) as ctx:
-                ctx.load(**pl)
-                pw8, pst8 = time_rhs(ctx, lib, lib.RHS_GRAVITY, max(1, args.steps // 2), 1, "plummer-shard")
-            pm8 = median_stats(pst8)
-            result["plummer"]["shard_of_8"] = {
-                "n_particles": n8, "value": n8 / pw8, "ms_per_step": pw8 * 1e3,
-                "gravity_ms": pm8["ms_gravity"]}
-            del pl
 
 
 if __name__ == "__main__":

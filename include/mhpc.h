@@ -60,9 +60,21 @@ enum {
 };
 
 /* Barnes-Hut acceptance forms: s/d < theta (computeForces_v1, gravity.cu:252-396) or
- * s^2/theta <= d^2 + eps^2 (computeForces_v2, gravity.cu:726-903), both evaluated for a
- * group of consecutive key-sorted particles against the group's bounding box. */
+ * s^2/theta <= d^2 + eps^2 (computeForces_v2, gravity.cu:726-903); mhpc_params.bh_accept
+ * selects who tests. */
 enum { MHPC_MAC_CLASSIC = 0, MHPC_MAC_SQRT_THETA = 1 };
+
+/* Who applies the acceptance test (mhpc_params.bh_accept):
+ *  BOX: once per group of MHPC_BH_GROUP consecutive key-sorted particles, with the distance
+ *       from the cell to the group's bounding box (stricter: opens more cells);
+ *  VOTE (default): every member of the group tests the cell with its own distance and the
+ *       group accepts it only if all do -- computeForces_v2's warp vote (gravity.cu:836) over
+ *       the group;
+ *  PER_PARTICLE: computeForces_v1 itself (gravity.cu:252-396): every particle walks the tree
+ *       on its own, in the reference's depth-first order, skipping itself; FP64 and monopole
+ *       only (mac must be MHPC_MAC_CLASSIC, quadrupole 0; precision is ignored).  The
+ *       reference's plain method, slow: one thread per particle. */
+enum { MHPC_BH_BOX = 0, MHPC_BH_VOTE = 1, MHPC_BH_PER_PARTICLE = 2 };
 
 /* Pair-sum precision. */
 enum { MHPC_PREC_FP64 = 0, MHPC_PREC_FP32_KAHAN = 1 };
@@ -82,7 +94,8 @@ typedef struct {
     int eos;
     double gamma;           /* eos.polytropic_gamma */
     double K;               /* eos.polytropic_K */
-    double cs_const;        /* sound speed for MHPC_EOS_ISOTHERMAL_CONST_CS */
+    double cs_const;        /* sound speed of MHPC_EOS_ISOTHERMAL_CONST_CS; 0: each particle keeps
+                               its uploaded cs (MHPC_F_CS) */
     double alpha, beta;     /* Monaghan artificial viscosity */
     int n_target;           /* Material::interactions (target neighbour count) */
     int band_lo, band_hi;   /* accepted neighbour-count band of the h-iteration */
@@ -94,6 +107,7 @@ typedef struct {
     double sml_max;         /* build extension: upper bound of h in the h-iteration; 0 = none (the
                                reference).  On a free boundary a particle that cannot reach the
                                band under d < min(h_i, h_j) otherwise grows h on every pass. */
+    int bh_accept;          /* MHPC_BH_VOTE (default), MHPC_BH_BOX or MHPC_BH_PER_PARTICLE */
 } mhpc_params;
 
 /* Field ids for upload / download / device pointers (reference Entry::Name, particles.cuh).
@@ -144,7 +158,9 @@ typedef struct {
     int64_t n_neighbours;              /* sum of noi */
     int64_t n_h_counts;                /* neighbour counts done by the h-iteration */
     int32_t max_noi;
-    int32_t h_passes;
+    int32_t h_passes;                  /* h-iteration passes that counted at least one particle */
+    float ms_nn_walk;                  /* the h-iteration's pass-0 group walk kernel alone (part of
+                                          ms_hiter) */
 } mhpc_rhs_stats;
 
 /* -------- context (replaces the Miluphpc ctor allocations, miluphpc.cpp:3-93, and the
@@ -165,6 +181,21 @@ int mhpc_synchronize(mhpc_ctx *ctx);
 int mhpc_upload(mhpc_ctx *ctx, int field, const void *host, int64_t n);
 int mhpc_download(mhpc_ctx *ctx, int field, void *host, int64_t n);
 int mhpc_field_ptr(mhpc_ctx *ctx, int field, void **device_ptr);
+/* Point an FP64 field at a caller-owned device array of at least the context's capacity
+ * (device_ptr NULL: back to the context's own array); every later stage reads and writes that
+ * array.  ParticleHandler::setPointer / resetPointer (particle_handler.cpp:742-900), which the
+ * predictor-corrector integrator uses to run rhs() on the predicted particle set
+ * (predictor_corrector_euler.cpp:119-133). */
+int mhpc_bind_field(mhpc_ctx *ctx, int field, void *device_ptr);
+/* Device memory for such arrays (IntegratedParticleHandler's allocations,
+ * particle_handler.cpp:1076-1230): zeroed; mhpc_device_free refuses an array still bound. */
+int mhpc_device_alloc(mhpc_ctx *ctx, int64_t bytes, void **device_ptr);
+int mhpc_device_free(mhpc_ctx *ctx, void *device_ptr);
+/* Blocking copies between host memory and such device memory, after the context stream's work
+ * (and, being null-stream copies, after earlier null-stream work) -- the cudaMemcpy calls of
+ * IntegratedParticleHandler's users. */
+int mhpc_device_upload(mhpc_ctx *ctx, void *device_ptr, const void *host, int64_t bytes);
+int mhpc_device_download(mhpc_ctx *ctx, void *host, const void *device_ptr, int64_t bytes);
 /* device-to-device copy of n elements into a field, ordered on the context stream (for
  * callers whose inputs already live in HBM, e.g. restoring a state between RHS calls) */
 int mhpc_upload_device(mhpc_ctx *ctx, int field, const void *device_src, int64_t n);

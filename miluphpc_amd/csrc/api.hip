@@ -44,6 +44,7 @@ extern "C" void mhpc_default_params(mhpc_params *p) {
     p->sml_avg = 1;
     p->precision = MHPC_PREC_FP64;
     p->cs_before_p = 1;
+    p->bh_accept = MHPC_BH_VOTE;
 }
 
 extern "C" const char *mhpc_status_string(int s) {
@@ -70,6 +71,8 @@ static int check_params(const mhpc_params *p) {
     if (p->n_target < 1 || p->band_lo > p->band_hi) return 0;
     if (p->precision != MHPC_PREC_FP64 && p->precision != MHPC_PREC_FP32_KAHAN) return 0;
     if (!(p->sml_max >= 0.0)) return 0;
+    if (p->bh_accept < MHPC_BH_BOX || p->bh_accept > MHPC_BH_PER_PARTICLE) return 0;
+    if (p->bh_accept == MHPC_BH_PER_PARTICLE && (p->mac != MHPC_MAC_CLASSIC || p->quadrupole)) return 0;
     return 1;
 }
 
@@ -113,7 +116,7 @@ static void free_all(mhpc_ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
-        if (c->f[f]) (void)hipFree(c->f[f]);
+        if (c->f_own[f]) (void)hipFree(c->f_own[f]);
     if (c->h_err) (void)hipHostFree(c->h_err);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     if (c->h_sort_check) (void)hipHostFree(c->h_sort_check);
@@ -130,8 +133,9 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     c->cap_pad = ceil_div(cap, 64) * 64;
     c->cap_cells = cap_cells;
     for (int f = 0; f < MHPC_F_NOI; f++) {
-        HIPC(dalloc(&c->f[f], cap));
-        HIPC(hipMemset(c->f[f], 0, sizeof(double) * (size_t)cap));
+        HIPC(dalloc(&c->f_own[f], cap));
+        HIPC(hipMemset(c->f_own[f], 0, sizeof(double) * (size_t)cap));
+        c->f[f] = c->f_own[f];
     }
     HIPC(dalloc(&c->noi, cap));
     HIPC(dalloc(&c->level, cap));
@@ -252,8 +256,17 @@ extern "C" int mhpc_destroy(mhpc_ctx *c) {
 extern "C" int mhpc_set_params(mhpc_ctx *c, const mhpc_params *p) {
     if (!c || !check_params(p)) return MHPC_E_ARG;
     const int old_target = c->p.n_target;
+    const bool changed = memcmp(&c->p, p, sizeof(*p)) != 0;
     c->p = *p;
     if (p->n_target != old_target) upload_htable(c);
+    if (changed) {
+        // keys depend on the curve, lists on max_nb, sums on the precision and the EOS: every
+        // stage after a parameter change recomputes from the keys (stage calls in between
+        // report MHPC_E_STATE)
+        c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
+        c->nn_valid = c->rho_valid = c->eos_valid = false;
+        c->nn_start_valid = false;
+    }
     return MHPC_OK;
 }
 
@@ -333,6 +346,51 @@ extern "C" int mhpc_field_ptr(mhpc_ctx *c, int field, void **device_ptr) {
     FieldDesc d = field_desc(c, field);
     if (!d.ptr) return MHPC_E_ARG;
     *device_ptr = d.ptr;
+    return MHPC_OK;
+}
+
+static void invalidate(mhpc_ctx *c) {
+    c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
+    c->nn_valid = c->rho_valid = c->eos_valid = false;
+}
+
+extern "C" int mhpc_bind_field(mhpc_ctx *c, int field, void *device_ptr) {
+    if (!c || field < 0 || field >= MHPC_F_NOI) return MHPC_E_ARG;
+    HIPC(hipStreamSynchronize(c->stream));   // work queued on the previous array finishes first
+    c->f[field] = device_ptr ? static_cast<double *>(device_ptr) : c->f_own[field];
+    invalidate(c);
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_device_alloc(mhpc_ctx *c, int64_t bytes, void **device_ptr) {
+    if (!c || !device_ptr || bytes < 1) return MHPC_E_ARG;
+    *device_ptr = nullptr;
+    HIPC(hipMalloc(device_ptr, (size_t)bytes));
+    HIPC(hipMemset(*device_ptr, 0, (size_t)bytes));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_device_upload(mhpc_ctx *c, void *device_ptr, const void *host, int64_t bytes) {
+    if (!c || !device_ptr || !host || bytes < 0) return MHPC_E_ARG;
+    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(hipMemcpy(device_ptr, host, (size_t)bytes, hipMemcpyHostToDevice));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_device_download(mhpc_ctx *c, void *host, const void *device_ptr, int64_t bytes) {
+    if (!c || !device_ptr || !host || bytes < 0) return MHPC_E_ARG;
+    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(hipMemcpy(host, device_ptr, (size_t)bytes, hipMemcpyDeviceToHost));
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_device_free(mhpc_ctx *c, void *device_ptr) {
+    if (!c) return MHPC_E_ARG;
+    if (!device_ptr) return MHPC_OK;
+    for (int f = 0; f < MHPC_F_NOI; f++)
+        if (c->f[f] == device_ptr) return MHPC_E_STATE;   // still bound to a field
+    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(hipFree(device_ptr));
     return MHPC_OK;
 }
 
@@ -802,24 +860,11 @@ extern "C" int mhpc_leapfrog_update_v(mhpc_ctx *c, double dt) {
 
 extern "C" int mhpc_compute_diagnostics(mhpc_ctx *c, mhpc_diagnostics *out) {
     if (!c || !out) return MHPC_E_ARG;
-    double q[32];
-    launch_diagnostics(c, c->diag_partial, c->diag_out);
+    mhpc_diagnostics *d = reinterpret_cast<mhpc_diagnostics *>(c->diag_out);
+    launch_diagnostics(c, c->diag_partial, d);
     HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(q, c->diag_out, sizeof(double) * diag_num_quantities(), hipMemcpyDeviceToHost,
-                        c->stream));
+    HIPC(hipMemcpyAsync(out, d, sizeof(*out), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
-    memset(out, 0, sizeof(*out));
-    out->kinetic = q[0];
-    out->potential = q[1];
-    out->internal = q[2];
-    out->total = q[0] + q[1] + q[2];
-    for (int a = 0; a < 3; a++) {
-        out->momentum[a] = q[3 + a];
-        out->angular_momentum[a] = q[6 + a];
-        out->com[a] = q[9] > 0.0 ? q[10 + a] / q[9] : 0.0;
-    }
-    out->angular_momentum_norm = sqrt(q[6] * q[6] + q[7] * q[7] + q[8] * q[8]);
-    out->mass = q[9];
     return MHPC_OK;
 }
 
@@ -927,7 +972,7 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     if (grav) launch_gravity(c, pot);
     mark(7);
     if (hiter) {
-        launch_h_iterate(c, c->p.max_counts, spec);
+        launch_h_iterate(c, c->p.max_counts, spec, c->ev + 14);
         if (!spec) launch_scatter_h(c);   // with lists from pass 0, the list fix-up publishes h
     }
     mark(8);
@@ -971,7 +1016,16 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
         stats->n_neighbours = (int64_t)c->h_counters[CNT_NEIGHBOURS];
         stats->max_noi = (int32_t)c->h_counters[CNT_MAX_NOI];
         stats->n_h_counts = (int64_t)c->h_counters[CNT_H_COUNTS];
-        stats->h_passes = hiter ? c->p.max_counts : 0;
+        stats->ms_nn_walk = hiter ? elapsed(c, 14, 15) : 0.f;
+        stats->h_passes = 0;
+        if (hiter) {
+            // pass 0, then every pass whose input list (the previous pass's unconverged
+            // particles) was not empty
+            uint32_t lc[MHPC_MAX_PASSES + 1];
+            stats->h_passes = 1;
+            if (hipMemcpy(lc, c->hlist_count, sizeof(lc), hipMemcpyDeviceToHost) == hipSuccess)
+                for (int pass = 1; pass < c->p.max_counts; pass++) stats->h_passes += lc[pass - 1] > 0 ? 1 : 0;
+        }
     }
     return st;
 }

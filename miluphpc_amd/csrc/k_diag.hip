@@ -9,6 +9,7 @@
 
 #define DIAG_Q 14          // quantities
 #define DIAG_BLOCKS 1024
+#define DIAG_PARTIAL (DIAG_BLOCKS * 2 * DIAG_Q)   // doubles of per-block partials
 
 struct Neu {
     double s = 0.0, c = 0.0;
@@ -42,8 +43,8 @@ __global__ __launch_bounds__(256) void k_diag_partial(int64_t n, const double *_
         const double mi = m[i], xi = x[i], yi = y[i], zi = z[i];
         const double a = vx[i], b = vy[i], cz = vz[i];
         q[0].add(0.5 * mi * (a * a + b * b + cz * cz));
-        q[1].add(u[i]);
-        q[2].add(mi * e[i]);
+        q[1].add(u ? u[i] : 0.0);
+        q[2].add(e ? mi * e[i] : 0.0);
         q[3].add(mi * a);
         q[4].add(mi * b);
         q[5].add(mi * cz);
@@ -73,27 +74,58 @@ __global__ __launch_bounds__(256) void k_diag_partial(int64_t n, const double *_
     }
 }
 
-__global__ void k_diag_final(const double *__restrict__ partial, int nb, double *__restrict__ out) {
+// folds the block partials in block order and writes the diagnostics record (the derived
+// quantities -- total, |L|, centre of mass -- with the same expressions for every caller)
+__global__ void k_diag_final(const double *__restrict__ partial, int nb, mhpc_diagnostics *__restrict__ out) {
+    __shared__ double q[DIAG_Q];
     const int k = threadIdx.x;
-    if (k >= DIAG_Q) return;
-    Neu acc;
-    for (int b = 0; b < nb; b++) {
-        acc.add(partial[(int64_t)b * 2 * DIAG_Q + 2 * k]);
-        acc.add(partial[(int64_t)b * 2 * DIAG_Q + 2 * k + 1]);
+    if (k < DIAG_Q) {
+        Neu acc;
+        for (int b = 0; b < nb; b++) {
+            acc.add(partial[(int64_t)b * 2 * DIAG_Q + 2 * k]);
+            acc.add(partial[(int64_t)b * 2 * DIAG_Q + 2 * k + 1]);
+        }
+        q[k] = acc.value();
     }
-    out[k] = acc.value();
+    __syncthreads();
+    if (k != 0) return;
+    mhpc_diagnostics d;
+    d.kinetic = q[0];
+    d.potential = q[1];
+    d.internal = q[2];
+    d.total = q[0] + q[1] + q[2];
+    for (int a = 0; a < 3; a++) {
+        d.momentum[a] = q[3 + a];
+        d.angular_momentum[a] = q[6 + a];
+        d.com[a] = q[9] > 0.0 ? q[10 + a] / q[9] : 0.0;
+    }
+    d.angular_momentum_norm = sqrt(q[6] * q[6] + q[7] * q[7] + q[8] * q[8]);
+    d.mass = q[9];
+    *out = d;
 }
 
-int diag_num_quantities() { return DIAG_Q; }
-int diag_partial_size() { return DIAG_BLOCKS * 2 * DIAG_Q; }
-
-void launch_diagnostics(mhpc_ctx *c, double *partial, double *out) {
-    int nb = (int)ceil_div(c->n, 256 * 4);
+static int diag_blocks(int64_t n) {
+    int nb = (int)ceil_div(n, 256 * 4);
     if (nb < 1) nb = 1;
     if (nb > DIAG_BLOCKS) nb = DIAG_BLOCKS;
+    return nb;
+}
+
+// device doubles the context keeps for the record (at least the five time-step terms)
+int diag_num_quantities() { return (int)(sizeof(mhpc_diagnostics) / sizeof(double)) > 5
+                                       ? (int)(sizeof(mhpc_diagnostics) / sizeof(double)) : 5; }
+int diag_partial_size() { return DIAG_PARTIAL; }
+
+static void diag_launch(hipStream_t stream, int64_t n, const double *x, const double *y, const double *z,
+                        const double *vx, const double *vy, const double *vz, const double *m,
+                        const double *u, const double *e, double *partial, mhpc_diagnostics *out) {
+    const int nb = diag_blocks(n);
+    hipLaunchKernelGGL(k_diag_partial, dim3(nb), dim3(256), 0, stream, n, x, y, z, vx, vy, vz, m, u, e, partial);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, stream, partial, nb, out);
+}
+
+void launch_diagnostics(mhpc_ctx *c, double *partial, mhpc_diagnostics *out) {
     double **f = c->f;
-    hipLaunchKernelGGL(k_diag_partial, dim3(nb), dim3(256), 0, c->stream, c->n, f[MHPC_F_X], f[MHPC_F_Y],
-                       f[MHPC_F_Z], f[MHPC_F_VX], f[MHPC_F_VY], f[MHPC_F_VZ], f[MHPC_F_MASS], f[MHPC_F_U],
-                       f[MHPC_F_E], partial);
-    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, c->stream, partial, nb, out);
+    diag_launch(c->stream, c->n, f[MHPC_F_X], f[MHPC_F_Y], f[MHPC_F_Z], f[MHPC_F_VX], f[MHPC_F_VY],
+                f[MHPC_F_VZ], f[MHPC_F_MASS], f[MHPC_F_U], f[MHPC_F_E], partial, out);
 }

@@ -4,9 +4,15 @@
 // or once per warp of 32 sorted particles with a warp vote on the acceptance test
 // (computeForces_v2, gravity.cu:726-903).  Here one wavefront owns a GROUP of 64
 // consecutive key-sorted particles and walks the tree once for all of them:
-//   * the 64 lanes test up to 64 stacked nodes at a time against the group's bounding
-//     box (FP64, exact, the acceptance test the oracle restates), so the traversal
-//     itself is parallel across lanes;
+//   * the 64 lanes test up to 64 stacked nodes at a time (FP64, exact, the acceptance
+//     test the oracle restates), so the traversal itself is parallel across lanes.  Vote
+//     form (the reference v2's warp vote, gravity.cu:836, over the group of 64): a cell is
+//     accepted only if every member accepts it with its own distance.  A lane's cell that
+//     passes the test against the group's bounding box passes it for every member (the box
+//     distance is a lower bound of each member's, also after rounding), so only the other
+//     cells are put to the vote: for each, all 64 lanes test it against their own particle
+//     and one ballot decides.  Box form: the bounding-box test alone (one test per group);
+//     it opens more cells;
 //   * leaves and accepted cells go to two LDS interaction lists shared by the wave;
 //   * each time a list holds 64 entries all 64 lanes evaluate it, every lane reading
 //     the same LDS entry (broadcast) for its own particle.  FP32 mode: entries are
@@ -232,6 +238,14 @@ __device__ __forceinline__ void eval_cells(const BHWaveLds64 &L, int cnt, double
     }
 }
 
+// a uniform lane's double (l must be the same in every lane)
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 struct BHArgs {
     int64_t n;
     const int32_t *perm;
@@ -247,7 +261,7 @@ struct BHArgs {
     unsigned long long *counters;
 };
 
-template <bool FP64, bool POT, bool QUAD>
+template <bool FP64, bool POT, bool QUAD, bool VOTE>
 __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
     typedef typename BHLds<FP64>::type Lds;
     __shared__ Lds lds_all[BH_WAVES];
@@ -296,26 +310,41 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
         bool accept = false, open = false;
         double ccx = 0, ccy = 0, ccz = 0;
         int64_t c = 0;
+        int lev = 0;
+        // the acceptance test of distance^2 d2 for a cell of level L (same expressions as the
+        // oracle)
+        auto mac_ok = [&](double d2, int L) {
+            if (A.mac == MHPC_MAC_CLASSIC) return (d2 + eps2) * theta2 > s2_0 * ldexp(1.0, -2 * L);
+            return dp_1 * ldexp(1.0, -2 * (L - 1)) <= d2 + eps2;
+        };
         if (isCell) {
             c = node - n;
             ccx = A.cx[c];
             ccy = A.cy[c];
             ccz = A.cz[c];
-            const int lev = A.clevel[c];
+            lev = A.clevel[c];
             double dx = 0.0, dy = 0.0, dz = 0.0;
             if (ccx < glo_x) dx = glo_x - ccx; else if (ccx > ghi_x) dx = ccx - ghi_x;
             if (ccy < glo_y) dy = glo_y - ccy; else if (ccy > ghi_y) dy = ccy - ghi_y;
             if (ccz < glo_z) dz = glo_z - ccz; else if (ccz > ghi_z) dz = ccz - ghi_z;
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            if (A.mac == MHPC_MAC_CLASSIC) {
-                const double s2 = s2_0 * ldexp(1.0, -2 * lev);
-                accept = (d2 + eps2) * theta2 > s2;
-            } else {
-                const double dp = dp_1 * ldexp(1.0, -2 * (lev - 1));
-                accept = dp <= d2 + eps2;
-            }
-            open = !accept;
+            accept = mac_ok(dx * dx + dy * dy + dz * dz, lev);
         }
+        if constexpr (VOTE) {
+            // cells the box test rejects: every member tests each with its own distance
+            uint64_t mAmb = wave_ballot(isCell && !accept);
+            uint64_t mVote = 0;
+            while (mAmb) {
+                const int b = __builtin_ctzll(mAmb);
+                mAmb &= mAmb - 1;
+                const double bx = readlane_d(ccx, b), by = readlane_d(ccy, b), bz = readlane_d(ccz, b);
+                const int bl = __builtin_amdgcn_readlane(lev, b);
+                const double dx = bx - px, dy = by - py, dz = bz - pz;
+                const bool ok = !valid || mac_ok(dx * dx + dy * dy + dz * dz, bl);
+                if (wave_ballot(ok) == ~0ull) mVote |= 1ull << b;
+            }
+            accept = accept || ((mVote >> lane) & 1ull);
+        }
+        open = isCell && !accept;
         const uint64_t mP = wave_ballot(isPart), mA = wave_ballot(accept);
         if (isPart) {
             const int pos = pl_n + popc64(mP & lt);
@@ -448,8 +477,106 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
 template <bool FP64, bool POT, bool QUAD>
 static void launch_bh(mhpc_ctx *c, const BHArgs &a) {
     const int64_t groups = ceil_div(c->n, 64);
-    hipLaunchKernelGGL((k_bh_walk<FP64, POT, QUAD>), dim3((unsigned)ceil_div(groups, BH_WAVES)),
-                       dim3(64 * BH_WAVES), 0, c->stream, a);
+    const dim3 grid((unsigned)ceil_div(groups, BH_WAVES)), block(64 * BH_WAVES);
+    if (c->p.bh_accept == MHPC_BH_VOTE)
+        hipLaunchKernelGGL((k_bh_walk<FP64, POT, QUAD, true>), grid, block, 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_bh_walk<FP64, POT, QUAD, false>), grid, block, 0, c->stream, a);
+}
+
+// ------------------------------------------------------------------------------------
+// a7' computeForces_v1 (gravity.cu:252-396) as the reference runs it: one thread per particle
+// walks the tree depth first from the root, resuming each level at its next child slot
+// (curNode / curChild per depth), skipping itself; a child is taken if it is a particle or
+// (d^2 + eps^2) theta^2 > cellSize[depth], cellSize[d] = 4 R^2 / 4^d, else descended into.
+// Monopole, FP64, in the reference's order and expressions (f = m / d^3 with d the square
+// root of the softened distance; u -= 1/2 m_c m_i / d).  The per-depth resume state lives in
+// LDS; the tree has at most MHPC_LEVELS + 1 levels below the root.
+#define BH1_THREADS 64
+#define BH1_DEPTH (MHPC_LEVELS + 3)
+
+template <bool POT>
+__global__ __launch_bounds__(BH1_THREADS) void k_bh_v1(BHArgs A) {
+    __shared__ int cur_node[BH1_DEPTH][BH1_THREADS];
+    __shared__ unsigned char cur_child[BH1_DEPTH][BH1_THREADS];
+    const int t = threadIdx.x;
+    const int64_t k = blockIdx.x * (int64_t)BH1_THREADS + t;
+    const int64_t n = A.n;
+    if (k >= n) return;
+    const double4 own = A.sxm[k];
+    const double px = own.x, py = own.y, pz = own.z;
+    const double R = 0.5 * (A.box[1] - A.box[0]);
+    const double cs0 = 4.0 * R * R;
+    const double thetasq = A.theta * A.theta;
+    double axi = 0.0, ayi = 0.0, azi = 0.0, ui = 0.0;
+    long long n_cells = 0, n_parts = 0;
+    int depth = 1;
+    cur_node[1][t] = (int)n;   // root
+    cur_child[1][t] = 0;
+    bool overflow = false;
+    do {
+        int child_number = cur_child[depth][t];
+        int node = cur_node[depth][t];
+        // cellSize[depth] of the reference's table: 4 R^2 / 4^depth, exact powers of 4
+        double cell_size = cs0 * ldexp(1.0, -2 * depth);
+        while (child_number < 8) {
+            int c;
+            do {
+                c = node >= n ? A.cchild[8 * (int64_t)(node - n) + child_number] : -1;
+                child_number++;
+            } while (c == -1 && child_number < 8);
+            if (c != -1 && c != (int)k) {
+                double cx, cy, cz, cm;
+                if (c < n) {
+                    const double4 r = A.sxm[c];
+                    cx = r.x; cy = r.y; cz = r.z; cm = r.w;
+                } else {
+                    const int64_t q = c - n;
+                    cx = A.cx[q]; cy = A.cy[q]; cz = A.cz[q]; cm = A.cmass[q];
+                }
+                const double dx = cx - px;
+                double dist = dx * dx + A.eps2;
+                const double dy = cy - py;
+                dist += dy * dy;
+                const double dz = cz - pz;
+                dist += dz * dz;
+                if (c < n || dist * thetasq > cell_size) {
+                    dist = sqrt(dist);
+                    const double f = cm / (dist * dist * dist);
+                    axi += f * dx;
+                    ayi += f * dy;
+                    azi += f * dz;
+                    if (POT) ui -= 0.5 * (cm * own.w) / dist;
+                    if (c < n) n_parts++; else n_cells++;
+                } else {
+                    cur_child[depth][t] = (unsigned char)child_number;
+                    cur_node[depth][t] = node;
+                    depth++;
+                    if (depth >= BH1_DEPTH) {
+                        overflow = true;
+                        depth = 0;
+                        break;
+                    }
+                    child_number = 0;
+                    node = c;
+                    cell_size = cs0 * ldexp(1.0, -2 * depth);
+                }
+            }
+        }
+        depth--;
+    } while (depth > 0);
+    if (overflow) set_err(A.err, ERRF_STACK);
+    const int32_t i = A.perm[k];
+    A.gax[i] = A.G * axi;
+    A.gay[i] = A.G * ayi;
+    A.gaz[i] = A.G * azi;
+    if (POT) A.u[i] = A.G * ui;
+    const unsigned long long sc = (unsigned long long)wave_sum(n_cells);
+    const unsigned long long sp = (unsigned long long)wave_sum(n_parts);
+    if (wave_lane() == 0) {
+        atomicAdd(&A.counters[CNT_CELL_INT], sc);
+        atomicAdd(&A.counters[CNT_PART_INT], sp);
+    }
 }
 
 void launch_gravity(mhpc_ctx *c, bool potential) {
@@ -471,6 +598,12 @@ void launch_gravity(mhpc_ctx *c, bool potential) {
     a.u = c->f[MHPC_F_U];
     a.err = c->err;
     a.counters = c->counters;
+    if (c->p.bh_accept == MHPC_BH_PER_PARTICLE) {
+        const dim3 grid((unsigned)ceil_div(c->n, BH1_THREADS)), block(BH1_THREADS);
+        if (potential) hipLaunchKernelGGL(k_bh_v1<true>, grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL(k_bh_v1<false>, grid, block, 0, c->stream, a);
+        return;
+    }
     const bool fp64 = c->p.precision == MHPC_PREC_FP64;
     const bool quad = c->p.quadrupole != 0;
     if (fp64) {

@@ -1092,7 +1092,7 @@ __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint8_t *dirt
 // a9: the plain h-iteration as Jacobi passes; sh holds the final h afterwards.  Pass 0
 // runs in group mode over all particles and lists the unconverged ones; later passes run
 // in per-particle mode over that list only.
-void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
+void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative, hipEvent_t *walk_events) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_h_init, dim3((unsigned)ceil_div(c->n > 64 ? c->n : 64, 256)), dim3(256), 0, c->stream,
                        c->n, c->sdone, c->siters, c->sdirty, c->hlist_count, 2 * (MHPC_MAX_PASSES + 1), c->ocount,
@@ -1118,6 +1118,7 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     const unsigned g1 = (unsigned)std::min<int64_t>(ceil_div(c->n, NN1_WAVES), NN1_MAX_BLOCKS);
     a.out_list = c->olist;
     a.out_count = c->ocount;   // zeroed by k_h_init
+    if (walk_events) (void)hipEventRecord(walk_events[0], c->stream);
     if (speculative) {
         a.nbr4 = reinterpret_cast<int4 *>(c->nbr);
         a.stride = c->cap_pad;
@@ -1125,6 +1126,7 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative) {
     } else {
         hipLaunchKernelGGL(k_nn_walk<1>, dim3(nn_grid(c)), dim3(64 * NN_WAVES), 0, c->stream, a);
     }
+    if (walk_events) (void)hipEventRecord(walk_events[1], c->stream);
     // the group mode's outliers, same pass, per-particle mode
     {
         NNArgs o = a;
@@ -1269,12 +1271,15 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // a12 / a13 sound speed (soundspeed.cu:38-97) and pressure (pressure.cu:6-87), fused;
 // also writes the records the force loop gathers: cs into {vx,vy,vz,cs} and
 // {rho, p/rho^2, m/rho, h}.
-__device__ __forceinline__ double eos_cs(int eos, double gamma, double K, double rho, double p,
+// cs_const > 0: the constant sound speed of MHPC_EOS_ISOTHERMAL_CONST_CS (otherwise that EOS
+// keeps each particle's uploaded cs)
+__device__ __forceinline__ double eos_cs(int eos, double gamma, double K, double cs_const, double rho, double p,
                                          double cs) {
     switch (eos) {
         case MHPC_EOS_POLYTROPIC: return sqrt(K * pow(rho, gamma - 1.0));
         case MHPC_EOS_IDEAL_GAS: return sqrt(gamma * p / rho);
         case MHPC_EOS_LOCALLY_ISOTHERMAL: return 0.0;
+        case MHPC_EOS_ISOTHERMAL_CONST_CS: return cs_const > 0.0 ? cs_const : cs;
         default: return cs;
     }
 }
@@ -1295,7 +1300,7 @@ __device__ __forceinline__ double eos_p(int eos, double gamma, double K, double 
 // density kernel's fused form)
 struct EosArgs {
     int eos, cs_before_p;
-    double gamma, K;
+    double gamma, K, cs_const;
     const double *se;
     double *sp, *scs;
     double4 *sv4, *sr4;
@@ -1306,12 +1311,13 @@ __device__ __forceinline__ void eos_particle(const EosArgs &E, int64_t k, int32_
                                              double h) {
     const double e = E.se[k];
     double p = E.sp[k], cs = E.scs[k];
+    if (E.eos == MHPC_EOS_ISOTHERMAL_CONST_CS) cs = eos_cs(E.eos, E.gamma, E.K, E.cs_const, rho, p, cs);
     if (E.cs_before_p) {
-        cs = eos_cs(E.eos, E.gamma, E.K, rho, p, cs);
+        cs = eos_cs(E.eos, E.gamma, E.K, E.cs_const, rho, p, cs);
         p = eos_p(E.eos, E.gamma, E.K, rho, e, cs);
     } else {
         p = eos_p(E.eos, E.gamma, E.K, rho, e, cs);
-        cs = eos_cs(E.eos, E.gamma, E.K, rho, p, cs);
+        cs = eos_cs(E.eos, E.gamma, E.K, E.cs_const, rho, p, cs);
     }
     E.sp[k] = p;
     E.scs[k] = cs;
@@ -1327,6 +1333,7 @@ static EosArgs eos_args(mhpc_ctx *c) {
     E.cs_before_p = c->p.cs_before_p;
     E.gamma = c->p.gamma;
     E.K = c->p.K;
+    E.cs_const = c->p.cs_const;
     E.se = c->se;
     E.sp = c->sp;
     E.scs = c->scs;

@@ -65,7 +65,9 @@ struct mhpc_ctx {
     hipStream_t stream = nullptr;
 
     // caller-order fields
-    double *f[MHPC_F_COUNT_] = {};   // FP64 fields (ids < MHPC_F_NOI)
+    double *f[MHPC_F_COUNT_] = {};   // FP64 fields (ids < MHPC_F_NOI) the kernels use
+    double *f_own[MHPC_F_COUNT_] = {};   // the context's own arrays (f differs while a field is bound
+                                         // to a caller's array, mhpc_bind_field)
     int32_t *noi = nullptr, *level = nullptr, *h_iters = nullptr;
     uint64_t *key = nullptr;         // sort key per particle (caller order)
     uint64_t *lkey = nullptr;        // Lebesgue key per particle (caller order)
@@ -171,7 +173,8 @@ void launch_gather(mhpc_ctx *c, bool gravity, bool sph, bool rhs);
 void launch_tree(mhpc_ctx *c);
 void launch_bottom_up(mhpc_ctx *c, bool gravity);
 void launch_gravity(mhpc_ctx *c, bool potential);
-void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative);
+// walk_events (may be null): two events recorded around the pass-0 group walk kernel
+void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative, hipEvent_t *walk_events = nullptr);
 void launch_neighbours(mhpc_ctx *c);
 void launch_neighbours_fixup(mhpc_ctx *c);
 void launch_density(mhpc_ctx *c);
@@ -198,7 +201,7 @@ void launch_pc_corrector(mhpc_ctx *c, double dt);
 void launch_time_step(mhpc_ctx *c, double *partial, double *out);
 int diag_num_quantities();
 int diag_partial_size();
-void launch_diagnostics(mhpc_ctx *c, double *partial, double *out);
+void launch_diagnostics(mhpc_ctx *c, double *partial, mhpc_diagnostics *out);
 
 // ------------------------------------------------------------------------------------
 // device helpers

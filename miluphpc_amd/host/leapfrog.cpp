@@ -4,7 +4,13 @@
 #include "leapfrog.h"
 
 Leapfrog::Leapfrog(const SimulationParameters &sp, integer numParticles, const Material &material)
-    : Miluphpc(sp, numParticles, material) {}
+    : Miluphpc(sp, numParticles, material) {
+    particleHandler->initLeapfrog();
+}
+
+Leapfrog::Leapfrog(const SimulationParameters &sp, const Material &material) : Miluphpc(sp, material) {
+    particleHandler->initLeapfrog();
+}
 
 Leapfrog::~Leapfrog() = default;
 

@@ -46,7 +46,24 @@ public:
 
     Miluphpc(const SimulationParameters &simulationParameters, integer numParticles,
              const Material &material = Material());
+    // the reference's construction (miluphpc.cpp:3-93, prepareSimulation :300-385): the
+    // particle count and the initial distribution come from simulationParameters.inputFile
+    // (a directory of .npy datasets, see npy_io.h)
+    explicit Miluphpc(const SimulationParameters &simulationParameters, const Material &material = Material());
     virtual ~Miluphpc();
+
+    // Miluphpc::distributionFromFile (miluphpc.cpp:157-300): /m, /x (N x 3), /v (N x 3),
+    // /materialId, /u (internal energy, SPH runs) and, when present, /rho and /sml (otherwise
+    // every particle gets the material's sml), into the host arrays and then the device.
+    // Throws std::runtime_error on a missing dataset or a size mismatch.
+    void distributionFromFile(const std::string &filename);
+    // Miluphpc::particles2file (miluphpc.cpp:3451-3760): the snapshot of the current state as
+    // simulationParameters.directory + "ts%06d" (datasets time, x, v, key, m, proc, rho, p, e,
+    // sml, noi, cs, and totalEnergy / totalAngularMomentum when those are computed).  Returns
+    // the elapsed milliseconds.
+    real particles2file(int step);
+    // particle count of a distribution directory (its /m dataset)
+    static integer particlesInFile(const std::string &filename);
 
     // Right-hand side (miluphpc.cpp:387-562).  Throws MhpcException on a device error
     // (where the reference exits or traps).

@@ -1,6 +1,7 @@
 // parameter.h -- host-side types of the drop-in facade.  Names follow the reference's
 // include/parameter.h (real/integer/keyType typedefs :12-19, SimulationParameters
-// :122-159, struct To :161-171) so calling code compiles against either.
+// :122-159, struct To :161-171, Execution :189-201, Curve :203-216, Entry :252-271) so
+// calling code compiles against either.
 #pragma once
 
 #include <string>
@@ -24,6 +25,20 @@ struct To {
 namespace Curve {
 enum Type { lebesgue = 0, hilbert = 1 };
 }
+
+struct Execution {
+    enum Location { host, device };
+    Location t_;
+    Execution(Location t) : t_(t) {}
+    operator Location() const { return t_; }
+};
+
+struct Entry {
+    enum Name { x, y, z, mass };
+    Name t_;
+    Entry(Name t) : t_(t) {}
+    operator Name() const { return t_; }
+};
 
 // Runtime parameters.  Fields of the reference that configure the multi-process, I/O or
 // load-balancing machinery (outputRank, loadBalancing*, particlesSent2H5, domainListSize,
@@ -51,11 +66,14 @@ struct SimulationParameters {
     // gravity
     real theta = 0.5;
     real smoothing = 0.0;          // eps^2
-    // acceptance form of the reference's kernel versions (miluphpc.cpp:2150-2280): 0 and 1
-    // (computeForces_v2, _v2_1) test s^2/theta <= d^2+eps^2; 2, 3, 4 (computeForces_v1,
-    // _v1_1, _v1_2) test (d^2+eps^2) theta^2 > s^2.  Both are evaluated per group of 64
-    // key-sorted particles against the group's bounding box.
+    // the reference's kernel versions (miluphpc.cpp:2150-2280): 0 and 1 (computeForces_v2,
+    // _v2_1) test s^2/theta <= d^2+eps^2, all members of a group of 64 key-sorted particles
+    // voting; 2, 3, 4 (computeForces_v1, _v1_1, _v1_2) test (d^2+eps^2) theta^2 > s^2 per
+    // particle, each particle walking the tree on its own
     int gravityForceVersion = 0;
+    // build extension: versions 2-4 with the group walk (the v1 test, every member of the group
+    // voting) instead of one walk per particle
+    bool groupWalk = false;
     // SPH
     int smoothingKernelSelection = 1;   // cubic spline
     int sphFixedRadiusNNVersion = 0;

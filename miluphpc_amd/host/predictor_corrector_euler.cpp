@@ -10,6 +10,9 @@ PredictorCorrectorEuler::PredictorCorrectorEuler(const SimulationParameters &sp,
                                                  const Material &material)
     : Miluphpc(sp, numParticles, material) {}
 
+PredictorCorrectorEuler::PredictorCorrectorEuler(const SimulationParameters &sp, const Material &material)
+    : Miluphpc(sp, material) {}
+
 PredictorCorrectorEuler::~PredictorCorrectorEuler() = default;
 
 void PredictorCorrectorEuler::integrate(int step) {

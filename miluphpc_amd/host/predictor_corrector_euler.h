@@ -10,6 +10,9 @@ class PredictorCorrectorEuler : public Miluphpc {
 public:
     PredictorCorrectorEuler(const SimulationParameters &simulationParameters, integer numParticles,
                             const Material &material = Material());
+    // particles from simulationParameters.inputFile
+    explicit PredictorCorrectorEuler(const SimulationParameters &simulationParameters,
+                                     const Material &material = Material());
     ~PredictorCorrectorEuler() override;
     void integrate(int step = 0) override;
     real lastTimeStep() const { return lastDt_; }

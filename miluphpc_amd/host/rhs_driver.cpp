@@ -5,8 +5,18 @@
 //
 //   mhpc_rhs_driver gravity <n> <steps> <out>   Leapfrog steps on a uniform ball (FP64)
 //   mhpc_rhs_driver sph <nside> <out>           one SPH rhs() on a jittered lattice
+//   mhpc_rhs_driver io <sph|gravity> <in> <outdir> <steps>
+//        the reference's run: integrator constructed from the initial distribution <in>
+//        (a directory of .npy datasets), snapshot ts000000 after the first rhs(), then
+//        <steps> integrate() calls, each followed by a snapshot ts%06d
+//   mhpc_rhs_driver npycopy <in.npy> <out.npy>   reads and rewrites one .npy file (CPU only)
+//   mhpc_rhs_driver pointer <nside> <outdir>
+//        setPointer / resetPointer: one SPH rhs() on a second particle set (an
+//        IntegratedParticleHandler holding a shifted copy of the lattice), then one on the
+//        particle set itself; both results as snapshots (ts000001: second set, ts000002:
+//        the particle set)
 //
-// File layout: int64 n, then n doubles per field in the order printed by the driver.
+// File layout of gravity / sph: int64 n, then n doubles per field in the order printed.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -15,6 +25,8 @@
 #include <vector>
 
 #include "leapfrog.h"
+#include "npy_io.h"
+#include "predictor_corrector_euler.h"
 
 static uint64_t rng_state = 0x9E3779B97F4A7C15ull;
 static double urand() {   // splitmix64 -> [0, 1)
@@ -45,6 +57,7 @@ static int run_gravity(int n, int steps, const char *out) {
     sp.theta = 0.5;
     sp.smoothing = 1e-6;
     sp.gravityForceVersion = 2;   // v1 acceptance form
+    sp.groupWalk = true;          // ... in the group walk
     sp.sfcSelection = Curve::hilbert;
     sp.timeStep = 1e-3;
     sp.timeEnd = 1e-3 * steps;
@@ -132,14 +145,117 @@ static int run_sph(int nside, const char *out) {
     return 0;
 }
 
+// read a .npy file and write it back (the reader and writer alone; no device needed)
+static int run_npycopy(const char *in, const char *out) {
+    npy::Array a;
+    std::string err = npy::read(in, a);
+    if (err.empty()) err = npy::write(out, a.type, a.shape, a.data.data());
+    if (!err.empty()) {
+        fprintf(stderr, "error: %s\n", err.c_str());
+        return 4;
+    }
+    printf("copied %lld elements, %d dimensions\n", (long long)a.count(), (int)a.shape.size());
+    return 0;
+}
+
+static int run_io(bool sph, const char *in, const char *outdir, int steps) {
+    SimulationParameters sp;
+    sp.inputFile = in;
+    sp.directory = outdir;
+    sp.gravitySim = !sph;
+    sp.sphSim = sph;
+    sp.variableSML = sph;
+    sp.theta = 0.5;
+    sp.smoothing = 1e-6;
+    sp.sfcSelection = Curve::hilbert;
+    sp.gravityForceVersion = 2;   // computeForces_v1: one walk per particle
+    sp.timeStep = 1e-3;
+    sp.calculateEnergy = !sph;
+    sp.calculateAngularMomentum = true;
+    Material mat;
+    mat.interactions = 64;
+    mat.sml = 0.1;
+    Miluphpc *sim = nullptr;
+    if (sph) sim = new PredictorCorrectorEuler(sp, mat);
+    else sim = new Leapfrog(sp, mat);
+    sim->rhs(0, true, true);
+    sim->afterIntegrationStep();
+    sim->particles2file(0);
+    for (int s = 0; s < steps; s++) {
+        sim->simulationTimeHandler.h_subEndTime = sim->simulationTimeHandler.h_currentTime + sp.timeStep;
+        sim->integrate(s + 1);
+        sim->afterIntegrationStep();
+        sim->particles2file(s + 1);
+    }
+    printf("particles %d time %.17g\n", sim->numParticles, sim->simulationTimeHandler.h_currentTime);
+    delete sim;
+    return 0;
+}
+
+static int run_pointer(int nside, const char *outdir) {
+    const int n = nside * nside * nside;
+    SimulationParameters sp;
+    sp.gravitySim = false;
+    sp.sphSim = true;
+    sp.variableSML = true;
+    sp.directory = outdir;
+    Material mat;
+    PredictorCorrectorEuler sim(sp, n, mat);
+    ParticleHandler *ph = sim.particleHandler;
+    const double h0 = 2.5 / nside;   // about 64 neighbours on the lattice
+    int i = 0;
+    for (int a = 0; a < nside; a++)
+        for (int b = 0; b < nside; b++)
+            for (int c = 0; c < nside; c++, i++) {
+                ph->h_x[i] = -0.5 + (c + 0.5 + 0.3 * (urand() - 0.5)) / nside;
+                ph->h_y[i] = -0.5 + (b + 0.5 + 0.3 * (urand() - 0.5)) / nside;
+                ph->h_z[i] = -0.5 + (a + 0.5 + 0.3 * (urand() - 0.5)) / nside;
+                ph->h_vx[i] = 0.1 * (urand() - 0.5);
+                ph->h_vy[i] = 0.1 * (urand() - 0.5);
+                ph->h_vz[i] = 0.1 * (urand() - 0.5);
+                ph->h_mass[i] = 1.0 / n;
+                ph->h_sml[i] = h0;
+                ph->h_e[i] = 1.0 + urand();
+                ph->h_p[i] = 0.5;
+                ph->h_rho[i] = 1.0;
+            }
+    // the particle set
+    ph->copyDistribution(To::device, true, false);
+    ph->copySPH(To::device);
+    // the second set: the same particles shifted by +0.25 in x, uploaded through the bound arrays
+    IntegratedParticleHandler iph(n, sim.numNodes, sim.context());
+    ph->setPointer(&iph);
+    if (ph->d_x != iph.d_x || ph->d_sml != iph.d_sml) return 5;
+    for (int k = 0; k < n; k++) ph->h_x[k] += 0.25;
+    ph->copyDistribution(To::device, true, false);
+    ph->copySPH(To::device);
+    sim.rhs(0, false, false);
+    ph->copyDistribution(To::host, true, false);
+    ph->copySPH(To::host);
+    sim.particles2file(1);
+    ph->resetPointer();
+    if (ph->d_x == iph.d_x) return 6;
+    // back on the particle set: its arrays were not touched by the rhs() above
+    ph->copyDistribution(To::host, true, false);
+    ph->copySPH(To::host);
+    sim.rhs(0, false, false);
+    sim.particles2file(2);
+    printf("pointer ok\n");
+    return ph->lastStatus() == MHPC_OK ? 0 : 3;
+}
+
 int main(int argc, char **argv) {
     try {
         if (argc == 5 && !strcmp(argv[1], "gravity")) return run_gravity(atoi(argv[2]), atoi(argv[3]), argv[4]);
         if (argc == 4 && !strcmp(argv[1], "sph")) return run_sph(atoi(argv[2]), argv[3]);
-    } catch (const MhpcException &e) {
+        if (argc == 6 && !strcmp(argv[1], "io")) return run_io(!strcmp(argv[2], "sph"), argv[3], argv[4], atoi(argv[5]));
+        if (argc == 4 && !strcmp(argv[1], "pointer")) return run_pointer(atoi(argv[2]), argv[3]);
+        if (argc == 4 && !strcmp(argv[1], "npycopy")) return run_npycopy(argv[2], argv[3]);
+    } catch (const std::exception &e) {
         fprintf(stderr, "error: %s\n", e.what());
         return 4;
     }
-    fprintf(stderr, "usage: %s gravity <n> <steps> <out> | sph <nside> <out>\n", argv[0]);
+    fprintf(stderr, "usage: %s gravity <n> <steps> <out> | sph <nside> <out> | io <sph|gravity> <in> <outdir> <steps>"
+                    " | pointer <nside> <outdir>\n", argv[0]);
     return 1;
 }

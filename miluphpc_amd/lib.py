@@ -28,6 +28,7 @@ CURVE_LEBESGUE, CURVE_HILBERT = 0, 1
 EOS_POLYTROPIC, EOS_ISOTHERMAL, EOS_IDEAL_GAS = 0, 3, 9
 EOS_LOCALLY_ISOTHERMAL, EOS_ISOTHERMAL_CONST_CS = 12, 100
 MAC_CLASSIC, MAC_SQRT_THETA = 0, 1
+BH_BOX, BH_VOTE, BH_PER_PARTICLE = 0, 1, 2
 PREC_FP64, PREC_FP32_KAHAN = 0, 1
 
 RHS_GRAVITY = 1 << 0
@@ -79,6 +80,7 @@ class Params(C.Structure):
         ("precision", C.c_int),
         ("cs_before_p", C.c_int),
         ("sml_max", C.c_double),
+        ("bh_accept", C.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -130,6 +132,7 @@ class RhsStats(C.Structure):
         ("n_h_counts", C.c_int64),
         ("max_noi", C.c_int32),
         ("h_passes", C.c_int32),
+        ("ms_nn_walk", C.c_float),
     ]
 
     def as_dict(self) -> dict:
@@ -160,6 +163,11 @@ def _load() -> C.CDLL:
         "mhpc_download": (i32, [vp, i32, vp, i64]),
         "mhpc_field_ptr": (i32, [vp, i32, C.POINTER(vp)]),
         "mhpc_upload_device": (i32, [vp, i32, vp, i64]),
+        "mhpc_bind_field": (i32, [vp, i32, vp]),
+        "mhpc_device_alloc": (i32, [vp, i64, C.POINTER(vp)]),
+        "mhpc_device_free": (i32, [vp, vp]),
+        "mhpc_device_upload": (i32, [vp, vp, vp, i64]),
+        "mhpc_device_download": (i32, [vp, vp, vp, i64]),
         "mhpc_download_neighbours": (i32, [vp, vp, vp, i64, i32]),
         "mhpc_tree_size": (i32, [vp, C.POINTER(i64)]),
         "mhpc_download_tree": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -297,6 +305,31 @@ class Context:
         n = self.n if n is None else int(n)
         self._check(LIB.mhpc_upload_device(self._h, FIELD_ID[field], C.c_void_p(int(device_ptr)), n),
                     f"upload_device {field}")
+
+    def bind_field(self, field: str, device_ptr: int | None) -> None:
+        """Point a field at a caller-owned device array (None: the context's own array)."""
+        self._check(LIB.mhpc_bind_field(self._h, FIELD_ID[field], C.c_void_p(device_ptr or None)),
+                    f"bind_field {field}")
+
+    def device_alloc(self, nbytes: int) -> int:
+        """Zeroed device memory (mhpc_device_alloc); free with device_free."""
+        p = C.c_void_p()
+        self._check(LIB.mhpc_device_alloc(self._h, int(nbytes), C.byref(p)), "mhpc_device_alloc")
+        return int(p.value)
+
+    def device_free(self, device_ptr: int) -> None:
+        self._check(LIB.mhpc_device_free(self._h, C.c_void_p(device_ptr)), "mhpc_device_free")
+
+    def device_upload(self, device_ptr: int, host: np.ndarray) -> None:
+        a = np.ascontiguousarray(host)
+        self._check(LIB.mhpc_device_upload(self._h, C.c_void_p(device_ptr), _ptr(a), a.nbytes),
+                    "mhpc_device_upload")
+
+    def device_download(self, device_ptr: int, count: int, dtype=np.float64) -> np.ndarray:
+        a = np.empty(int(count), dtype=dtype)
+        self._check(LIB.mhpc_device_download(self._h, _ptr(a), C.c_void_p(device_ptr), a.nbytes),
+                    "mhpc_device_download")
+        return a
 
     def field_ptr(self, field: str) -> int:
         p = C.c_void_p()

@@ -304,3 +304,56 @@ def test_outlier_particles_match_oracle(gpu_lib, orc, sedov24):
     st, onnl, onoi = orc.nn_grid(s["x"], s["y"], s["z"], oh, 180)
     for name in out:
         assert sets(*out[name][2]) == sets(onnl, onoi), name
+
+
+@pytest.mark.parametrize("eos,K,gamma", [(0, 0.35, 1.4), (3, 0.0, 5.0 / 3.0), (12, 0.0, 5.0 / 3.0),
+                                         (100, 0.0, 5.0 / 3.0)])
+def test_eos_variants_match_oracle(gpu_lib, orc, sedov24, eos, K, gamma):
+    """Polytropic (0), isothermal (3), locally isothermal (12) and the constant-c_s isothermal
+    build extension (100) through the full SPH RHS, FP64, against the oracle."""
+    print_case(f"eos {eos} sedov 24^3")
+    s = dict(sedov24)
+    s["cs"] = np.full(len(s["x"]), 0.25)
+    p = gpu_lib.default_params(curve=1, precision=0, eos=eos, K=K, gamma=gamma)
+    flags = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML
+    with make_ctx(gpu_lib, s, p) as ctx:
+        ctx.rhs(flags)
+        g = {k: ctx.download(k) for k in ("sml", "rho", "p", "cs", "ax", "dedt")}
+    o = orc.sph_rhs(s, eos=eos, K=K, gamma=gamma)
+    assert np.array_equal(g["sml"], o["sml"])
+    assert np.max(np.abs(g["rho"] - o["rho"]) / o["rho"]) < 1e-12
+    assert np.max(np.abs(g["cs"] - o["cs"]) / np.maximum(np.abs(o["cs"]), 1e-300)) < 1e-12
+    assert np.max(np.abs(g["p"] - o["p"]) / np.maximum(np.abs(o["p"]), 1e-300)) < 1e-12
+    for k in ("ax", "dedt"):
+        scale = np.sqrt(np.mean(o[k] ** 2))
+        assert np.max(np.abs(g[k] - o[k]) / np.maximum(np.abs(o[k]), scale)) < 1e-10, k
+
+
+def test_constant_sound_speed_parameter(gpu_lib, orc, sedov24):
+    """EOS 100 with mhpc_params.cs_const set and the particles' cs field left at zero: the
+    parameter is the sound speed (p = cs_const^2 rho)."""
+    print_case("eos 100 cs_const with a zero cs field")
+    s = dict(sedov24)
+    s["cs"] = np.zeros(len(s["x"]))
+    p = gpu_lib.default_params(curve=1, precision=0, eos=100, cs_const=0.167)
+    with make_ctx(gpu_lib, s, p) as ctx:
+        ctx.rhs(gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML)
+        g = {k: ctx.download(k) for k in ("rho", "p", "cs", "ax")}
+    assert np.all(g["cs"] == 0.167)
+    assert np.max(np.abs(g["p"] - 0.167 ** 2 * g["rho"]) / g["p"]) < 1e-15
+    o = orc.sph_rhs({**s, "cs": np.full(len(s["x"]), 0.167)}, eos=100)
+    scale = np.sqrt(np.mean(o["ax"] ** 2))
+    assert np.max(np.abs(g["ax"] - o["ax"]) / np.maximum(np.abs(o["ax"]), scale)) < 1e-10
+
+
+def test_set_params_invalidates_stages(gpu_lib, sedov24):
+    """A parameter change between stages (here the curve) makes the later stages report a
+    call-order error until the keys are recomputed."""
+    s = sedov24
+    with make_ctx(gpu_lib, s, gpu_lib.default_params(curve=0)) as ctx:
+        ctx.bbox(); ctx.keys(); ctx.sort()
+        ctx.set_params(curve=1)
+        with pytest.raises(gpu_lib.MhpcError) as e:
+            ctx.build_tree()
+        assert e.value.status == gpu_lib.E_STATE
+        ctx.bbox(); ctx.keys(); ctx.sort(); ctx.build_tree()

@@ -216,16 +216,17 @@ def run_gravity(gpu_lib, s, params, potential=False):
     return out
 
 
-@pytest.mark.parametrize("mac", [0, 1])
-def test_barnes_hut_fp64_parity(gpu_lib, orc, plummer64k, mac):
-    print_case(f"bh fp64 plummer 65536 mac={mac}")
+@pytest.mark.parametrize("mac,vote", [(0, 1), (1, 1), (0, 0), (1, 0)])
+def test_barnes_hut_fp64_parity(gpu_lib, orc, plummer64k, mac, vote):
+    print_case(f"bh fp64 plummer 65536 mac={mac} vote={vote}")
     s = plummer64k
-    p = gpu_lib.default_params(theta=0.5, smoothing=1e-6, curve=1, mac=mac, precision=0)
+    p = gpu_lib.default_params(theta=0.5, smoothing=1e-6, curve=1, mac=mac, precision=0, bh_accept=vote)
     g = run_gravity(gpu_lib, s, p, potential=True)
     box = orc.bbox(s["x"], s["y"], s["z"])
     t = orc.Tree(s["x"], s["y"], s["z"], box)
     t.com(s["mass"])
-    o = orc.bh_group(t, g["sorted"], box, 0.5, 1e-6, mac=mac, group=gpu_lib.BH_GROUP, potential=True)
+    o = orc.bh_group(t, g["sorted"], box, 0.5, 1e-6, mac=mac, group=gpu_lib.BH_GROUP, potential=True,
+                     vote=bool(vote))
     assert g["stats"]["n_cell_interactions"] == o["n_cell"]
     assert g["stats"]["n_part_interactions"] == o["n_part"]
     err = vec_rel_err(g["g_ax"], g["g_ay"], g["g_az"], o["ax"], o["ay"], o["az"])
@@ -243,7 +244,8 @@ def test_barnes_hut_fp32_quadrupole_parity(gpu_lib, orc, plummer64k):
     box = orc.bbox(s["x"], s["y"], s["z"])
     t = orc.Tree(s["x"], s["y"], s["z"], box)
     t.com(s["mass"])
-    o = orc.bh_group(t, g["sorted"], box, 0.6, 4e-8, mac=0, group=gpu_lib.BH_GROUP, quad=True, potential=True)
+    o = orc.bh_group(t, g["sorted"], box, 0.6, 4e-8, mac=0, group=gpu_lib.BH_GROUP, quad=True, potential=True,
+                     vote=p.bh_accept == gpu_lib.BH_VOTE)
     assert g["stats"]["n_cell_interactions"] == o["n_cell"]
     assert g["stats"]["n_part_interactions"] == o["n_part"]
     err = vec_rel_err(g["g_ax"], g["g_ay"], g["g_az"], o["ax"], o["ay"], o["az"])
@@ -300,3 +302,35 @@ def test_leapfrog_matches_oracle(gpu_lib, orc):
         orc.leapfrog_update_v(dt, st)
     for k in ("x", "y", "z", "vx", "vy", "vz"):
         assert np.max(np.abs(g[k] - st[k])) < 1e-12 * max(1.0, np.abs(st[k]).max())
+
+
+def test_barnes_hut_per_particle_v1_is_the_reference_walk(gpu_lib, orc):
+    """bh_accept = PER_PARTICLE runs computeForces_v1 itself (one depth-first walk per
+    particle, classic MAC, the particle itself skipped): accelerations and potential equal the
+    oracle's restatement orc_bh_v1, which walks in the same order with the same expressions."""
+    print_case("bh per-particle v1, plummer 20000, fp64")
+    s = ics.plummer(20000, seed=23)
+    p = gpu_lib.default_params(theta=0.5, smoothing=1e-6, curve=1, mac=0, precision=0,
+                               bh_accept=gpu_lib.BH_PER_PARTICLE)
+    g = run_gravity(gpu_lib, s, p, potential=True)
+    box = orc.bbox(s["x"], s["y"], s["z"])
+    t = orc.Tree(s["x"], s["y"], s["z"], box)
+    t.com(s["mass"])
+    o = orc.bh_v1(t, 0.5 * (box[1] - box[0]), 0.5, 1e-6, potential=True)
+    assert o["status"] == 0
+    err = vec_rel_err(g["g_ax"], g["g_ay"], g["g_az"], o["ax"], o["ay"], o["az"])
+    exact = np.mean((g["g_ax"] == o["ax"]) & (g["g_ay"] == o["ay"]) & (g["g_az"] == o["az"]))
+    print(f"v1: max rel err {err.max():.2e}, bit-identical fraction {exact:.4f}")
+    assert err.max() < 1e-14
+    assert np.max(np.abs(g["u"] - o["u"]) / np.abs(o["u"])) < 1e-14
+    # v1 is the G = 1 limit of the group walks: it needs the fewest interactions
+    vote = run_gravity(gpu_lib, s, gpu_lib.default_params(theta=0.5, smoothing=1e-6, curve=1, mac=0, precision=0))
+    tot = lambda st: st["n_cell_interactions"] + st["n_part_interactions"]
+    assert tot(g["stats"]) < tot(vote["stats"])
+
+
+def test_per_particle_walk_rejects_other_forms(gpu_lib):
+    for kw in ({"mac": 1}, {"quadrupole": 1}):
+        with pytest.raises(gpu_lib.MhpcError) as e:
+            gpu_lib.Context(1000, gpu_lib.default_params(bh_accept=gpu_lib.BH_PER_PARTICLE, **kw))
+        assert e.value.status == gpu_lib.E_ARG
