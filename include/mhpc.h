@@ -38,7 +38,8 @@ typedef enum {
     MHPC_OK = 0,
     MHPC_E_CAP_NODES = 1,        /* more cells than the node capacity */
     MHPC_E_CAP_NEIGHBOURS = 2,   /* a particle has >= max_nb neighbours, or 0 after h-iteration */
-    MHPC_E_DEPTH = 3,            /* two particles share a 21-level key (duplicate positions) */
+    MHPC_E_DEPTH = 3,            /* two particles share the 21-digit key and the octant of the next
+                                    bisection (equal positions; tree.cu:949-963) */
     MHPC_E_RHO_NONPOSITIVE = 4,  /* density <= 0 (density.cu:68-70) */
     MHPC_E_STACK = 5,            /* traversal stack overflow */
     MHPC_E_ARG = 6,              /* bad argument */

@@ -52,7 +52,7 @@ extern "C" const char *mhpc_status_string(int s) {
         case MHPC_OK: return "ok";
         case MHPC_E_CAP_NODES: return "node capacity exceeded";
         case MHPC_E_CAP_NEIGHBOURS: return "neighbour capacity exceeded (or no neighbours after h-iteration)";
-        case MHPC_E_DEPTH: return "duplicate 21-level keys (tree depth)";
+        case MHPC_E_DEPTH: return "particles do not separate within 22 bisections (tree depth)";
         case MHPC_E_RHO_NONPOSITIVE: return "non-positive density";
         case MHPC_E_STACK: return "traversal stack overflow";
         case MHPC_E_ARG: return "bad argument";

@@ -502,19 +502,20 @@ __global__ __launch_bounds__(BH1_THREADS) void k_bh_v1(BHArgs A) {
     const int t = threadIdx.x;
     const int64_t k = blockIdx.x * (int64_t)BH1_THREADS + t;
     const int64_t n = A.n;
-    if (k >= n) return;
-    const double4 own = A.sxm[k];
+    // lanes past the last particle walk nothing but take part in the wave's counter sums
+    const bool valid = k < n;
+    const double4 own = valid ? A.sxm[k] : make_double4(0.0, 0.0, 0.0, 0.0);
     const double px = own.x, py = own.y, pz = own.z;
     const double R = 0.5 * (A.box[1] - A.box[0]);
     const double cs0 = 4.0 * R * R;
     const double thetasq = A.theta * A.theta;
     double axi = 0.0, ayi = 0.0, azi = 0.0, ui = 0.0;
     long long n_cells = 0, n_parts = 0;
-    int depth = 1;
+    int depth = valid ? 1 : 0;
     cur_node[1][t] = (int)n;   // root
     cur_child[1][t] = 0;
     bool overflow = false;
-    do {
+    while (depth > 0) {
         int child_number = cur_child[depth][t];
         int node = cur_node[depth][t];
         // cellSize[depth] of the reference's table: 4 R^2 / 4^depth, exact powers of 4
@@ -564,13 +565,15 @@ __global__ __launch_bounds__(BH1_THREADS) void k_bh_v1(BHArgs A) {
             }
         }
         depth--;
-    } while (depth > 0);
+    }
     if (overflow) set_err(A.err, ERRF_STACK);
-    const int32_t i = A.perm[k];
-    A.gax[i] = A.G * axi;
-    A.gay[i] = A.G * ayi;
-    A.gaz[i] = A.G * azi;
-    if (POT) A.u[i] = A.G * ui;
+    if (valid) {
+        const int32_t i = A.perm[k];
+        A.gax[i] = A.G * axi;
+        A.gay[i] = A.G * ayi;
+        A.gaz[i] = A.G * azi;
+        if (POT) A.u[i] = A.G * ui;
+    }
     const unsigned long long sc = (unsigned long long)wave_sum(n_cells);
     const unsigned long long sp = (unsigned long long)wave_sum(n_parts);
     if (wave_lane() == 0) {

@@ -48,7 +48,6 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_count(const uint64_t *__res
             const int a = (k > 0) ? lcp_levels(skey[k - 1], kk) : 0;
             const int b = (k + 1 < n) ? lcp_levels(kk, skey[k + 1]) : 0;
             lcp[k] = (uint8_t)b;
-            if (b >= MHPC_LEVELS) set_err(err, ERRF_DEPTH);
             for (int L = a + 1; L <= b; L++) atomicAdd(&cnt[L], 1u);
         }
     }
@@ -157,12 +156,34 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_cells(
     }
 }
 
+// The octant of (x, y, z) at bisection MHPC_LEVELS + 1 of the root box: the keys' bisection
+// (lower half = bit) carried one level past the key's 21 digits.
+__device__ __forceinline__ int octant_past_key(double x, double y, double z, const double *__restrict__ box) {
+    double mnx = box[0], mxx = box[1], mny = box[2], mxy = box[3], mnz = box[4], mxz = box[5];
+    int son = 0;
+    for (int level = 0; level <= MHPC_LEVELS; level++) {
+        son = 0;
+        double m = 0.5 * (mnx + mxx);
+        if (x < m) { son += 1; mxx = m; } else { mnx = m; }
+        m = 0.5 * (mny + mxy);
+        if (y < m) { son += 2; mxy = m; } else { mny = m; }
+        m = 0.5 * (mnz + mxz);
+        if (z < m) { son += 4; mxz = m; } else { mnz = m; }
+    }
+    return son;
+}
+
+// Particles whose 21-digit keys are equal share a level-21 cell.  The reference's insertion
+// keeps bisecting such a pair geometrically (leaves may sit one level below the keys, the
+// depth limit is MAX_LEVEL + 1, tree.cu:949-963), so these particles take the child slot of
+// their octant at the next bisection; two of them in the same slot would need a level-22 cell,
+// which this tree does not build: that is the depth error (MHPC_E_DEPTH).
 __global__ __launch_bounds__(TREE_BLOCK) void k_tree_link(
     const uint8_t *__restrict__ lcp, const uint64_t *__restrict__ slkey,
     const int32_t *__restrict__ perm, int64_t n, const int32_t *__restrict__ cell_base,
     const int32_t *__restrict__ cstart, const int32_t *__restrict__ clevel, const int32_t *__restrict__ ppar,
     const int32_t *__restrict__ cpar, int32_t *__restrict__ cchild, int32_t *__restrict__ plevel,
-    int64_t cap_cells) {
+    int64_t cap_cells, const double4 *__restrict__ sxm, const double *__restrict__ box, uint32_t *err) {
     __shared__ int32_t cb[MHPC_LEVELS + 3];
     if (threadIdx.x < MHPC_LEVELS + 3) cb[threadIdx.x] = cell_base[threadIdx.x];
     __syncthreads();
@@ -178,9 +199,14 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_link(
         const int b = (int)lcp[k];
         const int Lp = max(a, b) + 1;
         plevel[perm[k]] = Lp;
-        if (Lp > MHPC_LEVELS) continue;   // duplicate key, flagged by k_tree_count
         const int32_t parent = ppar[k];
         if (parent >= cap_cells) continue;
+        if (Lp > MHPC_LEVELS) {
+            const double4 r = sxm[k];
+            const int oct = octant_past_key(r.x, r.y, r.z, box);
+            if (atomicCAS(&cchild[8 * (int64_t)parent + oct], -1, k) != -1) set_err(err, ERRF_DEPTH);
+            continue;
+        }
         const int digit = (int)((slkey[k] >> (3 * (MHPC_LEVELS - Lp))) & 7u);
         cchild[8 * (int64_t)parent + digit] = k;
     } else {
@@ -211,7 +237,7 @@ void launch_tree(mhpc_ctx *c) {
     const int64_t link_blocks = std::min<int64_t>(ceil_div(items, TREE_BLOCK), 8192);
     hipLaunchKernelGGL(k_tree_link, dim3((unsigned)link_blocks), dim3(TREE_BLOCK), 0,
                        c->stream, c->lcp, slkey, c->perm, n, c->cell_base, c->cstart, c->clevel, c->ppar,
-                       c->cpar, c->cchild, c->level, c->cap_cells);
+                       c->cpar, c->cchild, c->level, c->cap_cells, c->sxm, c->box, c->err);
 }
 
 // ------------------------------------------------------------------------------------

@@ -36,7 +36,7 @@ def test_default_params_and_status_strings():
     assert p.theta == 0.5 and p.max_nb == 180 and p.n_target == 64
     assert (p.band_lo, p.band_hi, p.max_counts) == (50, 80, 7)
     assert lib.LIB.mhpc_status_string(0) == b"ok"
-    assert b"duplicate" in lib.LIB.mhpc_status_string(lib.E_DEPTH)
+    assert b"tree depth" in lib.LIB.mhpc_status_string(lib.E_DEPTH)
 
 
 def test_struct_layout_matches_header():

@@ -4,8 +4,9 @@
   pass 0) against the FP64 oracle's full SPH RHS -- h, iteration counts, neighbour counts and
   neighbour sets bit-exact, density / pressure to 1e-12, derivatives to 1e-11;
 * Boss-Bodenheimer 1,048,576 (self-gravity + SPH, FP32 + Kahan pair sums): against the FP64
-  oracle -- h and neighbour sets bit-exact (the h-iteration is FP64), density to 1e-5 per
-  particle, the other quantities' measured errors printed and bounded;
+  oracle -- h and neighbour sets bit-exact (the h-iteration is FP64), density, pressure,
+  accelerations and the time derivatives within 1e-5 per particle (the derivatives that rigid
+  rotation cancels against the sum of their terms' magnitudes), every measured error printed;
 * Plummer 16,777,216 (theta 0.6, quadrupole, FP32 + Kahan): accelerations against FP64
   direct summation on a seeded subsample of 4,096 particles (99th percentile), and
   momentum / energy bookkeeping over a leapfrog step.
@@ -34,11 +35,54 @@ def sorted_rows(nnl, noi):
     return a
 
 
-def deriv_err(g, o):
-    """per-particle error relative to max(|value|, RMS of the field): sums of signed pair terms
-    can cancel to ~0 for one particle"""
-    scale = np.sqrt(np.mean(o ** 2))
-    return np.abs(g - o) / np.maximum(np.abs(o), scale)
+def deriv_err(g, o, scale=None):
+    """per-particle error relative to max(|value|, scale); scale defaults to the RMS of the field
+    (sums of signed pair terms can cancel to ~0 for one particle); absolute where both are 0"""
+    if scale is None:
+        scale = np.sqrt(np.mean(o ** 2))
+    den = np.maximum(np.abs(o), scale)
+    return np.abs(g - o) / np.where(den == 0.0, 1.0, den)
+
+
+def abs_pair_sums(s, o, alpha, beta, chunk=1 << 17):
+    """Per particle, the error scales of d rho/dt, de/dt and dh/dt (cubic spline, kernel length
+    (h_i + h_j)/2):
+      rho_i sum (m_j/rho_j)|v_ij| |gradW|,
+      sum 1/2 m_j (p_i/rho_i^2 + p_j/rho_j^2 + |Pi_ij|)|v_ij| |gradW|,
+      1/3 h_i sum (m_j/rho_j)|v_ij| |gradW|.
+    A sum of dot products a_k . b_k computed in a precision eps is accurate to about
+    eps * sum |a_k| |b_k|, not to eps * |sum|: when the products cancel (rigid rotation has
+    v_ij . x_ij = 0 for every pair, so these three derivatives vanish up to rounding, term by
+    term), that sum is the error scale."""
+    x, y, z, vx, vy, vz, m = (s[k] for k in ("x", "y", "z", "vx", "vy", "vz", "mass"))
+    h, rho, p, cs, nnl, noi = o["sml"], o["rho"], o["p"], o["cs"], o["nnl"], o["noi"]
+    n = len(x)
+    out = {k: np.zeros(n) for k in ("drhodt", "dedt", "dsmldt")}
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        cnt = noi[a:b]
+        i = np.repeat(np.arange(a, b), cnt)
+        j = nnl[a:b][np.arange(nnl.shape[1])[None, :] < cnt[:, None]]
+        dx, dy, dz = x[i] - x[j], y[i] - y[j], z[i] - z[j]
+        r = np.sqrt(dx * dx + dy * dy + dz * dz)
+        hb = 0.5 * (h[i] + h[j])
+        q = r / hb
+        f = 8.0 / (np.pi * hb ** 3)
+        dwdr = np.where(q <= 0.5, 6.0 * f / hb * (3.0 * q * q - 2.0 * q),
+                        np.where(q <= 1.0, -6.0 * f / hb * (1.0 - q) ** 2, 0.0))
+        dvx, dvy, dvz = vx[i] - vx[j], vy[i] - vy[j], vz[i] - vz[j]
+        vdotgw = np.sqrt(dvx * dvx + dvy * dvy + dvz * dvz) * np.abs(dwdr)   # |v_ij| |grad W|
+        t = m[j] / rho[j] * vdotgw
+        s1 = np.bincount(i - a, weights=t, minlength=b - a)
+        out["drhodt"][a:b] = rho[a:b] * s1
+        out["dsmldt"][a:b] = h[a:b] * s1 / 3.0
+        # the artificial viscosity's term, on approaching pairs (mu, Pi as the force kernel)
+        vr = dvx * dx + dvy * dy + dvz * dz
+        mu = hb * vr / (r * r + 0.01 * hb * hb)
+        pij = np.where(vr < 0, (beta * mu - alpha * 0.5 * (cs[i] + cs[j])) * mu / (0.5 * (rho[i] + rho[j])), 0.0)
+        te = 0.5 * m[j] * (p[i] / rho[i] ** 2 + p[j] / rho[j] ** 2 + np.abs(pij)) * vdotgw
+        out["dedt"][a:b] = np.bincount(i - a, weights=te, minlength=b - a)
+    return out
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -49,11 +93,21 @@ def oracle_threads(orc):
 def test_sedov128_benchmarked_rhs_equals_oracle(gpu_lib, orc):
     print_case("full size: sedov 128^3 speculative rhs vs oracle")
     s = ics.sedov_lattice(128)
+    # a seeded velocity field (radial outflow plus noise), so that the velocity-dependent
+    # derivatives and the artificial viscosity are not zero as in the blast's initial state
+    rng = np.random.default_rng(128)
+    for k, c in (("vx", "x"), ("vy", "y"), ("vz", "z")):
+        s[k] = 0.5 * s[c] + 0.05 * rng.standard_normal(len(s[c]))
     p = ics.params_sedov()
     flags = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML | gpu_lib.RHS_SPECULATIVE_NN
     with make_ctx(gpu_lib, s, p) as ctx:
         st = ctx.rhs(flags)
-        st = ctx.rhs(flags)   # the second RHS of a context: depth-hinted sort, as in the benchmark
+        # the second RHS of a context (depth-hinted sort, as in the benchmark) from the same
+        # initial state: the first one replaced h, p and cs
+        for k, v in s.items():
+            if k in gpu_lib.FIELD_ID:
+                ctx.upload(k, v)
+        st = ctx.rhs(flags)
         g = {k: ctx.download(k) for k in ("sml", "h_iters", "noi", "rho", "p", "cs", "ax", "ay", "az",
                                           "drhodt", "dedt", "dsmldt")}
         nnl, noi = ctx.neighbour_lists()
@@ -66,7 +120,8 @@ def test_sedov128_benchmarked_rhs_equals_oracle(gpu_lib, orc):
     assert np.array_equal(sorted_rows(nnl, noi), sorted_rows(o["nnl"], o["noi"]))
     del nnl
     for k in ("rho", "p", "cs"):
-        e = np.abs(g[k] - o[k]) / np.abs(o[k])
+        # cs is 0 everywhere in the first RHS (the sound speed comes from the previous p = 0)
+        e = np.abs(g[k] - o[k]) / np.where(o[k] == 0.0, 1.0, np.abs(o[k]))
         print(f"{k}: max rel err {e.max():.2e}")
         assert e.max() < 1e-12, k
     for k in ("ax", "ay", "az", "drhodt", "dedt", "dsmldt"):
@@ -93,8 +148,13 @@ def test_boss_bodenheimer_1m_mixed_precision_vs_fp64(gpu_lib, orc):
     errs = {}
     for k in ("rho", "p"):
         errs[k] = np.abs(g[k] - o[k]) / np.abs(o[k])
-    for k in ("ax", "ay", "az", "drhodt", "dedt", "dsmldt"):
+    for k in ("ax", "ay", "az"):
         errs[k] = deriv_err(g[k], o[k])
+    # rigid rotation: v_ij . x_ij = 0, so these derivatives are sums of terms that cancel to
+    # rounding level; their error is measured against the sum of the terms' magnitudes
+    scales = abs_pair_sums(s, o, p.alpha, p.beta)
+    for k in ("drhodt", "dedt", "dsmldt"):
+        errs[k] = deriv_err(g[k], o[k], scales[k])
     # gravity against the FP64 oracle of the same walk
     box = orc.bbox(s["x"], s["y"], s["z"])
     t = orc.Tree(s["x"], s["y"], s["z"], box)
@@ -107,8 +167,7 @@ def test_boss_bodenheimer_1m_mixed_precision_vs_fp64(gpu_lib, orc):
     assert errs["rho"].max() < 1e-5 and errs["p"].max() < 1e-5
     assert np.percentile(errs["g"], 99) < 1e-5
     for k in ("ax", "ay", "az", "drhodt", "dedt", "dsmldt"):
-        assert np.percentile(errs[k], 99) < 1e-5, k
-        assert errs[k].max() < 1e-4, k
+        assert errs[k].max() < 1e-5, k
 
 
 def test_plummer16m_vs_direct_summation(gpu_lib, orc):

@@ -165,11 +165,56 @@ def canonical_oracle(t, lkeys):
 def test_tree_and_com_bitexact(gpu_lib, orc, plummer64k, case):
     print_case(f"tree+com {case}")
     if case == "plummer":
-        s = plummer64k
-        curve = 1
+        check_tree_against_oracle(gpu_lib, orc, plummer64k, 1)
     else:
-        s = ics.sedov_lattice(32)
-        curve = 0
+        check_tree_against_oracle(gpu_lib, orc, ics.sedov_lattice(32), 0)
+
+
+def cell_at_level(x, y, z, box, levels):
+    """The box of the cell containing (x, y, z) after `levels` bisections of the root box (the
+    keys' expressions, in FP64)."""
+    mn, mx = [box[0], box[2], box[4]], [box[1], box[3], box[5]]
+    for _ in range(levels):
+        for a, v in enumerate((x, y, z)):
+            m = 0.5 * (mn[a] + mx[a])
+            if v < m:
+                mx[a] = m
+            else:
+                mn[a] = m
+    return mn, mx
+
+
+def test_tree_pair_sharing_a_key(gpu_lib, orc, plummer64k):
+    """Two particles with equal 21-digit keys that separate at the next bisection: the
+    reference's insertion puts them below a level-21 cell (leaf level 22); tree, centres of
+    mass and the Barnes-Hut walk match the oracle."""
+    print_case("tree+com+bh plummer 65536 with a pair sharing a 21-digit key")
+    s = {k: v.copy() for k, v in plummer64k.items()}
+    r2 = s["x"] ** 2 + s["y"] ** 2 + s["z"] ** 2
+    a, b = (int(i) for i in np.argsort(r2)[:2])   # two core particles (neither sets the box)
+    box = orc.bbox(s["x"], s["y"], s["z"])
+    mn, mx = cell_at_level(s["x"][b], s["y"][b], s["z"][b], box, 21)
+    # particle a: the sub-octant of b's level-21 cell opposite to b's on every axis
+    for ax, k in enumerate(("x", "y", "z")):
+        mid = 0.5 * (mn[ax] + mx[ax])
+        q = 0.25 * (mx[ax] - mn[ax])
+        s[k][a] = mid + q if s[k][b] < mid else mid - q
+    box = orc.bbox(s["x"], s["y"], s["z"])
+    lk = orc.keys(s["x"], s["y"], s["z"], box, 0)
+    assert lk[a] == lk[b]
+    check_tree_against_oracle(gpu_lib, orc, s, 1, expect_level22=(a, b))
+    p = gpu_lib.default_params(theta=0.5, smoothing=1e-6, curve=1, mac=0, precision=0)
+    g = run_gravity(gpu_lib, s, p, potential=True)
+    t = orc.Tree(s["x"], s["y"], s["z"], box)
+    t.com(s["mass"])
+    o = orc.bh_group(t, g["sorted"], box, 0.5, 1e-6, mac=0, group=gpu_lib.BH_GROUP, potential=True,
+                     vote=p.bh_accept == gpu_lib.BH_VOTE)
+    assert g["stats"]["n_cell_interactions"] == o["n_cell"]
+    assert g["stats"]["n_part_interactions"] == o["n_part"]
+    assert np.max(vec_rel_err(g["g_ax"], g["g_ay"], g["g_az"], o["ax"], o["ay"], o["az"])) < 1e-12
+
+
+def check_tree_against_oracle(gpu_lib, orc, s, curve, expect_level22=()):
     n = len(s["x"])
     p = gpu_lib.default_params(curve=curve, quadrupole=1)
     with make_ctx(gpu_lib, s, p) as ctx:
@@ -188,6 +233,8 @@ def test_tree_and_com_bitexact(gpu_lib, orc, plummer64k, case):
     assert set(og) == set(gg)
     assert len(tree["level"]) == t.ncells
     assert np.array_equal(plevel, t.level[:n])
+    for i in expect_level22:
+        assert plevel[i] == 22
     # children, canonically: particle children by original index, cells by (L, prefix)
     ochild = t.child.reshape(-1, 8)
     inv = {v: k for k, v in gg.items()}
@@ -261,6 +308,8 @@ def test_barnes_hut_fp32_quadrupole_parity(gpu_lib, orc, plummer64k):
 
 
 def test_tree_errors(gpu_lib):
+    """Equal positions never separate (the reference asserts at depth MAX_LEVEL + 2,
+    tree.cu:949-963); too small a cell capacity."""
     print_case("tree errors: duplicate positions, node capacity")
     s = ics.plummer(5000, seed=9)
     s["x"][10], s["y"][10], s["z"][10] = s["x"][20], s["y"][20], s["z"][20]
