@@ -44,7 +44,8 @@ typedef enum {
     MHPC_E_STACK = 5,            /* traversal stack overflow */
     MHPC_E_ARG = 6,              /* bad argument */
     MHPC_E_HIP = 7,              /* HIP runtime error */
-    MHPC_E_STATE = 8             /* call order (e.g. forces before density) */
+    MHPC_E_STATE = 8,            /* call order (e.g. forces before density) */
+    MHPC_E_SCRATCH = 9           /* more elements than a no-allocation entry point's scratch holds */
 } mhpc_status;
 
 /* Space-filling curves (reference Curve::Type, subdomain.cuh). */
@@ -254,6 +255,20 @@ typedef struct {
     double com[3];                 /* centre of mass */
 } mhpc_diagnostics;
 int mhpc_compute_diagnostics(mhpc_ctx *ctx, mhpc_diagnostics *out);
+
+/* -------- key sort without allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  mhpc_sort's result (a stable sort by key, ties in ascending
+ * position: cub::DeviceRadixSort::SortPairs over keys and 0..n-1, helper.cu:150-167) on
+ * caller-owned device arrays: keys[n] in, sorted_keys[n] and sorted[n] (the permutation) out,
+ * queued on `stream` (a hipStream_t; NULL = the null stream) without synchronising.  It calls
+ * no allocator: its working memory is one static array per GPU of about 26 MiB, held until
+ * the process ends, so n is at most MHPC_NOALLOC_SORT_MAX (else MHPC_E_SCRATCH, nothing
+ * queued).  Every call on one GPU uses that GPU's array: calls that overlap in time on one GPU
+ * (different streams, host threads) race and must be serialised by the caller. */
+#define MHPC_NOALLOC_SORT_MAX 2097152
+int mhpc_sort_keys_noalloc(const uint64_t *keys, int64_t n, uint64_t *sorted_keys, int32_t *sorted,
+                           void *stream);
 
 /* -------- predictor-corrector Euler (PredictorCorrectorEulerNS, predictor_corrector_euler.cpp,
  * device_predictor_corrector_euler.cu:130-540) -------- */

@@ -58,6 +58,7 @@ extern "C" const char *mhpc_status_string(int s) {
         case MHPC_E_ARG: return "bad argument";
         case MHPC_E_HIP: return "HIP runtime error";
         case MHPC_E_STATE: return "call order";
+        case MHPC_E_SCRATCH: return "more elements than the no-allocation scratch holds";
         default: return "unknown";
     }
 }

@@ -23,10 +23,12 @@ static int grid_for(int64_t n, int block, int max_blocks) {
     return (int)g;
 }
 
-void launch_fill_u32(mhpc_ctx *c, uint32_t *p, uint32_t v, int64_t n) {
+void fill_u32_on(hipStream_t stream, uint32_t *p, uint32_t v, int64_t n) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->stream, p, v, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, stream, p, v, n);
 }
+
+void launch_fill_u32(mhpc_ctx *c, uint32_t *p, uint32_t v, int64_t n) { fill_u32_on(c->stream, p, v, n); }
 
 void launch_fill_u64(mhpc_ctx *c, unsigned long long *p, unsigned long long v, int64_t n) {
     if (n <= 0) return;

@@ -10,6 +10,8 @@
 //              = lanes holding the same digit), the block stages the tile in LDS in
 //              digit order and writes each digit run contiguously.
 // HBM bytes per pass: count reads 8 B/key, scatter reads 12 and writes 12.
+#include <mutex>
+
 #include "mhpc_internal.h"
 
 #define SORT_BLOCK 256
@@ -158,7 +160,20 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_scatter(const uint64_t *__r
     }
 }
 
-static void sort_passes(mhpc_ctx *c, int first, int64_t ntiles);
+// the buffers of the LSD passes: input keys (values = positions), the even passes' outputs,
+// the odd passes' outputs (the last pass's, so the result) and the per-pass tables
+struct SortBufs {
+    const uint64_t *kin;
+    uint64_t *keyB;
+    int32_t *valB;
+    uint64_t *kout;
+    int32_t *vout;
+    uint32_t *tile_hist, *tile_off, *digit_hist;
+};
+static void sort_passes(hipStream_t stream, int64_t n, int first, int64_t ntiles, const SortBufs &b);
+static SortBufs ctx_bufs(mhpc_ctx *c) {
+    return SortBufs{c->key, c->keyB, c->valB, c->skey, c->perm, c->tile_hist, c->tile_off, c->digit_hist};
+}
 
 // Adjacent sorted keys equal in the bits the passes sorted (>= shift): their order in the low
 // bits is not established.  Sets *flag.
@@ -185,7 +200,7 @@ void launch_sort(mhpc_ctx *c) {
     hipLaunchKernelGGL(k_sort_hist_all, dim3(hb), dim3(SORT_BLOCK), 0, c->stream, c->key, n,
                        c->digit_hist, c->sort_check, first, MHPC_SORT_PASSES);
     if (first > 0) {
-        sort_passes(c, first, ntiles);
+        sort_passes(c->stream, n, first, ntiles, ctx_bufs(c));
         hipLaunchKernelGGL(k_sort_check, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, c->skey, n,
                            MHPC_SORT_BITS * first, c->sort_check);
         if (hipMemcpyAsync(c->h_sort_check, c->sort_check, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream) ==
@@ -196,26 +211,94 @@ void launch_sort(mhpc_ctx *c) {
         hipLaunchKernelGGL(k_sort_hist_all, dim3(hb), dim3(SORT_BLOCK), 0, c->stream, c->key, n,
                            c->digit_hist, c->sort_check, 0, first);
     }
-    sort_passes(c, 0, ntiles);
+    sort_passes(c->stream, n, 0, ntiles, ctx_bufs(c));
 }
 
-// LSD passes first .. MHPC_SORT_PASSES - 1 from the input keys; the last pass writes skey / perm
-static void sort_passes(mhpc_ctx *c, int first, int64_t ntiles) {
-    const int64_t n = c->n;
-    const uint64_t *kin = c->key;
+// LSD passes first .. MHPC_SORT_PASSES - 1 from the input keys; the last (odd) pass writes
+// kout / vout
+static void sort_passes(hipStream_t stream, int64_t n, int first, int64_t ntiles, const SortBufs &b) {
+    const uint64_t *kin = b.kin;
     const int32_t *vin = nullptr;
     for (int pass = first; pass < MHPC_SORT_PASSES; pass++) {
-        uint64_t *kout = (pass & 1) ? c->skey : c->keyB;
-        int32_t *vout = (pass & 1) ? c->perm : c->valB;
+        uint64_t *kout = (pass & 1) ? b.kout : b.keyB;
+        int32_t *vout = (pass & 1) ? b.vout : b.valB;
         const int shift = MHPC_SORT_BITS * pass;
-        hipLaunchKernelGGL(k_sort_count, dim3((unsigned)ntiles), dim3(SORT_BLOCK), 0, c->stream, kin, n,
-                           shift, c->tile_hist, ntiles);
-        hipLaunchKernelGGL(k_sort_scan, dim3(MHPC_SORT_BUCKETS), dim3(SORT_BLOCK), 0, c->stream,
-                           c->tile_hist, c->tile_off, ntiles,
-                           c->digit_hist + pass * MHPC_SORT_BUCKETS);
-        hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)ntiles), dim3(SORT_BLOCK), 0, c->stream, kin,
-                           vin, kout, vout, n, shift, c->tile_off, ntiles);
+        hipLaunchKernelGGL(k_sort_count, dim3((unsigned)ntiles), dim3(SORT_BLOCK), 0, stream, kin, n, shift,
+                           b.tile_hist, ntiles);
+        hipLaunchKernelGGL(k_sort_scan, dim3(MHPC_SORT_BUCKETS), dim3(SORT_BLOCK), 0, stream, b.tile_hist,
+                           b.tile_off, ntiles, b.digit_hist + pass * MHPC_SORT_BUCKETS);
+        hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)ntiles), dim3(SORT_BLOCK), 0, stream, kin, vin, kout,
+                           vout, n, shift, b.tile_off, ntiles);
         kin = kout;
         vin = vout;
     }
+}
+
+// ------------------------------------------------------------------------------------
+// The key sort without allocation (mhpc_sort_keys_noalloc): the same passes on caller-owned
+// device arrays, with every buffer beyond the inputs and outputs carved from `scratch`, one
+// static array per GPU that lives until the process ends.  Layout (byte offsets, each
+// 256-aligned): even passes' keys and values, per-tile histograms and offsets, the all-pass
+// digit histogram, the check word k_sort_hist_all clears.
+#define NA_TILES (MHPC_NOALLOC_SORT_MAX / MHPC_SORT_TILE)
+#define NA_KEYB 0
+#define NA_VALB (NA_KEYB + 8 * (size_t)MHPC_NOALLOC_SORT_MAX)
+#define NA_THIST (NA_VALB + 4 * (size_t)MHPC_NOALLOC_SORT_MAX)
+#define NA_TOFF (NA_THIST + 4 * (size_t)MHPC_SORT_BUCKETS * NA_TILES)
+#define NA_DHIST (NA_TOFF + 4 * (size_t)MHPC_SORT_BUCKETS * NA_TILES)
+#define NA_CHECK (NA_DHIST + 4 * (size_t)MHPC_SORT_PASSES * MHPC_SORT_BUCKETS)
+#define NA_BYTES (NA_CHECK + 256)
+static_assert(MHPC_NOALLOC_SORT_MAX % MHPC_SORT_TILE == 0, "whole tiles");
+static_assert(NA_VALB % 256 == 0 && NA_THIST % 256 == 0 && NA_TOFF % 256 == 0 && NA_DHIST % 256 == 0 &&
+                  NA_CHECK % 256 == 0, "256-aligned parts");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+#define NA_MAX_DEVICES 64
+static std::mutex na_mutex;
+static unsigned char *na_ptr[NA_MAX_DEVICES];   // device address of `scratch` on each GPU
+
+// `scratch`'s address on the current device, looked up once per device (the host-side name is
+// not a device address)
+static int na_scratch(unsigned char **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEVICES) return MHPC_E_HIP;
+    std::lock_guard<std::mutex> lock(na_mutex);
+    if (!na_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MHPC_E_HIP;
+        na_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = na_ptr[dev];
+    return MHPC_OK;
+}
+
+extern "C" int mhpc_sort_keys_noalloc(const uint64_t *keys, int64_t n, uint64_t *sorted_keys, int32_t *sorted,
+                                      void *stream) {
+    if (n < 0 || (n > 0 && (!keys || !sorted_keys || !sorted))) return MHPC_E_ARG;
+    if (n > MHPC_NOALLOC_SORT_MAX) return MHPC_E_SCRATCH;
+    if (n == 0) return MHPC_OK;
+    unsigned char *ptr = nullptr;
+    const int st = na_scratch(&ptr);
+    if (st != MHPC_OK) return st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint32_t *digit_hist = reinterpret_cast<uint32_t *>(ptr + NA_DHIST);
+    uint32_t *check = reinterpret_cast<uint32_t *>(ptr + NA_CHECK);
+    // the digit histogram accumulates with atomics: cleared first (the previous call's counts
+    // are still there)
+    fill_u32_on(s, digit_hist, 0u, MHPC_SORT_PASSES * MHPC_SORT_BUCKETS);
+    const int64_t ntiles = ceil_div(n, MHPC_SORT_TILE);
+    const int hb = (int)(ceil_div(n, SORT_BLOCK * 16) < 1024 ? ceil_div(n, SORT_BLOCK * 16) : 1024);
+    hipLaunchKernelGGL(k_sort_hist_all, dim3(hb), dim3(SORT_BLOCK), 0, s, keys, n, digit_hist, check, 0,
+                       MHPC_SORT_PASSES);
+    const SortBufs b{keys,
+                     reinterpret_cast<uint64_t *>(ptr + NA_KEYB),
+                     reinterpret_cast<int32_t *>(ptr + NA_VALB),
+                     sorted_keys,
+                     sorted,
+                     reinterpret_cast<uint32_t *>(ptr + NA_THIST),
+                     reinterpret_cast<uint32_t *>(ptr + NA_TOFF),
+                     digit_hist};
+    sort_passes(s, n, 0, ntiles, b);
+    return hipGetLastError() == hipSuccess ? MHPC_OK : MHPC_E_HIP;
 }

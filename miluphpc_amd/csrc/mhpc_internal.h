@@ -165,6 +165,7 @@ struct mhpc_ctx {
 // ------------------------------------------------------------------------------------
 // host-side launchers (defined in the k_*.hip files)
 void launch_fill_u32(mhpc_ctx *c, uint32_t *p, uint32_t v, int64_t n);
+void fill_u32_on(hipStream_t stream, uint32_t *p, uint32_t v, int64_t n);
 void launch_fill_u64(mhpc_ctx *c, unsigned long long *p, unsigned long long v, int64_t n);
 void launch_bbox(mhpc_ctx *c);
 void launch_keys(mhpc_ctx *c);

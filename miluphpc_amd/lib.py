@@ -23,6 +23,7 @@ E_STACK = 5
 E_ARG = 6
 E_HIP = 7
 E_STATE = 8
+E_SCRATCH = 9
 
 CURVE_LEBESGUE, CURVE_HILBERT = 0, 1
 EOS_POLYTROPIC, EOS_ISOTHERMAL, EOS_IDEAL_GAS = 0, 3, 9
@@ -42,6 +43,7 @@ RHS_FEATURES = 1 << 7
 
 MLP_NPARAM = 4801   # W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3
 BH_GROUP = 64       # particles sharing one Barnes-Hut acceptance test and list (MHPC_BH_GROUP)
+NOALLOC_SORT_MAX = 2097152   # elements the no-allocation key sort takes (MHPC_NOALLOC_SORT_MAX)
 MLP_HIDDEN = 64
 MLP_IN = 8          # 7 features + 1 zero pad
 
@@ -185,6 +187,7 @@ def _load() -> C.CDLL:
         "mhpc_eos": (i32, [vp, fp]),
         "mhpc_internal_forces": (i32, [vp, fp]),
         "mhpc_compute_diagnostics": (i32, [vp, C.POINTER(Diagnostics)]),
+        "mhpc_sort_keys_noalloc": (i32, [vp, i64, vp, vp, vp]),
         "mhpc_time_step": (i32, [vp, C.c_double, C.c_double, C.c_double, dp, dp]),
         "mhpc_pc_predictor": (i32, [vp, C.c_double]),
         "mhpc_pc_corrector": (i32, [vp, C.c_double]),
