@@ -76,9 +76,11 @@ def median_stats(stats):
     return out
 
 
-def time_steps(step, steps: int, warmup: int, label: str):
+def time_steps(step, steps: int, warmup: int, label: str, ctx=None):
     """W untimed steps, then K steps each bracketed by device synchronisations; returns the
-    per-step wall times (s), the total wall time of the K steps and the steps' results."""
+    per-step wall times (s), the total wall time of the K steps and the steps' results.  With
+    ctx, the context's own (spinning) synchronisation runs first, so the device-wide one that
+    follows finds the work done."""
     for i in range(warmup):
         step()
         log(f"{label}: warmup {i + 1}/{warmup} done")
@@ -88,6 +90,8 @@ def time_steps(step, steps: int, warmup: int, label: str):
     for i in range(steps):
         t0 = time.perf_counter()
         res.append(step())
+        if ctx is not None:
+            ctx.synchronize()
         sync()
         walls.append(time.perf_counter() - t0)
     total = time.perf_counter() - t_all
@@ -175,6 +179,10 @@ def main() -> int:
     ap.add_argument("--learned-nside", type=int, default=128)
     ap.add_argument("--learned-steps", type=int, default=10)
     ap.add_argument("--learned-train-iters", type=int, default=100)
+    ap.add_argument("--learned-workload", choices=("sedov", "bb"), default="sedov")
+    ap.add_argument("--learned-bb-n", type=int, default=1 << 20)
+    ap.add_argument("--learned-band", type=int, nargs=2, default=None, metavar=("LO", "HI"),
+                    help="neighbour-count band of the learned-vs-plain comparison (default: the workload's 50-80)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -249,11 +257,15 @@ def run_sedov(args, lib, ics, result):
     with lib.Context(n, p) as ctx:
         ctx.load(**{k: v for k, v in s.items() if k in lib.FIELD_ID})
         walls, total, res = time_steps(lambda: ctx.pc_euler_step(flags, courant_fact=0.3), args.steps,
-                                       args.warmup, "sedov")
+                                       args.warmup, "sedov", ctx)
         state = {k: ctx.download(k) for k in STATE_FIELDS}
+        # two more steps with every phase timed (per-phase events are left out of the timed steps;
+        # those time the total, the pass-0 walk, the density and the forces kernels)
+        phase_res = [ctx.pc_euler_step(flags | lib.RHS_TIMING, courant_fact=0.3) for _ in range(2)]
     stats = [st for _, s0, s1 in res for st in (s0, s1)]
     dts = [dt for dt, _, _ in res]
     med = median_stats(stats)
+    phase_med = median_stats([st for _, s0, s1 in phase_res for st in (s0, s1)])
     nnb = float(np.mean([st["n_neighbours"] for st in stats])) / n
     table = sph_kernel_table(nnb)
     kernels = {}
@@ -262,9 +274,9 @@ def run_sedov(args, lib, ics, result):
         kernels[k[3:]] = {"kernel": name, "ms": round(med[k], 4), "alg_bytes_per_particle": round(bpp, 1),
                           "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     kernels["nn_fixup"] = {"kernel": "k_nn_mark_dirty + k_nn_walk<0> + k_nn_one<0> (re-lists)",
-                           "ms": round(med["ms_nn"], 4), "note": "re-lists only the changed neighbourhoods"}
+                           "ms": round(phase_med["ms_nn"], 4), "note": "re-lists only the changed neighbourhoods"}
     kernels["hiter_rest"] = {"kernel": "pass-0 outliers + tail passes (k_nn_one, k_nn_count_direct, k_h_finish)",
-                             "ms": round(med["ms_hiter"] - med["ms_nn_walk"], 4)}
+                             "ms": round(phase_med["ms_hiter"] - phase_med["ms_nn_walk"], 4)}
     # the roofline is quoted on the kernel with the largest time per RHS
     dom = max(table, key=lambda k: med[k])
     name, bpp = table[dom]
@@ -290,7 +302,8 @@ def run_sedov(args, lib, ics, result):
                      "kernel_ms": med[dom], "timing": "hipEvents around the kernel on the context stream, "
                                                       "median over the timed RHS"},
         "sph_kernels": kernels,
-        "phases_ms": {k[3:]: round(v, 4) for k, v in med.items() if k.startswith("ms_")},
+        "phases_ms": {k[3:]: round(v, 4) for k, v in phase_med.items() if k.startswith("ms_")},
+        "phases_note": "per-phase times from two steps after the timed ones, with every phase timed",
         "mean_neighbours": nnb,
         "mean_h_counts": float(np.mean([st["n_h_counts"] for st in stats])) / n,
         "h_passes_max": int(max(st["h_passes"] for st in stats)),
@@ -313,13 +326,13 @@ def run_sedov_initial(args, lib, ics, result):
     h0 = torch.tensor(s["sml"], dtype=torch.float64, device="cuda")
     with lib.Context(n, p) as ctx:
         ctx.load(**{k: v for k, v in s.items() if k in lib.FIELD_ID})
-        for name, flags in (("speculative", lib.RHS_SPH | lib.RHS_VARIABLE_SML | lib.RHS_SPECULATIVE_NN),
-                            ("two_walk", lib.RHS_SPH | lib.RHS_VARIABLE_SML)):
+        for name, flags in (("speculative", lib.RHS_SPH | lib.RHS_VARIABLE_SML | lib.RHS_SPECULATIVE_NN | lib.RHS_TIMING),
+                            ("two_walk", lib.RHS_SPH | lib.RHS_VARIABLE_SML | lib.RHS_TIMING)):
             def one():
                 ctx.upload_device("sml", h0.data_ptr())
                 return ctx.rhs(flags)
 
-            walls, _, res = time_steps(one, 10, 2, f"sedov-initial-{name}")
+            walls, _, res = time_steps(one, 10, 2, f"sedov-initial-{name}", ctx)
             med = median_stats(res)
             out[name] = {"ms_per_rhs": float(np.median(walls)) * 1e3, "value": n / float(np.median(walls)),
                          "phases_ms": {k[3:]: round(v, 4) for k, v in med.items() if k.startswith("ms_")}}
@@ -338,12 +351,24 @@ def run_learned(args, lib, ics, result):
     import torch
 
     nside = args.learned_nside
-    s = ics.sedov_lattice(nside)
+    band = tuple(args.learned_band) if args.learned_band else (50, 80)
+    if args.learned_workload == "bb":
+        # Boss-Bodenheimer: self-gravity + SPH, random particles in a sphere (no lattice shells,
+        # so narrow bands are reachable), FP32 + Kahan pair sums
+        s = ics.boss_bodenheimer(args.learned_bb_n)
+        p = ics.params_boss_bodenheimer()
+        p.band_lo, p.band_hi = band
+        p.sml_max = 4.0 * float(s["sml"].max())
+        base = lib.RHS_GRAVITY | lib.RHS_SPH | lib.RHS_VARIABLE_SML | lib.RHS_TIMING
+        wl = f"Boss-Bodenheimer {len(s['x'])} (self-gravity + SPH, FP32 + Kahan)"
+    else:
+        s = ics.sedov_lattice(nside)
+        p = ics.params_sedov(sml_max=4.0 * float(s["sml"].max()), band_lo=band[0], band_hi=band[1])
+        base = lib.RHS_SPH | lib.RHS_VARIABLE_SML | lib.RHS_TIMING
+        wl = f"Sedov {nside}^3"
     n = len(s["x"])
     for k in ("ax", "ay", "az", "dedt", "g_ax", "g_ay", "g_az", "muijmax", "drhodt", "dsmldt"):
         s.setdefault(k, np.zeros(n))
-    p = ics.params_sedov(sml_max=4.0 * float(s["sml"].max()))
-    base = lib.RHS_SPH | lib.RHS_VARIABLE_SML
     train = dict(nsamples=65536, iters=args.learned_train_iters, lr=2e-3, seed=5)
     out = {}
     with lib.Context(n, p) as ctx:
@@ -376,8 +401,8 @@ def run_learned(args, lib, ics, result):
             log(f"learned-vs-plain: {name}: {out[name]['rhs_ms']:.3f} ms/RHS, "
                 f"{out[name]['mean_h_iterations']:.4f} h-iterations")
     result["learned_h"] = {
-        "workload": f"Sedov {nside}^3 evolving (PC-Euler, CFL 0.3): 5 plain training steps, then "
-                    f"{args.learned_steps} steps each way from the same state; band 50-80, FP64",
+        "workload": f"{wl} evolving (PC-Euler, CFL 0.3): 5 plain training steps, then "
+                    f"{args.learned_steps} steps each way from the same state; band {band[0]}-{band[1]}, FP64",
         "n_particles": n, "train_loss_first_last": [losses[0], losses[-1]],
         **out,
         "speedup_learned_vs_plain": out["plain"]["rhs_ms"] / out["learned"]["rhs_ms"],
@@ -411,7 +436,8 @@ def run_plummer(args, lib, ics, result):
     pp = ics.params_plummer_large()
     with lib.Context(args.plummer_n, pp) as ctx:
         ctx.load(**pl)
-        walls, _, pst = time_steps(lambda: ctx.rhs(lib.RHS_GRAVITY), max(1, args.steps // 2), 1, "plummer")
+        walls, _, pst = time_steps(lambda: ctx.rhs(lib.RHS_GRAVITY | lib.RHS_TIMING), max(1, args.steps // 2), 1,
+                                   "plummer", ctx)
         pm = median_stats(pst)
         acc = None if args.no_accuracy else plummer_accuracy(ctx, lib, pl, pp)
         drift = None
@@ -437,7 +463,8 @@ def run_plummer(args, lib, ics, result):
     step_ms = float(np.median(walls)) * 1e3
     result["plummer"] = {
         "workload": f"Plummer {args.plummer_n} Barnes-Hut RHS, theta 0.6, eps 2e-4, monopole + quadrupole, "
-                    "per-member (vote) group acceptance, FP32 + Kahan far field, FP64 positions and keys",
+                    f"{'per-member (vote)' if pp.bh_accept == lib.BH_VOTE else 'group-box'} acceptance, "
+                    "FP32 + Kahan far field, FP64 positions and keys",
         "value": args.plummer_n / (step_ms * 1e-3), "unit": "particles/s", "ms_per_step": step_ms,
         "phases_ms": {k[3:]: round(v, 4) for k, v in pm.items() if k.startswith("ms_")},
         "interactions_per_particle": (pm["n_part_interactions"] + pm["n_cell_interactions"]) / args.plummer_n,
@@ -457,7 +484,9 @@ def run_plummer(args, lib, ics, result):
         pl = ics.plummer(n8)
         with lib.Context(n8, pp) as ctx:
             ctx.load(**pl)
-            w8, _, pst8 = time_steps(lambda: ctx.rhs(lib.RHS_GRAVITY), max(1, args.steps // 2), 1, "plummer-shard")
+            w8, _, pst8 = time_steps(lambda: ctx.rhs(lib.RHS_GRAVITY | lib.RHS_TIMING), max(1, args.steps // 2), 1,
+                                     "plummer-shard",
+                                     ctx)
         pm8 = median_stats(pst8)
         result["plummer"]["shard_of_8"] = {
             "n_particles": n8, "value": n8 / float(np.median(w8)), "ms_per_step": float(np.median(w8)) * 1e3,

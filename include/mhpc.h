@@ -140,7 +140,9 @@ enum {
     MHPC_RHS_SPH = 1u << 1,
     MHPC_RHS_VARIABLE_SML = 1u << 2,   /* plain h-iteration before the neighbour search */
     MHPC_RHS_POTENTIAL = 1u << 3,      /* also accumulate the gravitational potential */
-    MHPC_RHS_TIMING = 1u << 4,         /* per-phase hipEvent timing (adds no sync) */
+    MHPC_RHS_TIMING = 1u << 4,         /* per-phase hipEvent timing (adds no sync; without it
+                                          only ms_total, ms_nn_walk, ms_density and ms_forces
+                                          are measured) */
     MHPC_RHS_LEARNED_H = 1u << 5,      /* MLP predicts h; the plain iteration corrects */
     MHPC_RHS_SPECULATIVE_NN = 1u << 6, /* pass 0 of the h-iteration writes the neighbour
                                           lists; only particles whose list can change are

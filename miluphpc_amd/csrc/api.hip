@@ -105,24 +105,23 @@ static void free_all(mhpc_ctx *c) {
                     c->valB, c->tile_hist, c->tile_off, c->digit_hist, c->slkey, c->sxm, c->sv4,
                     c->sr4, c->sh, c->sh2, c->se, c->srho, c->sp, c->scs, c->snoi, c->siters, c->sdone,
                     c->lcp, c->tcnt, c->toff,
-                    c->level_count, c->cell_base, c->cstart, c->cend, c->clevel, c->cchild,
+                    c->level_count, c->cstart, c->cend, c->clevel, c->cchild,
                     c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax,
-                    c->nbr, c->htable, c->bbox_partial, c->box, c->err, c->counters,
+                    c->nbr, c->htable, c->bbox_partial, c->box, c->dstatus,
                     c->diag_partial, c->diag_out, c->pc_backup, c->mlp_params, c->mlp_m1, c->mlp_m2,
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
                     c->mlp_partial, c->mlp_loss_partial, c->mlp_loss, c->mlp_io_x, c->mlp_io_y,
-                    c->hlist[0], c->hlist[1], c->hlist[2], c->hlist_count, c->sdirty,
+                    c->hlist[0], c->hlist[1], c->hlist[2], c->sdirty,
                     c->h0slot, c->olist, c->ocount, c->hcnt, c->nn_start, c->nn_start_rng, c->hhard,
-                    c->xyzm, c->ppar, c->cpar, c->sort_check};
+                    c->xyzm, c->ppar, c->cpar};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < MHPC_F_NOI; f++)
         if (c->f_own[f]) (void)hipFree(c->f_own[f]);
-    if (c->h_err) (void)hipHostFree(c->h_err);
-    if (c->h_counters) (void)hipHostFree(c->h_counters);
-    if (c->h_sort_check) (void)hipHostFree(c->h_sort_check);
+    if (c->h_status) (void)hipHostFree(c->h_status);
     for (int e = 0; e < MHPC_NUM_EVENTS; e++)
         if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
+    if (c->sync_ev) (void)hipEventDestroy(c->sync_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -130,6 +129,16 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(hipGetDevice(&c->device));
     HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (int e = 0; e < MHPC_NUM_EVENTS; e++) HIPC(hipEventCreate(&c->ev[e]));
+    HIPC(hipEventCreateWithFlags(&c->sync_ev, hipEventDisableTiming));
+    HIPC(hipMalloc((void **)&c->dstatus, sizeof(DevStatus)));
+    HIPC(hipMemset(c->dstatus, 0, sizeof(DevStatus)));
+    HIPC(hipHostMalloc((void **)&c->h_status, sizeof(DevStatus), hipHostMallocDefault));
+    memset(c->h_status, 0, sizeof(DevStatus));
+    c->err = &c->dstatus->err;
+    c->counters = c->dstatus->counters;
+    c->sort_check = &c->dstatus->sort_check;
+    c->cell_base = c->dstatus->cell_base;
+    c->hlist_count = c->dstatus->hlist_count;
     c->cap = cap;
     c->cap_pad = ceil_div(cap, 64) * 64;
     c->cap_cells = cap_cells;
@@ -151,7 +160,6 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->slkey, cap));
     HIPC(dalloc(&c->perm, cap));
     HIPC(dalloc(&c->valB, cap));
-    HIPC(dalloc(&c->sort_check, 1));
     HIPC(dalloc(&c->xyzm, cap));
     c->sort_tiles_cap = ceil_div(cap, MHPC_SORT_TILE);
     HIPC(dalloc(&c->tile_hist, MHPC_SORT_BUCKETS * c->sort_tiles_cap));
@@ -174,7 +182,6 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->hhard, cap));
     HIPC(dalloc(&c->olist, cap));
     HIPC(dalloc(&c->ocount, 4));
-    HIPC(dalloc(&c->hlist_count, 2 * (MHPC_MAX_PASSES + 1)));
     HIPC(dalloc(&c->hcnt, cap));
     // start table levels: the smallest K with 8^K >= cap (one geometric cell per particle on
     // average at the deepest level), 1 <= K <= MHPC_NN_KMAX
@@ -188,8 +195,6 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->toff, (MHPC_LEVELS + 1) * c->tree_tiles_cap));
     HIPC(dalloc(&c->level_count, MHPC_LEVELS + 2));
     HIPC(hipMemset(c->level_count, 0, sizeof(int32_t) * (MHPC_LEVELS + 2)));
-    HIPC(dalloc(&c->cell_base, MHPC_LEVELS + 3));
-    HIPC(hipMemset(c->cell_base, 0, sizeof(int32_t) * (MHPC_LEVELS + 3)));
     HIPC(dalloc(&c->cstart, cap_cells));
     HIPC(dalloc(&c->cend, cap_cells));
     HIPC(dalloc(&c->clevel, cap_cells));
@@ -211,14 +216,6 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->diag_out, diag_num_quantities()));
     HIPC(dalloc(&c->box, 6));
     HIPC(hipMemset(c->box, 0, sizeof(double) * 6));
-    HIPC(dalloc(&c->err, 1));
-    HIPC(hipMemset(c->err, 0, sizeof(uint32_t)));
-    HIPC(dalloc(&c->counters, CNT_COUNT_));
-    HIPC(hipMemset(c->counters, 0, sizeof(unsigned long long) * CNT_COUNT_));
-    HIPC(hipHostMalloc((void **)&c->h_err, sizeof(uint32_t), hipHostMallocDefault));
-    HIPC(hipHostMalloc((void **)&c->h_sort_check, sizeof(uint32_t), hipHostMallocDefault));
-    HIPC(hipHostMalloc((void **)&c->h_counters, sizeof(unsigned long long) * CNT_COUNT_,
-                       hipHostMallocDefault));
     return MHPC_OK;
 }
 
@@ -248,7 +245,7 @@ extern "C" int mhpc_create(const mhpc_params *p, int64_t cap_particles, int64_t 
 
 extern "C" int mhpc_destroy(mhpc_ctx *c) {
     if (!c) return MHPC_E_ARG;
-    (void)hipStreamSynchronize(c->stream);
+    (void)ctx_sync(c);
     free_all(c);
     delete c;
     return MHPC_OK;
@@ -291,7 +288,7 @@ extern "C" const char *mhpc_last_error(mhpc_ctx *c) { return c ? c->last_error.c
 
 extern "C" int mhpc_synchronize(mhpc_ctx *c) {
     if (!c) return MHPC_E_ARG;
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     return MHPC_OK;
 }
 
@@ -317,7 +314,7 @@ extern "C" int mhpc_upload(mhpc_ctx *c, int field, const void *host, int64_t n) 
     FieldDesc d = field_desc(c, field);
     if (!d.ptr) return MHPC_E_ARG;
     HIPC(hipMemcpyAsync(d.ptr, host, d.elem * (size_t)n, hipMemcpyHostToDevice, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
     c->nn_valid = c->rho_valid = c->eos_valid = false;
     return MHPC_OK;
@@ -328,7 +325,7 @@ extern "C" int mhpc_download(mhpc_ctx *c, int field, void *host, int64_t n) {
     FieldDesc d = field_desc(c, field);
     if (!d.ptr) return MHPC_E_ARG;
     HIPC(hipMemcpyAsync(host, d.ptr, d.elem * (size_t)n, hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     return MHPC_OK;
 }
 
@@ -357,7 +354,7 @@ static void invalidate(mhpc_ctx *c) {
 
 extern "C" int mhpc_bind_field(mhpc_ctx *c, int field, void *device_ptr) {
     if (!c || field < 0 || field >= MHPC_F_NOI) return MHPC_E_ARG;
-    HIPC(hipStreamSynchronize(c->stream));   // work queued on the previous array finishes first
+    HIPC(ctx_sync(c));   // work queued on the previous array finishes first
     c->f[field] = device_ptr ? static_cast<double *>(device_ptr) : c->f_own[field];
     invalidate(c);
     return MHPC_OK;
@@ -373,14 +370,14 @@ extern "C" int mhpc_device_alloc(mhpc_ctx *c, int64_t bytes, void **device_ptr) 
 
 extern "C" int mhpc_device_upload(mhpc_ctx *c, void *device_ptr, const void *host, int64_t bytes) {
     if (!c || !device_ptr || !host || bytes < 0) return MHPC_E_ARG;
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     HIPC(hipMemcpy(device_ptr, host, (size_t)bytes, hipMemcpyHostToDevice));
     return MHPC_OK;
 }
 
 extern "C" int mhpc_device_download(mhpc_ctx *c, void *host, const void *device_ptr, int64_t bytes) {
     if (!c || !device_ptr || !host || bytes < 0) return MHPC_E_ARG;
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     HIPC(hipMemcpy(host, device_ptr, (size_t)bytes, hipMemcpyDeviceToHost));
     return MHPC_OK;
 }
@@ -390,7 +387,7 @@ extern "C" int mhpc_device_free(mhpc_ctx *c, void *device_ptr) {
     if (!device_ptr) return MHPC_OK;
     for (int f = 0; f < MHPC_F_NOI; f++)
         if (c->f[f] == device_ptr) return MHPC_E_STATE;   // still bound to a field
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     HIPC(hipFree(device_ptr));
     return MHPC_OK;
 }
@@ -398,7 +395,7 @@ extern "C" int mhpc_device_free(mhpc_ctx *c, void *device_ptr) {
 extern "C" int mhpc_get_box(mhpc_ctx *c, double box[6]) {
     if (!c || !box) return MHPC_E_ARG;
     HIPC(hipMemcpyAsync(box, c->box, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     return MHPC_OK;
 }
 
@@ -408,14 +405,23 @@ static void reset_flags(mhpc_ctx *c) {
     launch_fill_u64(c, c->counters, 0ull, CNT_COUNT_);
 }
 
+hipError_t ctx_sync(mhpc_ctx *c) {
+    hipError_t e = hipEventRecord(c->sync_ev, c->stream);
+    if (e != hipSuccess) return e;
+    while ((e = hipEventQuery(c->sync_ev)) == hipErrorNotReady) __builtin_ia32_pause();
+    return e;
+}
+
+hipError_t ctx_fetch_status(mhpc_ctx *c) {
+    hipError_t e = hipMemcpyAsync(c->h_status, c->dstatus, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream);
+    return e != hipSuccess ? e : ctx_sync(c);
+}
+
 static int finish(mhpc_ctx *c) {
     HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(c->h_err, c->err, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipMemcpyAsync(c->h_counters, c->counters, sizeof(unsigned long long) * CNT_COUNT_,
-                        hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
-    const uint32_t e = *c->h_err;
-    if (c->h_counters[CNT_TREE_DEPTH] > 0) c->com_depth_hint = (int)c->h_counters[CNT_TREE_DEPTH];
+    HIPC(ctx_fetch_status(c));
+    const uint32_t e = c->h_status->err;
+    if (c->h_status->counters[CNT_TREE_DEPTH] > 0) c->com_depth_hint = (int)c->h_status->counters[CNT_TREE_DEPTH];
     if (e == 0) return MHPC_OK;
     int st = MHPC_OK;
     if (e & ERRF_DEPTH) st = MHPC_E_DEPTH;
@@ -573,7 +579,7 @@ extern "C" int mhpc_time_step(mhpc_ctx *c, double courant_fact, double forces_fa
     launch_time_step(c, c->diag_partial, c->diag_out);
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(q, c->diag_out, sizeof(q), hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     double t = fmin(courant_fact * q[1], forces_fact * q[0]);
     t = fmin(t, courant_fact * q[2]);
     if (q[3] > 0.0 && q[4] > 0.0) t = fmin(t, q[4] / q[3]);
@@ -689,7 +695,7 @@ extern "C" int mhpc_mlp_features(mhpc_ctx *c, double dt) {
     if (!c->mlp_params) return MHPC_E_STATE;
     launch_mlp_features(c, dt);
     HIPC(hipGetLastError());
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     c->mlp_x_valid = true;
     return MHPC_OK;
 }
@@ -764,7 +770,7 @@ extern "C" int mhpc_mlp_train(mhpc_ctx *c, int nsamples, int iters, float lr, ui
     HIPC(hipGetLastError());
     float l = 0.f;
     if (iters > 0) HIPC(hipMemcpyAsync(&l, c->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     if (loss) *loss = l;
     return MHPC_OK;
 }
@@ -790,7 +796,7 @@ extern "C" int mhpc_mlp_forward(mhpc_ctx *c, const float *X, int64_t n, float *Y
     launch_mlp_forward_raw(c, n, c->mlp_io_x, c->mlp_io_y);
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(Y, c->mlp_io_y, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     return MHPC_OK;
 }
 
@@ -808,7 +814,7 @@ extern "C" int mhpc_mlp_gradient(mhpc_ctx *c, const float *X, const float *T, in
                         c->stream));
     float l = 0.f;
     HIPC(hipMemcpyAsync(&l, c->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     if (loss) *loss = l;
     return MHPC_OK;
 }
@@ -825,7 +831,7 @@ extern "C" int mhpc_pc_predictor(mhpc_ctx *c, double dt) {
     if (st != MHPC_OK) return st;
     launch_pc_predictor(c, dt);
     HIPC(hipGetLastError());
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
     c->nn_valid = c->rho_valid = c->eos_valid = false;
     return MHPC_OK;
@@ -835,7 +841,7 @@ extern "C" int mhpc_pc_corrector(mhpc_ctx *c, double dt) {
     if (!c || !c->pc_backup) return c ? MHPC_E_STATE : MHPC_E_ARG;
     launch_pc_corrector(c, dt);
     HIPC(hipGetLastError());
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
     c->nn_valid = c->rho_valid = c->eos_valid = false;
     return MHPC_OK;
@@ -845,7 +851,7 @@ extern "C" int mhpc_leapfrog_update_x(mhpc_ctx *c, double dt) {
     if (!c) return MHPC_E_ARG;
     launch_leapfrog_x(c, dt);
     HIPC(hipGetLastError());
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
     c->nn_valid = c->rho_valid = c->eos_valid = false;
     return MHPC_OK;
@@ -855,7 +861,7 @@ extern "C" int mhpc_leapfrog_update_v(mhpc_ctx *c, double dt) {
     if (!c) return MHPC_E_ARG;
     launch_leapfrog_v(c, dt);
     HIPC(hipGetLastError());
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     return MHPC_OK;
 }
 
@@ -865,7 +871,7 @@ extern "C" int mhpc_compute_diagnostics(mhpc_ctx *c, mhpc_diagnostics *out) {
     launch_diagnostics(c, c->diag_partial, d);
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(out, d, sizeof(*out), hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     return MHPC_OK;
 }
 
@@ -880,7 +886,7 @@ extern "C" int mhpc_download_neighbours(mhpc_ctx *c, int32_t *nnl, int32_t *noi,
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(nnl, d_nnl, sizeof(int32_t) * (size_t)(n * max_nb), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipMemcpyAsync(noi, d_noi, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     (void)hipFree(d_nnl);
     (void)hipFree(d_noi);
     return MHPC_OK;
@@ -891,7 +897,7 @@ extern "C" int mhpc_tree_size(mhpc_ctx *c, int64_t *num_cells) {
     if (!c->tree_valid) return MHPC_E_STATE;
     int32_t cb[MHPC_LEVELS + 3];
     HIPC(hipMemcpyAsync(cb, c->cell_base, sizeof(cb), hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     *num_cells = cb[MHPC_LEVELS + 1];
     return MHPC_OK;
 }
@@ -917,7 +923,7 @@ extern "C" int mhpc_download_tree(mhpc_ctx *c, int32_t *cell_level, int32_t *cel
         for (int e = 0; e < 6; e++)
             HIPC(hipMemcpyAsync(cell_quad + e * nc, c->cquad + e * c->cap_cells, d8, hipMemcpyDeviceToHost,
                                 c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(ctx_sync(c));
     return MHPC_OK;
 }
 
@@ -950,7 +956,12 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     // features are still taken (raw) so that a following mhpc_mlp_train can use them
     const bool predict = learned && c->mlp_norm_set;
     int e = 0;
-    auto mark = [&](int idx) { (void)hipEventRecord(c->ev[idx], c->stream); };
+    // per-phase events only with MHPC_RHS_TIMING; the total (13 -> 12), the pass-0 walk
+    // (14 -> 15), the density (9 -> 10) and the forces (11 -> 12) are always timed
+    const bool timing = flags & MHPC_RHS_TIMING;
+    auto mark = [&](int idx) {
+        if (timing || idx >= 9) (void)hipEventRecord(c->ev[idx], c->stream);
+    };
     reset_flags(c);
     mark(13);
     if (feats) {
@@ -995,37 +1006,36 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     if (feats) c->mlp_x_valid = st == MHPC_OK;
     if (stats) {
         memset(stats, 0, sizeof(*stats));
-        stats->ms_bbox = elapsed(c, 0, 1);
-        stats->ms_keys = elapsed(c, 1, 2);
-        stats->ms_sort = elapsed(c, 2, 3);
-        stats->ms_gather = elapsed(c, 3, 4);
-        stats->ms_tree = elapsed(c, 4, 5);
-        stats->ms_com = elapsed(c, 5, 6);
-        stats->ms_gravity = elapsed(c, 6, 7);
-        stats->ms_hiter = elapsed(c, 7, 8);
-        stats->ms_nn = elapsed(c, 8, 9);
-        stats->ms_density = elapsed(c, 9, 10);
-        stats->ms_eos = elapsed(c, 10, 11);
-        stats->ms_forces = elapsed(c, 11, 12);
-        stats->ms_mlp = elapsed(c, 13, 0);
+        if (timing) {
+            stats->ms_bbox = elapsed(c, 0, 1);
+            stats->ms_keys = elapsed(c, 1, 2);
+            stats->ms_sort = elapsed(c, 2, 3);
+            stats->ms_gather = elapsed(c, 3, 4);
+            stats->ms_tree = elapsed(c, 4, 5);
+            stats->ms_com = elapsed(c, 5, 6);
+            stats->ms_gravity = elapsed(c, 6, 7);
+            stats->ms_hiter = elapsed(c, 7, 8);
+            stats->ms_nn = elapsed(c, 8, 9);
+            stats->ms_eos = elapsed(c, 10, 11);
+            stats->ms_mlp = elapsed(c, 13, 0);
+        }
+        stats->ms_density = sph ? elapsed(c, 9, 10) : 0.f;
+        stats->ms_forces = sph ? elapsed(c, 11, 12) : 0.f;
         stats->ms_total = elapsed(c, 13, 12);
-        int32_t cb[MHPC_LEVELS + 3];
-        if (hipMemcpy(cb, c->cell_base, sizeof(cb), hipMemcpyDeviceToHost) == hipSuccess)
-            stats->num_cells = cb[MHPC_LEVELS + 1];
-        stats->n_cell_interactions = (int64_t)c->h_counters[CNT_CELL_INT];
-        stats->n_part_interactions = (int64_t)c->h_counters[CNT_PART_INT];
-        stats->n_neighbours = (int64_t)c->h_counters[CNT_NEIGHBOURS];
-        stats->max_noi = (int32_t)c->h_counters[CNT_MAX_NOI];
-        stats->n_h_counts = (int64_t)c->h_counters[CNT_H_COUNTS];
+        const DevStatus &hs = *c->h_status;   // fetched by finish()
+        stats->num_cells = hs.cell_base[MHPC_LEVELS + 1];
+        stats->n_cell_interactions = (int64_t)hs.counters[CNT_CELL_INT];
+        stats->n_part_interactions = (int64_t)hs.counters[CNT_PART_INT];
+        stats->n_neighbours = (int64_t)hs.counters[CNT_NEIGHBOURS];
+        stats->max_noi = (int32_t)hs.counters[CNT_MAX_NOI];
+        stats->n_h_counts = (int64_t)hs.counters[CNT_H_COUNTS];
         stats->ms_nn_walk = hiter ? elapsed(c, 14, 15) : 0.f;
         stats->h_passes = 0;
         if (hiter) {
             // pass 0, then every pass whose input list (the previous pass's unconverged
             // particles) was not empty
-            uint32_t lc[MHPC_MAX_PASSES + 1];
             stats->h_passes = 1;
-            if (hipMemcpy(lc, c->hlist_count, sizeof(lc), hipMemcpyDeviceToHost) == hipSuccess)
-                for (int pass = 1; pass < c->p.max_counts; pass++) stats->h_passes += lc[pass - 1] > 0 ? 1 : 0;
+            for (int pass = 1; pass < c->p.max_counts; pass++) stats->h_passes += hs.hlist_count[pass - 1] > 0 ? 1 : 0;
         }
     }
     return st;

@@ -203,10 +203,7 @@ void launch_sort(mhpc_ctx *c) {
         sort_passes(c->stream, n, first, ntiles, ctx_bufs(c));
         hipLaunchKernelGGL(k_sort_check, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, c->skey, n,
                            MHPC_SORT_BITS * first, c->sort_check);
-        if (hipMemcpyAsync(c->h_sort_check, c->sort_check, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream) ==
-                hipSuccess &&
-            hipStreamSynchronize(c->stream) == hipSuccess && *c->h_sort_check == 0u)
-            return;
+        if (ctx_fetch_status(c) == hipSuccess && c->h_status->sort_check == 0u) return;
         // the full sort also needs the low passes' histograms
         hipLaunchKernelGGL(k_sort_hist_all, dim3(hb), dim3(SORT_BLOCK), 0, c->stream, c->key, n,
                            c->digit_hist, c->sort_check, 0, first);

@@ -58,6 +58,16 @@ enum {
     CNT_COUNT_ = 8
 };
 
+// The words the host reads after an RHS or a stage, in one device block so that one copy
+// brings them all to the pinned host mirror (one round trip per call).
+struct DevStatus {
+    unsigned long long counters[CNT_COUNT_];
+    uint32_t err;
+    uint32_t sort_check;                         // set when a partial sort left equal prefixes
+    int32_t cell_base[MHPC_LEVELS + 3];          // level-major cell offsets, [23] = total
+    uint32_t hlist_count[2 * (MHPC_MAX_PASSES + 1)];
+};
+
 struct mhpc_ctx {
     mhpc_params p;
     int64_t cap = 0, cap_pad = 0, cap_cells = 0, n = 0;
@@ -77,8 +87,7 @@ struct mhpc_ctx {
     int32_t *perm = nullptr, *valB = nullptr;    // sorted position -> particle index
     double4 *xyzm = nullptr;                     // {x,y,z,m} in caller order (written with the keys)
     int32_t *ppar = nullptr;                     // sorted particle -> parent cell (tree build)
-    uint32_t *sort_check = nullptr;              // set when a partial sort left equal sorted prefixes
-    uint32_t *h_sort_check = nullptr;            // pinned host copy
+    uint32_t *sort_check = nullptr;              // = &dstatus->sort_check
     int32_t *cpar = nullptr;                     // cell -> parent cell (tree build)
     uint32_t *tile_hist = nullptr, *tile_off = nullptr;
     uint32_t *digit_hist = nullptr;              // [passes][256]
@@ -99,7 +108,7 @@ struct mhpc_ctx {
     uint32_t *tcnt = nullptr, *toff = nullptr;   // [22][tree tiles]
     int64_t tree_tiles_cap = 0;
     int32_t *level_count = nullptr;              // [22]
-    int32_t *cell_base = nullptr;                // [24] level-major offsets, [23] = total
+    int32_t *cell_base = nullptr;                // = dstatus->cell_base
     int32_t *cstart = nullptr, *cend = nullptr, *clevel = nullptr, *cchild = nullptr;
     double *cmass = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr;
     double *cquad = nullptr;                     // [6][cap_cells]
@@ -124,7 +133,7 @@ struct mhpc_ctx {
     uint32_t *ocount = nullptr;
     double *h0slot = nullptr;                    // pass-0 h of the unconverged particles (list slot)
     int32_t *hhard = nullptr;                    // tail passes: slots left to the split counts
-    uint32_t *hlist_count = nullptr;             // [2 (MHPC_MAX_PASSES + 1)]: list lengths of the
+    uint32_t *hlist_count = nullptr;             // = dstatus->hlist_count: list lengths of the
                                                  // passes, then the hard-list lengths of the passes
     uint32_t *hcnt = nullptr;                    // neighbour count per list slot
 
@@ -142,11 +151,12 @@ struct mhpc_ctx {
     int mlp_adam_step = 0;
     bool mlp_x_valid = false, mlp_norm_set = false;
     double *box = nullptr;                       // [6]
-    uint32_t *err = nullptr;
-    unsigned long long *counters = nullptr;
-    uint32_t *h_err = nullptr;                   // pinned
-    unsigned long long *h_counters = nullptr;    // pinned
+    DevStatus *dstatus = nullptr;                // device block: err, counters, cell_base, list lengths
+    DevStatus *h_status = nullptr;               // its pinned host mirror
+    uint32_t *err = nullptr;                     // = &dstatus->err
+    unsigned long long *counters = nullptr;      // = dstatus->counters
     hipEvent_t ev[MHPC_NUM_EVENTS] = {};
+    hipEvent_t sync_ev = nullptr;                // ctx_sync's marker (no timing)
 
     bool keys_valid = false, sorted_valid = false, tree_valid = false, com_valid = false;
     // neighbour-walk start table: for levels 1..nn_kmax, the tree node covering each
@@ -164,6 +174,11 @@ struct mhpc_ctx {
 
 // ------------------------------------------------------------------------------------
 // host-side launchers (defined in the k_*.hip files)
+// waits for the context stream's queued work by spinning on an event (no blocking wait: the
+// host thread sees completion within microseconds)
+hipError_t ctx_sync(mhpc_ctx *c);
+// copies the status block to its host mirror and waits for it (ctx_sync)
+hipError_t ctx_fetch_status(mhpc_ctx *c);
 void launch_fill_u32(mhpc_ctx *c, uint32_t *p, uint32_t v, int64_t n);
 void fill_u32_on(hipStream_t stream, uint32_t *p, uint32_t v, int64_t n);
 void launch_fill_u64(mhpc_ctx *c, unsigned long long *p, unsigned long long v, int64_t n);

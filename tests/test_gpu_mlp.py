@@ -147,3 +147,51 @@ def test_learned_training_reduces_loss_and_keeps_density(gpu_lib, orc):
     st_o, rho_o = orc.density(f["x"], f["y"], f["z"], f["mass"], f["sml"], nnl, noi)
     assert st_o == 0
     assert np.max(np.abs(f["rho"] - rho_o) / rho_o) <= 1e-12
+
+
+def test_learned_h_cuts_iterations_on_boss_bodenheimer(gpu_lib, orc):
+    """What the trained network gains: on the rotating cloud (self-gravity + SPH, random
+    particles, so every count is reachable) with the reference's stop tolerance |N - 64| < 5
+    (band 60-68), a network trained online on the plain iteration's own pairs for five steps
+    starts the iteration closer to the converged h: from the same state, three steps need fewer
+    neighbour counts per particle than three plain steps.  The learned path's density is the
+    FP64 SPH density of its own converged h (the oracle's, to 1e-12)."""
+    print_case("learned-h vs plain, boss-bodenheimer 131072, band 60-68")
+    n = 1 << 17
+    s = ics.boss_bodenheimer(n)
+    for k in ("ax", "ay", "az", "dedt", "g_ax", "g_ay", "g_az", "muijmax", "drhodt", "dsmldt"):
+        s.setdefault(k, np.zeros(n))
+    p = ics.params_boss_bodenheimer()
+    p.band_lo, p.band_hi = 60, 68
+    p.precision = gpu_lib.PREC_FP64
+    p.sml_max = 4.0 * float(s["sml"].max())
+    base = gpu_lib.RHS_GRAVITY | gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML
+    fields = ("x", "y", "z", "vx", "vy", "vz", "mass", "sml", "e", "rho", "p", "cs", "ax", "ay", "az",
+              "g_ax", "g_ay", "g_az", "drhodt", "dedt", "dsmldt", "muijmax", "curlv", "gradrho", "noi")
+    with gpu_lib.Context(n, p) as ctx:
+        ctx.load(**{k: v for k, v in s.items() if k in gpu_lib.FIELD_ID})
+        ctx.mlp_init(20261203)
+        train = dict(nsamples=8192, iters=100, lr=2e-3, seed=5)
+        losses = []
+        for _ in range(5):
+            _, s0, s1 = ctx.pc_euler_step(base | gpu_lib.RHS_FEATURES, courant_fact=0.3, train=train)
+            losses += [s0["mlp_loss"], s1["mlp_loss"]]
+        snap = {k: ctx.download(k) for k in fields}
+        counts = {}
+        for name, flags in (("plain", base), ("learned", base | gpu_lib.RHS_LEARNED_H)):
+            ctx.load(**snap)
+            c = []
+            for _ in range(3):
+                _, s0, s1 = ctx.pc_euler_step(flags, courant_fact=0.3)
+                c += [s0["n_h_counts"] / n, s1["n_h_counts"] / n]
+            counts[name] = float(np.mean(c))
+        ctx.rhs(base | gpu_lib.RHS_LEARNED_H)   # one more learned RHS at the current positions
+        f = {k: ctx.download(k) for k in ("x", "y", "z", "mass", "sml", "rho")}
+        nnl, noi = ctx.neighbour_lists(p.max_nb)
+    print(f"loss {losses[0]:.3g} -> {losses[-1]:.3g}; mean h-counts plain {counts['plain']:.4f}, "
+          f"learned {counts['learned']:.4f}")
+    assert losses[-1] < 0.1 * losses[0]
+    assert counts["learned"] < counts["plain"] - 0.05
+    st_o, rho_o = orc.density(f["x"], f["y"], f["z"], f["mass"], f["sml"], nnl, noi)
+    assert st_o == 0
+    assert np.max(np.abs(f["rho"] - rho_o) / rho_o) <= 1e-12
