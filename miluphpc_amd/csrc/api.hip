@@ -106,7 +106,7 @@ static void free_all(mhpc_ctx *c) {
                     c->sr4, c->sh, c->sh2, c->se, c->srho, c->sp, c->scs, c->snoi, c->siters, c->sdone,
                     c->lcp, c->tcnt, c->toff,
                     c->level_count, c->cstart, c->cend, c->clevel, c->cchild,
-                    c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax,
+                    c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin, c->cbmax, c->chmax,
                     c->nbr, c->htable, c->bbox_partial, c->box, c->dstatus,
                     c->diag_partial, c->diag_out, c->pc_backup, c->mlp_params, c->mlp_m1, c->mlp_m2,
                     c->mlp_grad, c->mlp_norm, c->mlp_x, c->mlp_t, c->mlp_hprev, c->mlp_idx,
@@ -209,6 +209,7 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(dalloc(&c->cquadf, 2 * cap_cells));
     HIPC(dalloc(&c->cbmin, 3 * cap_cells));
     HIPC(dalloc(&c->cbmax, 3 * cap_cells));
+    HIPC(dalloc(&c->chmax, cap_cells));
     HIPC(dalloc(&c->htable, MHPC_HTABLE_MAX + 1));
     upload_htable(c);
     HIPC(dalloc(&c->bbox_partial, 2 * 1024));
@@ -489,7 +490,7 @@ extern "C" int mhpc_build_tree(mhpc_ctx *c, float *ms) {
     int st = run_stage(c, ms, [&] {
         launch_gather(c, true, true, false);
         launch_tree(c);
-        launch_bottom_up(c, false);
+        launch_bottom_up(c, false, true);
         return (int)MHPC_OK;
     });
     c->tree_valid = st == MHPC_OK;
@@ -500,7 +501,7 @@ extern "C" int mhpc_centres_of_mass(mhpc_ctx *c, float *ms) {
     if (!c) return MHPC_E_ARG;
     if (!c->tree_valid) return MHPC_E_STATE;
     int st = run_stage(c, ms, [&] {
-        launch_bottom_up(c, true);
+        launch_bottom_up(c, true, true);
         return (int)MHPC_OK;
     });
     c->com_valid = st == MHPC_OK;
@@ -979,7 +980,7 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     mark(4);
     launch_tree(c);
     mark(5);
-    launch_bottom_up(c, grav);
+    launch_bottom_up(c, grav, sph);
     mark(6);
     if (grav) launch_gravity(c, pot);
     mark(7);

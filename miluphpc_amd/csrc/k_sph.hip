@@ -96,7 +96,28 @@ struct NNArgs {
     uint32_t *out_count;
     uint32_t *err;
     unsigned long long *counters;
+    const double *chmax;      // per cell: an upper bound of its particles' h
 };
+
+// The positions a search is for: one particle's, or the box of a group's (active) particles.
+struct NNCore {
+    double x0, x1, y0, y1, z0, z1;
+};
+__device__ __forceinline__ NNCore nn_point(double x, double y, double z) { return NNCore{x, x, y, y, z, z}; }
+
+// Can cell c hold a neighbour j of some particle i of the core?  A pair needs d < h_j^2, and
+// the squared distance from the core to the cell's particle box, taken per axis as
+// max(0, lo_cell - hi_core, lo_core - hi_cell) and summed in the pair test's order, is at most
+// every such pair's d (the roundings are monotone), while h_j^2 is at most chmax^2: a cell that
+// fails d2 < chmax^2 holds no neighbour of the core and its subtree is skipped.
+__device__ __forceinline__ bool nn_cell_reaches(const NNArgs &A, const NNCore &q, int64_t c) {
+    const int64_t cc = A.cap_cells;
+    const double ex = fmax(fmax(A.cbmin[c] - q.x1, q.x0 - A.cbmax[c]), 0.0);
+    const double ey = fmax(fmax(A.cbmin[cc + c] - q.y1, q.y0 - A.cbmax[cc + c]), 0.0);
+    const double ez = fmax(fmax(A.cbmin[2 * cc + c] - q.z1, q.z0 - A.cbmax[2 * cc + c]), 0.0);
+    const double hm = A.chmax[c];
+    return ex * ex + ey * ey + ez * ez < hm * hm;
+}
 
 // Level-K index along one axis of the cell containing v: the keys' bisection of [mn, mx]
 // (lower half = key bit 1), counted upwards, so it is monotone in v.
@@ -177,8 +198,8 @@ __device__ __forceinline__ bool nn_win_clip(int w0, int w1, int &rs, int &rc) {
 // clipped against it.  Returns the stack count; nr receives the range count.
 template <typename LdsT>
 __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
-                                             double hiy, double loz, double hiz, int part, int nparts, int w0,
-                                             int w1, int g0, int &nr) {
+                                             double hiy, double loz, double hiz, const NNCore &q, int part,
+                                             int nparts, int w0, int w1, int g0, int &nr) {
     const int lane = wave_lane();
     int node = -1;
     int2 rng = make_int2(0, 0x7fffffff);
@@ -203,6 +224,7 @@ __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lo
             if (node >= 0) rng = A.srng[nn_tab_off(K) + (int64_t)P];
         }
     }
+    if (node >= A.n && !nn_cell_reaches(A, q, node - A.n)) node = -1;
     if (node >= 0 && !nn_win_clip(w0, w1, rng.x, rng.y)) node = -1;
     int r0 = 0;   // the window's ranges: own group, then the window before and after it
     if (w1 > w0) {
@@ -248,8 +270,8 @@ __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lo
 // (counted over the whole box, so every part of a split search decides alike).
 template <typename LdsT, typename VisitFn>
 __device__ __forceinline__ bool nn_direct(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
-                                          double hiy, double loz, double hiz, int part, int nparts,
-                                          VisitFn &&visit) {
+                                          double hiy, double loz, double hiz, const NNCore &q, int part,
+                                          int nparts, VisitFn &&visit) {
     const int lane = wave_lane();
     const uint64_t lt = lanemask_lt();
     const NNStart S = nn_start_level<NN_DSPAN>(A, lox, hix, loy, hiy, loz, hiz);
@@ -271,7 +293,8 @@ __device__ __forceinline__ bool nn_direct(LdsT &L, const NNArgs &A, double lox, 
                 P = (P << 3) | d;
             }
             const int64_t e = nn_tab_off(K) + (int64_t)P;
-            if (A.stab[e] >= 0) {
+            const int node = A.stab[e];
+            if (node >= 0 && (node < A.n || nn_cell_reaches(A, q, node - A.n))) {
                 const int2 r = A.srng[e];
                 first = r.x;
                 num = r.y;
@@ -328,23 +351,24 @@ __device__ __forceinline__ bool nn_direct(LdsT &L, const NNArgs &A, double lox, 
 }
 
 // Walk the (sub)tree of node `start` (the root is node n) for the axis-aligned box [lo, hi]
-// and hand the candidate particles to
-// `tile` in LDS tiles of up to 64 (L.tx/ty/tz/th2/tj, count m).  Leaves of at most
+// (the core q grown by the search radius) and hand the candidate particles to
+// `tile` in LDS tiles of up to 64 (L.tx/ty/tz/th2/tj, count m).  Cells that cannot hold a
+// neighbour of the core (nn_cell_reaches) are skipped with their subtrees.  Leaves of at most
 // NN_LEAF particles and single particles are taken as ranges of sorted positions; larger
 // cells are opened.  With FILTER, only candidates with keep(x, y, z, h^2) true reach the
 // tiles (compacted in walk order, so the tile sequence is the unfiltered one minus the
 // dropped candidates).  Returns false on stack overflow.  Wave-uniform control flow.
 template <bool FILTER, typename LdsT, typename KeepFn, typename TileFn>
 __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
-                                            double hiy, double loz, double hiz, int start, int w0, int w1,
-                                            int g0, KeepFn &&keep, TileFn &&tile) {
+                                            double hiy, double loz, double hiz, const NNCore &q, int start,
+                                            int w0, int w1, int g0, KeepFn &&keep, TileFn &&tile) {
     const int lane = wave_lane();
     const int64_t n = A.n;
     int top = 1, nr = 0;
     if (start >= 0) {
         if (lane == 0) L.stack[0] = start;
     } else {
-        top = nn_start_push(L, A, lox, hix, loy, hiy, loz, hiz, start == NN_START_ALL ? 0 : NN_START_PART0 - start,
+        top = nn_start_push(L, A, lox, hix, loy, hiy, loz, hiz, q, start == NN_START_ALL ? 0 : NN_START_PART0 - start,
                             start == NN_START_ALL ? 1 : 8, w0, w1, g0, nr);
     }
     wave_sync();
@@ -375,7 +399,8 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
                 const int64_t cc = A.cap_cells;
                 const bool overlap = A.cbmin[c] <= hix && A.cbmax[c] >= lox &&
                                      A.cbmin[cc + c] <= hiy && A.cbmax[cc + c] >= loy &&
-                                     A.cbmin[2 * cc + c] <= hiz && A.cbmax[2 * cc + c] >= loz;
+                                     A.cbmin[2 * cc + c] <= hiz && A.cbmax[2 * cc + c] >= loz &&
+                                     nn_cell_reaches(A, q, c);
                 if (overlap) {
                     rs = A.cstart[c];
                     rc = A.cend[c] - rs + 1;
@@ -626,7 +651,8 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     // with the window on, the group's own particles are the first candidates streamed, so all of
     // them that pass the filter are in the first tile: only its pair tests must skip j == k
     bool first_tile = true;
-    const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, w0, w1,
+    const NNCore core = {bx0, bx1, by0, by1, bz0, bz1};
+    const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, core, NN_START_ALL, w0, w1,
                                       (int)(group * 64), keep, [&](int m) {
         if (!active) return;
         if (MODE == 1) {
@@ -797,10 +823,10 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
         };
         const double lox = own.x - grow, hix = own.x + grow, loy = own.y - grow, hiy = own.y + grow;
         const double loz = own.z - grow, hiz = own.z + grow;
-        const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, 0, 1,
+        const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z), 0, 1,
                                       [&](bool v, int j, const double4 &p, double h) { take(v, j, p.x, p.y, p.z, h * h); });
         if (!direct) {
-            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, 0, 0, -1, KeepAll(), [&](int m) {
+            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z), NN_START_ALL, 0, 0, -1, KeepAll(), [&](int m) {
                 const bool v = lane < m;
                 take(v, v ? L.tj[lane] : -1, v ? L.tx[lane] : 0.0, v ? L.ty[lane] : 0.0, v ? L.tz[lane] : 0.0,
                      v ? L.th2[lane] : 0.0);
@@ -870,10 +896,10 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
         };
         const double lox = own.x - grow, hix = own.x + grow, loy = own.y - grow, hiy = own.y + grow;
         const double loz = own.z - grow, hiz = own.z + grow;
-        const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, 0, 1,
+        const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z), 0, 1,
                                       [&](bool v, int i, const double4 &p, double h) { mark(v, i, p.x, p.y, p.z, h * h); });
         if (!direct) {
-            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, NN_START_ALL, 0, 0, -1, KeepAll(), [&](int m) {
+            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z), NN_START_ALL, 0, 0, -1, KeepAll(), [&](int m) {
                 if (lane < m) mark(true, L.tj[lane], L.tx[lane], L.ty[lane], L.tz[lane], L.th2[lane]);
             });
             if (!ok) {
@@ -905,7 +931,8 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_direct(NNArgs A) {
         const double grow = hi * (1.0 + 1e-9);
         int count = 0;
         const bool direct = nn_direct(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                      own.z + grow, 0, 1, [&](bool v, int j, const double4 &p, double h) {
+                                      own.z + grow, nn_point(own.x, own.y, own.z), 0, 1,
+                                      [&](bool v, int j, const double4 &p, double h) {
             if (v) {
                 const double dx = own.x - p.x;
                 const double dy = own.y - p.y;
@@ -947,7 +974,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
         const int start = NN_START_PART0 - o;   // eighth o of the start cells
         int count = 0;
         const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, start, 0, 0, -1, KeepAll(), [&](int m) {
+                                    own.z + grow, nn_point(own.x, own.y, own.z), start, 0, 0, -1, KeepAll(), [&](int m) {
             bool hit = false;
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
@@ -1069,6 +1096,7 @@ static NNArgs nn_common(mhpc_ctx *c) {
     a.max_nb = c->p.max_nb;
     a.err = c->err;
     a.counters = c->counters;
+    a.chmax = c->chmax;
     return a;
 }
 
@@ -1087,6 +1115,32 @@ __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint8_t *dirt
     }
     if (k < nlc) list_counts[k] = 0;
     if (k < 2) ocount[k] = 0;
+}
+
+// Raise the cells' h bounds (chmax) to the new h of the listed particles: every ancestor cell of
+// such a particle whose bound is smaller (an atomic maximum on the bits of the positive double,
+// which orders like the value), stopping at the first ancestor that already bounds it.  Bounds
+// never shrink within an RHS, so they stay upper bounds of the h every later search uses.
+__global__ __launch_bounds__(256) void k_hmax_raise(const int32_t *__restrict__ list, const uint32_t *count,
+                                                    const double *__restrict__ h, const int32_t *__restrict__ ppar,
+                                                    const int32_t *__restrict__ cpar, double *chmax,
+                                                    int64_t cap_cells) {
+    const uint32_t cnt = *count;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t k = list[t];
+        const unsigned long long hb = (unsigned long long)__double_as_longlong(h[k]);
+        int64_t c = ppar[k];
+        while (c >= 0 && c < cap_cells) {
+            const unsigned long long old = atomicMax(reinterpret_cast<unsigned long long *>(chmax + c), hb);
+            if (old >= hb || c == 0) break;
+            c = cpar[c];
+        }
+    }
+}
+
+static void launch_hmax_raise(mhpc_ctx *c, const int32_t *list, const uint32_t *count) {
+    hipLaunchKernelGGL(k_hmax_raise, dim3(256), dim3(256), 0, c->stream, list, count, c->sh, c->ppar, c->cpar,
+                       c->chmax, c->cap_cells);
 }
 
 // a9: the plain h-iteration as Jacobi passes; sh holds the final h afterwards.  Pass 0
@@ -1140,6 +1194,8 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative, hipEvent_t *wal
     a.out_list = nullptr;
     a.h0_slot = nullptr;   // the pass-0 slots stay for the dirty marking
     std::swap(c->sh, c->sh2);
+    // the particles still iterating are the ones whose h changed
+    launch_hmax_raise(c, c->hlist[2], c->hlist_count + 0);
     // passes 1.., per-particle mode over the list of the previous pass
     for (int pass = 1; pass < passes; pass++) {
         a.pass = pass;
@@ -1160,6 +1216,7 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative, hipEvent_t *wal
         a.slot_count = hard_count;
         hipLaunchKernelGGL(k_nn_count_split, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
         hipLaunchKernelGGL(k_h_finish, dim3(256), dim3(256), 0, c->stream, a);
+        launch_hmax_raise(c, a.list_out, a.count_out);
     }
 }
 

@@ -19,8 +19,8 @@
 //             digit, the reference's slot convention).
 // No locks, no retries, and the same input always gives the same numbering.
 // The bottom-up pass then runs level 21..0: mass, centre of mass (children in slot
-// order, FP64, as calculateCentersOfMass, tree.cu:1416-1497), traceless quadrupole and
-// the particles' bounding box per cell.
+// order, FP64, as calculateCentersOfMass, tree.cu:1416-1497), traceless quadrupole, the
+// particles' bounding box and, for SPH, their largest smoothing length per cell.
 #include "mhpc_internal.h"
 
 #define TREE_BLOCK 256
@@ -243,7 +243,8 @@ void launch_tree(mhpc_ctx *c) {
 // ------------------------------------------------------------------------------------
 // bottom-up, one launch per level (cells of a level are contiguous): FP64 mass and
 // centre of mass over the children in slot order (tree.cu:1416-1497), traceless
-// quadrupole about the COM with the parallel-axis shift, and the particles' AABB.
+// quadrupole about the COM with the parallel-axis shift, the particles' AABB and (chmax
+// set) their largest h.
 // Each launch reads its level's range from cell_base on the device.
 // Eight lanes per cell, lane s loading child s (all children's loads in flight at once);
 // the children's terms go through LDS and the cell's first lane sums them in slot order, so
@@ -258,11 +259,12 @@ __global__ __launch_bounds__(BU_BLOCK) void k_bottom_up(
     const double4 *__restrict__ sxm, double *__restrict__ cmass, double *__restrict__ cx,
     double *__restrict__ cy, double *__restrict__ cz, double *__restrict__ cquad,
     float4 *__restrict__ cquadf, double *__restrict__ cbmin, double *__restrict__ cbmax,
-    int64_t cap_cells, int gravity, int quad, unsigned long long *depth_out) {
+    int64_t cap_cells, int gravity, int quad, unsigned long long *depth_out, const double *__restrict__ sh,
+    double *__restrict__ chmax) {
     // a cell's eight lanes are consecutive lanes of one wavefront, so they exchange their
     // terms through LDS with wavefront barriers only; lane s of the cell then sums one
     // quantity over the eight children in slot order (the sequential loop's rounding)
-    __shared__ double term[10][BU_BLOCK];
+    __shared__ double term[11][BU_BLOCK];
     __shared__ double red[4][BU_BLOCK / 8];
     const int tid = threadIdx.x, sub = tid & 7, g = tid >> 3, c8 = tid & ~7;
     float *quadf = reinterpret_cast<float *>(cquadf);   // 8 floats per cell
@@ -284,13 +286,16 @@ __global__ __launch_bounds__(BU_BLOCK) void k_bottom_up(
         double px = 0.0, py = 0.0, pz = 0.0, pm = 0.0;
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
         double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double hk = 0.0;   // the child's h (particle) or largest h (cell)
         if (k >= 0 && k < n) {
             const double4 r = sxm[k];
             px = r.x; py = r.y; pz = r.z; pm = r.w;
             lo[0] = hi[0] = px; lo[1] = hi[1] = py; lo[2] = hi[2] = pz;
+            if (chmax) hk = sh[k];
         } else if (k >= n) {
             const int64_t cc = k - n;
             px = cx[cc]; py = cy[cc]; pz = cz[cc]; pm = cmass[cc];
+            if (chmax) hk = chmax[cc];
             lo[0] = cbmin[cc]; lo[1] = cbmin[cap_cells + cc]; lo[2] = cbmin[2 * cap_cells + cc];
             hi[0] = cbmax[cc]; hi[1] = cbmax[cap_cells + cc]; hi[2] = cbmax[2 * cap_cells + cc];
             if (quad) {
@@ -307,6 +312,7 @@ __global__ __launch_bounds__(BU_BLOCK) void k_bottom_up(
             term[4 + a][tid] = lo[a];
             term[7 + a][tid] = hi[a];
         }
+        term[10][tid] = hk;
         wave_sync();
         if (valid) {
             if (sub < 4) {
@@ -325,6 +331,11 @@ __global__ __launch_bounds__(BU_BLOCK) void k_bottom_up(
                 }
                 cbmin[a * cap_cells + c] = bmin;
                 cbmax[a * cap_cells + c] = bmax;
+            } else if (chmax) {
+                double hm = 0.0;
+#pragma unroll
+                for (int s = 0; s < 8; s++) hm = fmax(hm, term[10][c8 + s]);
+                chmax[c] = hm;
             }
         }
         wave_sync();
@@ -375,7 +386,7 @@ __global__ __launch_bounds__(BU_BLOCK) void k_bottom_up(
     if (L_hi != L_lo) __syncthreads();
     }
 }
-void launch_bottom_up(mhpc_ctx *c, bool gravity) {
+void launch_bottom_up(mhpc_ctx *c, bool gravity, bool sph) {
     if (c->n <= 0) return;
     // a level holds fewer cells than particles; eight threads per cell, grid-stride beyond
     int grid = (int)ceil_div(c->n * 8, BU_BLOCK * 4);
@@ -385,7 +396,8 @@ void launch_bottom_up(mhpc_ctx *c, bool gravity) {
     auto run = [&](int hi, int lo, int g, unsigned long long *depth_out) {
         hipLaunchKernelGGL(k_bottom_up, dim3(g), dim3(BU_BLOCK), 0, c->stream, hi, lo, c->n, c->cell_base,
                            c->cchild, c->sxm, c->cmass, c->cx, c->cy, c->cz, c->cquad, c->cquadf, c->cbmin,
-                           c->cbmax, c->cap_cells, gravity ? 1 : 0, quad, depth_out);
+                           c->cbmax, c->cap_cells, gravity ? 1 : 0, quad, depth_out, sph ? c->sh : nullptr,
+                           sph ? c->chmax : nullptr);
     };
     // levels below the last call's deepest (normally empty) in one single-workgroup sweep,
     // then one launch per level down to 4, then levels 3..0 (<= 585 cells) in one sweep

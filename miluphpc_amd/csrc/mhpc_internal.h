@@ -114,6 +114,8 @@ struct mhpc_ctx {
     double *cquad = nullptr;                     // [6][cap_cells]
     float4 *cquadf = nullptr;                    // packed FP32 quadrupole (xx,xy,xz,yy),(yz,zz,-,-)
     double *cbmin = nullptr, *cbmax = nullptr;   // [3][cap_cells] particle AABB per cell
+    double *chmax = nullptr;                     // [cap_cells] an upper bound of the particles' h
+                                                 // per cell (neighbour searches prune with it)
 
     // neighbours
     int32_t *nbr = nullptr;                      // [max_nb][cap_pad]
@@ -187,7 +189,8 @@ void launch_keys(mhpc_ctx *c);
 void launch_sort(mhpc_ctx *c);
 void launch_gather(mhpc_ctx *c, bool gravity, bool sph, bool rhs);
 void launch_tree(mhpc_ctx *c);
-void launch_bottom_up(mhpc_ctx *c, bool gravity);
+// sph: also the cells' largest h from the key-sorted h (sh must hold the gathered h)
+void launch_bottom_up(mhpc_ctx *c, bool gravity, bool sph);
 void launch_gravity(mhpc_ctx *c, bool potential);
 // walk_events (may be null): two events recorded around the pass-0 group walk kernel
 void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative, hipEvent_t *walk_events = nullptr);
