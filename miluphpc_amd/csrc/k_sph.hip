@@ -99,23 +99,28 @@ struct NNArgs {
     const double *chmax;      // per cell: an upper bound of its particles' h
 };
 
-// The positions a search is for: one particle's, or the box of a group's (active) particles.
+// The positions a search is for -- one particle's, or the box of a group's (active) particles --
+// and the largest h among them (the search radius).
 struct NNCore {
     double x0, x1, y0, y1, z0, z1;
+    double h;
 };
-__device__ __forceinline__ NNCore nn_point(double x, double y, double z) { return NNCore{x, x, y, y, z, z}; }
+__device__ __forceinline__ NNCore nn_point(double x, double y, double z, double h) {
+    return NNCore{x, x, y, y, z, z, h};
+}
 
-// Can cell c hold a neighbour j of some particle i of the core?  A pair needs d < h_j^2, and
-// the squared distance from the core to the cell's particle box, taken per axis as
-// max(0, lo_cell - hi_core, lo_core - hi_cell) and summed in the pair test's order, is at most
-// every such pair's d (the roundings are monotone), while h_j^2 is at most chmax^2: a cell that
-// fails d2 < chmax^2 holds no neighbour of the core and its subtree is skipped.
+// Can cell c hold a neighbour j of some particle i of the core?  A pair needs d < h_i^2 and
+// d < h_j^2, and the squared distance from the core to the cell's particle box, taken per axis
+// as max(0, lo_cell - hi_core, lo_core - hi_cell) and summed in the pair test's order, is at
+// most every such pair's d (the roundings are monotone), while h_i <= q.h and h_j <= chmax: a
+// cell that fails d2 < min(q.h, chmax)^2 holds no neighbour of the core and its subtree is
+// skipped.
 __device__ __forceinline__ bool nn_cell_reaches(const NNArgs &A, const NNCore &q, int64_t c) {
     const int64_t cc = A.cap_cells;
     const double ex = fmax(fmax(A.cbmin[c] - q.x1, q.x0 - A.cbmax[c]), 0.0);
     const double ey = fmax(fmax(A.cbmin[cc + c] - q.y1, q.y0 - A.cbmax[cc + c]), 0.0);
     const double ez = fmax(fmax(A.cbmin[2 * cc + c] - q.z1, q.z0 - A.cbmax[2 * cc + c]), 0.0);
-    const double hm = A.chmax[c];
+    const double hm = fmin(A.chmax[c], q.h);
     return ex * ex + ey * ey + ez * ez < hm * hm;
 }
 
@@ -651,7 +656,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     // with the window on, the group's own particles are the first candidates streamed, so all of
     // them that pass the filter are in the first tile: only its pair tests must skip j == k
     bool first_tile = true;
-    const NNCore core = {bx0, bx1, by0, by1, bz0, bz1};
+    const NNCore core = {bx0, bx1, by0, by1, bz0, bz1, hmax};
     const bool ok = nn_walk_box<true>(L, A, lox, hix, loy, hiy, loz, hiz, core, NN_START_ALL, w0, w1,
                                       (int)(group * 64), keep, [&](int m) {
         if (!active) return;
@@ -823,10 +828,10 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
         };
         const double lox = own.x - grow, hix = own.x + grow, loy = own.y - grow, hiy = own.y + grow;
         const double loz = own.z - grow, hiz = own.z + grow;
-        const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z), 0, 1,
+        const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z, hi), 0, 1,
                                       [&](bool v, int j, const double4 &p, double h) { take(v, j, p.x, p.y, p.z, h * h); });
         if (!direct) {
-            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z), NN_START_ALL, 0, 0, -1, KeepAll(), [&](int m) {
+            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z, hi), NN_START_ALL, 0, 0, -1, KeepAll(), [&](int m) {
                 const bool v = lane < m;
                 take(v, v ? L.tj[lane] : -1, v ? L.tx[lane] : 0.0, v ? L.ty[lane] : 0.0, v ? L.tz[lane] : 0.0,
                      v ? L.th2[lane] : 0.0);
@@ -883,7 +888,8 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
         const double4 own = A.sxm[jc];
         const double h0 = A.h0_slot[t], h1 = A.h[jc];
         const double h02 = h0 * h0, h12 = h1 * h1;
-        const double grow = (h0 > h1 ? h0 : h1) * (1.0 + 1e-9);
+        const double hq = h0 > h1 ? h0 : h1;
+        const double grow = hq * (1.0 + 1e-9);
         if (lane == 0) A.dirty[jc] = 1;
         auto mark = [&](bool v, int i, double x, double y, double z, double hi2) {
             if (v) {
@@ -896,10 +902,10 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
         };
         const double lox = own.x - grow, hix = own.x + grow, loy = own.y - grow, hiy = own.y + grow;
         const double loz = own.z - grow, hiz = own.z + grow;
-        const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z), 0, 1,
+        const bool direct = nn_direct(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z, hq), 0, 1,
                                       [&](bool v, int i, const double4 &p, double h) { mark(v, i, p.x, p.y, p.z, h * h); });
         if (!direct) {
-            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z), NN_START_ALL, 0, 0, -1, KeepAll(), [&](int m) {
+            const bool ok = nn_walk_box<false>(L, A, lox, hix, loy, hiy, loz, hiz, nn_point(own.x, own.y, own.z, hq), NN_START_ALL, 0, 0, -1, KeepAll(), [&](int m) {
                 if (lane < m) mark(true, L.tj[lane], L.tx[lane], L.ty[lane], L.tz[lane], L.th2[lane]);
             });
             if (!ok) {
@@ -931,7 +937,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_direct(NNArgs A) {
         const double grow = hi * (1.0 + 1e-9);
         int count = 0;
         const bool direct = nn_direct(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                      own.z + grow, nn_point(own.x, own.y, own.z), 0, 1,
+                                      own.z + grow, nn_point(own.x, own.y, own.z, hi), 0, 1,
                                       [&](bool v, int j, const double4 &p, double h) {
             if (v) {
                 const double dx = own.x - p.x;
@@ -974,7 +980,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
         const int start = NN_START_PART0 - o;   // eighth o of the start cells
         int count = 0;
         const bool ok = nn_walk_box<false>(L, A, own.x - grow, own.x + grow, own.y - grow, own.y + grow, own.z - grow,
-                                    own.z + grow, nn_point(own.x, own.y, own.z), start, 0, 0, -1, KeepAll(), [&](int m) {
+                                    own.z + grow, nn_point(own.x, own.y, own.z, hi), start, 0, 0, -1, KeepAll(), [&](int m) {
             bool hit = false;
             if (lane < m) {
                 const double dx = own.x - L.tx[lane];
