@@ -130,12 +130,27 @@ __global__ __launch_bounds__(256) void k_time_step_partial(int64_t n, const doub
     }
 }
 
-__global__ void k_time_step_final(const double *__restrict__ partial, int nb, double *__restrict__ out) {
-    if (threadIdx.x >= 5) return;
-    double v = partial[threadIdx.x];
-    for (int b = 1; b < nb; b++)
-        v = threadIdx.x < 3 ? fmin(v, partial[5 * b + threadIdx.x]) : fmax(v, partial[5 * b + threadIdx.x]);
-    out[threadIdx.x] = v;
+// one wavefront folds the block partials: each lane a strided share, then the wave (minima and
+// maxima are exact, so the result does not depend on the order)
+__global__ __launch_bounds__(64) void k_time_step_final(const double *__restrict__ partial, int nb,
+                                                        double *__restrict__ out) {
+    const int l = threadIdx.x;
+    double v0 = 1.7976931348623157e308, v1 = v0, v2 = v0, v3 = 0.0, v4 = 0.0;
+    for (int b = l; b < nb; b += 64) {
+        v0 = fmin(v0, partial[5 * b]);
+        v1 = fmin(v1, partial[5 * b + 1]);
+        v2 = fmin(v2, partial[5 * b + 2]);
+        v3 = fmax(v3, partial[5 * b + 3]);
+        v4 = fmax(v4, partial[5 * b + 4]);
+    }
+    v0 = wave_min(v0);
+    v1 = wave_min(v1);
+    v2 = wave_min(v2);
+    v3 = wave_max(v3);
+    v4 = wave_max(v4);
+    if (l == 0) {
+        out[0] = v0; out[1] = v1; out[2] = v2; out[3] = v3; out[4] = v4;
+    }
 }
 
 static PCArgs pc_args(mhpc_ctx *c, double dt) {

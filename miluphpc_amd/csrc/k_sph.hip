@@ -97,6 +97,9 @@ struct NNArgs {
     uint32_t *err;
     unsigned long long *counters;
     const double *chmax;      // per cell: an upper bound of its particles' h
+    // h-iteration passes: the bounds the new h of the particles still iterating raise
+    double *chmax_raise;
+    const int32_t *ppar, *cpar;
 };
 
 // The positions a search is for -- one particle's, or the box of a group's (active) particles --
@@ -122,6 +125,22 @@ __device__ __forceinline__ bool nn_cell_reaches(const NNArgs &A, const NNCore &q
     const double ez = fmax(fmax(A.cbmin[2 * cc + c] - q.z1, q.z0 - A.cbmax[2 * cc + c]), 0.0);
     const double hm = fmin(A.chmax[c], q.h);
     return ex * ex + ey * ey + ez * ez < hm * hm;
+}
+
+// Raise the cells' h bounds (chmax) to a particle's new h: every ancestor cell whose bound is
+// smaller (an atomic maximum on the bits of the positive double, which orders like the value),
+// stopping at the first ancestor that already bounds it.  Bounds never shrink within an RHS, so
+// they stay upper bounds of the h every later search uses.
+__device__ __forceinline__ void hmax_raise(double *chmax, const int32_t *__restrict__ ppar,
+                                           const int32_t *__restrict__ cpar, int64_t cap_cells, int64_t k,
+                                           double h) {
+    const unsigned long long hb = (unsigned long long)__double_as_longlong(h);
+    int64_t c = ppar[k];
+    while (c >= 0 && c < cap_cells) {
+        const unsigned long long old = atomicMax(reinterpret_cast<unsigned long long *>(chmax + c), hb);
+        if (old >= hb || c == 0) break;
+        c = cpar[c];
+    }
 }
 
 // Level-K index along one axis of the cell containing v: the keys' bisection of [mn, mx]
@@ -1023,6 +1042,8 @@ __global__ __launch_bounds__(256) void k_h_finish(NNArgs A) {
             A.h_next[k] = hn;
             if (fin) A.done[k] = 1;
             stay = !fin;
+            // the next pass counts with hn: the cell bounds must cover it
+            if (stay && A.chmax_raise) hmax_raise(A.chmax_raise, A.ppar, A.cpar, A.cap_cells, k, hn);
         }
         // wave-aggregated list append and counter (list order is free)
         const uint64_t mS = wave_ballot(stay);
@@ -1123,10 +1144,8 @@ __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint8_t *dirt
     if (k < 2) ocount[k] = 0;
 }
 
-// Raise the cells' h bounds (chmax) to the new h of the listed particles: every ancestor cell of
-// such a particle whose bound is smaller (an atomic maximum on the bits of the positive double,
-// which orders like the value), stopping at the first ancestor that already bounds it.  Bounds
-// never shrink within an RHS, so they stay upper bounds of the h every later search uses.
+// the raise for the listed particles (after pass 0: the particles still iterating are the ones
+// whose h changed)
 __global__ __launch_bounds__(256) void k_hmax_raise(const int32_t *__restrict__ list, const uint32_t *count,
                                                     const double *__restrict__ h, const int32_t *__restrict__ ppar,
                                                     const int32_t *__restrict__ cpar, double *chmax,
@@ -1134,13 +1153,7 @@ __global__ __launch_bounds__(256) void k_hmax_raise(const int32_t *__restrict__ 
     const uint32_t cnt = *count;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
         const int32_t k = list[t];
-        const unsigned long long hb = (unsigned long long)__double_as_longlong(h[k]);
-        int64_t c = ppar[k];
-        while (c >= 0 && c < cap_cells) {
-            const unsigned long long old = atomicMax(reinterpret_cast<unsigned long long *>(chmax + c), hb);
-            if (old >= hb || c == 0) break;
-            c = cpar[c];
-        }
+        hmax_raise(chmax, ppar, cpar, cap_cells, k, h[k]);
     }
 }
 
@@ -1221,8 +1234,11 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative, hipEvent_t *wal
         a.slot_list = c->hhard;
         a.slot_count = hard_count;
         hipLaunchKernelGGL(k_nn_count_split, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
-        hipLaunchKernelGGL(k_h_finish, dim3(256), dim3(256), 0, c->stream, a);
-        launch_hmax_raise(c, a.list_out, a.count_out);
+        NNArgs f = a;
+        f.chmax_raise = c->chmax;
+        f.ppar = c->ppar;
+        f.cpar = c->cpar;
+        hipLaunchKernelGGL(k_h_finish, dim3(256), dim3(256), 0, c->stream, f);
     }
 }
 

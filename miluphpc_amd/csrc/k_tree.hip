@@ -400,11 +400,11 @@ void launch_bottom_up(mhpc_ctx *c, bool gravity, bool sph) {
                            sph ? c->chmax : nullptr);
     };
     // levels below the last call's deepest (normally empty) in one single-workgroup sweep,
-    // then one launch per level down to 4, then levels 3..0 (<= 585 cells) in one sweep
+    // then one launch per level down to 3, then levels 2..0 (<= 73 cells) in one sweep
     int deep = c->com_depth_hint;
-    if (deep < 4) deep = 4;
+    if (deep < 3) deep = 3;
     if (deep > MHPC_LEVELS) deep = MHPC_LEVELS;
     if (deep < MHPC_LEVELS) run(MHPC_LEVELS, deep + 1, 1, nullptr);
-    for (int L = deep; L >= 4; L--) run(L, L, grid, nullptr);
-    run(3, 0, 1, c->counters + CNT_TREE_DEPTH);
+    for (int L = deep; L >= 3; L--) run(L, L, grid, nullptr);
+    run(2, 0, 1, c->counters + CNT_TREE_DEPTH);
 }
