@@ -31,6 +31,11 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 #define MLP_IN 8      // 7 features + 1 zero pad
 #define MLP_H 64
 #define MLP_KPAD 16   // layer-1 K padded to one MFMA step
+// LDS row strides (BF16 elements): four more than the row length, so that the 16 rows one MFMA
+// operand read touches fall in different banks (with the bare length every row starts in the
+// same bank: 16-way conflicts)
+#define MLP_KLD (MLP_KPAD + 4)
+#define MLP_HLD (MLP_H + 4)
 #define MLP_NPARAM (MLP_H * MLP_IN + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H + 1)   // 4801
 
 // parameter offsets in the flat FP32 vector
@@ -47,8 +52,8 @@ int mlp_num_params() { return MLP_NPARAM; }
 
 // weights as the kernels stage them: W1 [64][16] bf16 (cols >= 8 zero), W2 [64][64] bf16
 struct alignas(16) MlpLds {
-    unsigned short w1[MLP_H][MLP_KPAD];
-    unsigned short w2[MLP_H][MLP_H];
+    unsigned short w1[MLP_H][MLP_KLD];
+    unsigned short w2[MLP_H][MLP_HLD];
     float b1[MLP_H], b2[MLP_H], w3[MLP_H];
     float b3;
 };
@@ -69,9 +74,9 @@ __device__ void load_weights(MlpLds &W, const float *__restrict__ p) {
 
 // per-wave activations
 struct alignas(16) MlpWave {
-    unsigned short x[64][MLP_KPAD];
-    unsigned short a1[64][MLP_H];
-    unsigned short a2[64][MLP_H];
+    unsigned short x[64][MLP_KLD];
+    unsigned short a1[64][MLP_HLD];
+    unsigned short a2[64][MLP_HLD];
 };
 
 // tanh(x) = 1 - 2 / (exp(2x) + 1) from the hardware exp2 and reciprocal estimates: absolute
@@ -87,7 +92,7 @@ __device__ __forceinline__ float tanh_fast(float x) {
 template <int K, int LDA>
 __device__ __forceinline__ void dense_tanh(const unsigned short (*Ain)[LDA],
                                            const unsigned short (*Wt)[LDA], const float *bias,
-                                           unsigned short (*Aout)[MLP_H]) {
+                                           unsigned short (*Aout)[MLP_HLD]) {
     const int lane = wave_lane(), r16 = lane & 15, q = lane >> 4;
     for (int mt = 0; mt < 4; mt++) {
         for (int nt = 0; nt < 4; nt++) {
@@ -111,9 +116,9 @@ __device__ __forceinline__ void dense_tanh(const unsigned short (*Ain)[LDA],
 }
 
 __device__ __forceinline__ float mlp_forward_wave(const MlpLds &W, MlpWave &S) {
-    dense_tanh<MLP_KPAD, MLP_KPAD>(S.x, W.w1, W.b1, S.a1);
+    dense_tanh<MLP_KPAD, MLP_KLD>(S.x, W.w1, W.b1, S.a1);
     wave_sync();
-    dense_tanh<MLP_H, MLP_H>(S.a1, W.w2, W.b2, S.a2);
+    dense_tanh<MLP_H, MLP_HLD>(S.a1, W.w2, W.b2, S.a2);
     wave_sync();
     const int lane = wave_lane();
     float y = W.b3;
@@ -178,8 +183,11 @@ void launch_mlp_renorm(mhpc_ctx *c) {
                        c->mlp_norm, c->mlp_x);
 }
 
-// inference: h = h_prev * exp(y * sigma_t + mu_t), clamped to [1/4, 4] x h_prev
+// inference: h = h_prev * exp(y * sigma_t + mu_t), clamped to [1/4, 4] x h_prev.  A bounded
+// grid whose wavefronts loop over 64-sample tiles: each block converts the weights into LDS once
+// (4,801 loads and BF16 conversions), not once per 256 samples.
 #define MLP_INF_WAVES 4
+#define MLP_INF_BLOCKS 1024
 __global__ __launch_bounds__(64 * MLP_INF_WAVES) void k_mlp_predict(int64_t n, const float *__restrict__ X,
                                                                    const float *__restrict__ params,
                                                                    const float *__restrict__ norm,
@@ -191,18 +199,20 @@ __global__ __launch_bounds__(64 * MLP_INF_WAVES) void k_mlp_predict(int64_t n, c
     __syncthreads();
     const int w = threadIdx.x / 64, lane = wave_lane();
     MlpWave &S = Sw[w];
-    const int64_t base = ((int64_t)blockIdx.x * MLP_INF_WAVES + w) * 64;
-    if (base >= n) return;
-    const int64_t i = base + lane;
-    const bool valid = i < n;
+    for (int64_t base = ((int64_t)blockIdx.x * MLP_INF_WAVES + w) * 64; base < n;
+         base += (int64_t)gridDim.x * MLP_INF_WAVES * 64) {
+        const int64_t i = base + lane;
+        const bool valid = i < n;
 #pragma unroll
-    for (int k = 0; k < MLP_KPAD; k++) S.x[lane][k] = (valid && k < MLP_IN) ? f2bf(X[i * MLP_IN + k]) : 0;
-    wave_sync();
-    const float y = mlp_forward_wave(W, S);
-    if (valid) {
-        double t = (double)y * norm[16 + 1] + norm[16];
-        t = fmin(fmax(t, -1.3862943611198906), 1.3862943611198906);   // ln 4
-        h_out[i] = hprev[i] * exp(t);
+        for (int k = 0; k < MLP_KPAD; k++) S.x[lane][k] = (valid && k < MLP_IN) ? f2bf(X[i * MLP_IN + k]) : 0;
+        wave_sync();
+        const float y = mlp_forward_wave(W, S);
+        if (valid) {
+            double t = (double)y * norm[16 + 1] + norm[16];
+            t = fmin(fmax(t, -1.3862943611198906), 1.3862943611198906);   // ln 4
+            h_out[i] = hprev[i] * exp(t);
+        }
+        wave_sync();   // the tile's activations are read before the next tile overwrites them
     }
 }
 
@@ -232,8 +242,8 @@ __global__ __launch_bounds__(64 * MLP_INF_WAVES) void k_mlp_forward_raw(int64_t 
 #define MLP_TR_WAVES 2
 struct MlpTrainWave {
     MlpWave f;
-    unsigned short dz2[64][MLP_H];
-    unsigned short dz1[64][MLP_H];
+    unsigned short dz2[64][MLP_HLD];
+    unsigned short dz1[64][MLP_HLD];
     float dy[64];
 };
 
@@ -413,7 +423,7 @@ void launch_mlp_features(mhpc_ctx *c, double dt) {
 
 void launch_mlp_predict(mhpc_ctx *c) {
     if (c->n <= 0) return;
-    const unsigned g = (unsigned)ceil_div(c->n, 64 * MLP_INF_WAVES);
+    const unsigned g = (unsigned)std::min<int64_t>(ceil_div(c->n, 64 * MLP_INF_WAVES), MLP_INF_BLOCKS);
     hipLaunchKernelGGL(k_mlp_predict, dim3(g), dim3(64 * MLP_INF_WAVES), 0, c->stream, c->n, c->mlp_x,
                        c->mlp_params, c->mlp_norm, c->mlp_hprev, c->f[MHPC_F_SML]);
 }
