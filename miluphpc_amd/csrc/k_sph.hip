@@ -100,7 +100,18 @@ struct NNArgs {
     // h-iteration passes: the bounds the new h of the particles still iterating raise
     double *chmax_raise;
     const int32_t *ppar, *cpar;
+    // tail passes: set when this pass changed the h of a particle that keeps iterating; a pass
+    // whose previous pass changed none (prev_changed set and zero) repeats that pass's counts
+    uint32_t *changed;
+    const uint32_t *prev_changed;
 };
+
+// A tail pass after a pass that changed no h sees the same h everywhere as that pass (the
+// particles that finished kept theirs), so every count it would take equals the count the
+// previous pass stored in snoi: the counting kernels skip it and k_h_finish reuses snoi.
+__device__ __forceinline__ bool h_pass_frozen(const NNArgs &A) {
+    return A.prev_changed && *A.prev_changed == 0u;
+}
 
 // The positions a search is for -- one particle's, or the box of a group's (active) particles --
 // and the largest h among them (the search radius).
@@ -946,6 +957,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_direct(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
     NNWaveLds &L = lds_all[w];
+    if (h_pass_frozen(A)) return;
     const uint32_t cnt = *A.count_in;
     const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
     for (int64_t t = (int64_t)blockIdx.x * NN1_WAVES + w; t < cnt; t += nwaves) {
@@ -986,6 +998,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
     __shared__ NNWaveLds lds_all[NN1_WAVES];
     const int w = threadIdx.x / 64, lane = wave_lane();
     NNWaveLds &L = lds_all[w];
+    if (h_pass_frozen(A)) return;
     const int64_t work = 8 * (int64_t)*A.slot_count;
     const int64_t nwaves = (int64_t)gridDim.x * NN1_WAVES;
     for (int64_t tt = (int64_t)blockIdx.x * NN1_WAVES + w; tt < work; tt += nwaves) {
@@ -1024,26 +1037,29 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
 // cnt_slot for the next pass and appends the unconverged particles to list_out
 __global__ __launch_bounds__(256) void k_h_finish(NNArgs A) {
     const uint32_t cnt = *A.count_in;
+    const bool frozen = h_pass_frozen(A);
     const int lane = wave_lane();
     const uint64_t lt = lanemask_lt();
     for (int64_t t0 = blockIdx.x * (int64_t)blockDim.x; t0 < cnt; t0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t t = t0 + threadIdx.x;
         const bool valid = t < cnt;
-        bool stay = false;
+        bool stay = false, changed = false;
         int64_t k = 0;
         if (valid) {
             k = A.list_in[t];
-            const int N = (int)A.cnt_slot[t];
+            const int N = frozen ? A.snoi[k] : (int)A.cnt_slot[t];
             A.cnt_slot[t] = 0u;
+            const double hi = A.h[k];
             double hn;
-            const bool fin = h_update(A, N, A.h[k], hn);
+            const bool fin = h_update(A, N, hi, hn);
             A.iters[k] = A.pass + 1;
             A.snoi[k] = N;
             A.h_next[k] = hn;
             if (fin) A.done[k] = 1;
             stay = !fin;
-            // the next pass counts with hn: the cell bounds must cover it
-            if (stay && A.chmax_raise) hmax_raise(A.chmax_raise, A.ppar, A.cpar, A.cap_cells, k, hn);
+            // the next pass counts with hn: the cell bounds must cover it (they cover hi)
+            changed = stay && hn != hi;
+            if (changed && A.chmax_raise) hmax_raise(A.chmax_raise, A.ppar, A.cpar, A.cap_cells, k, hn);
         }
         // wave-aggregated list append and counter (list order is free)
         const uint64_t mS = wave_ballot(stay);
@@ -1053,6 +1069,7 @@ __global__ __launch_bounds__(256) void k_h_finish(NNArgs A) {
         if (stay) A.list_out[b + popc64(mS & lt)] = (int32_t)k;
         const uint64_t mV = wave_ballot(valid);
         if (lane == 0 && mV != 0) atomicAdd(&A.counters[CNT_H_COUNTS], (unsigned long long)popc64(mV));
+        if (A.changed && wave_ballot(changed) != 0 && lane == 0) *A.changed = 1u;
     }
 }
 
@@ -1129,8 +1146,10 @@ static NNArgs nn_common(mhpc_ctx *c) {
 
 static unsigned nn_grid(mhpc_ctx *c) { return (unsigned)ceil_div(ceil_div(c->n, 64), NN_WAVES); }
 
-// done flags, iteration counts and dirty flags to zero; the per-pass list counters and the
-// outlier counters (pass 0, list fix-up) to zero
+// done flags, iteration counts and dirty flags to zero; the per-pass list counters, the
+// per-pass changed flags and the outlier counters (pass 0, list fix-up) to zero
+static_assert(offsetof(DevStatus, hchanged) == offsetof(DevStatus, hlist_count) + sizeof(DevStatus::hlist_count),
+              "k_h_init zeroes the list counters and the changed flags as one range");
 __global__ void k_h_init(int64_t n, uint8_t *done, int32_t *iters, uint8_t *dirty, uint32_t *list_counts,
                          int nlc, uint32_t *ocount, uint32_t *cnt_slot) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1168,7 +1187,7 @@ static void launch_hmax_raise(mhpc_ctx *c, const int32_t *list, const uint32_t *
 void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative, hipEvent_t *walk_events) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_h_init, dim3((unsigned)ceil_div(c->n > 64 ? c->n : 64, 256)), dim3(256), 0, c->stream,
-                       c->n, c->sdone, c->siters, c->sdirty, c->hlist_count, 2 * (MHPC_MAX_PASSES + 1), c->ocount,
+                       c->n, c->sdone, c->siters, c->sdirty, c->hlist_count, 3 * (MHPC_MAX_PASSES + 1), c->ocount,
                        c->hcnt);
     NNArgs a = nn_common(c);
     a.done = c->sdone;
@@ -1225,6 +1244,8 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative, hipEvent_t *wal
         a.count_out = c->hlist_count + pass;
         a.h_next = c->sh;   // k_h_finish updates h in place once every count is done
         a.cnt_slot = c->hcnt;
+        a.changed = c->dstatus->hchanged + pass;   // zeroed by k_h_init
+        a.prev_changed = pass >= 2 ? c->dstatus->hchanged + (pass - 1) : nullptr;
         // direct counts; the particles they cannot take go to the split walk
         uint32_t *hard_count = c->hlist_count + MHPC_MAX_PASSES + 1 + pass;   // zeroed by k_h_init
         NNArgs d = a;

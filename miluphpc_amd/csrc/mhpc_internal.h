@@ -66,6 +66,8 @@ struct DevStatus {
     uint32_t sort_check;                         // set when a partial sort left equal prefixes
     int32_t cell_base[MHPC_LEVELS + 3];          // level-major cell offsets, [23] = total
     uint32_t hlist_count[2 * (MHPC_MAX_PASSES + 1)];
+    uint32_t hchanged[MHPC_MAX_PASSES + 1];      // set when a pass changed some h (follows
+                                                 // hlist_count: k_h_init zeroes both)
 };
 
 struct mhpc_ctx {

@@ -77,6 +77,47 @@ def test_h_iteration_bitexact(gpu_lib, orc, sedov24, cap):
     assert np.array_equal(h, oh)
 
 
+@pytest.mark.parametrize("cap", [1.02, 1.1])
+def test_h_iteration_frozen_tail(gpu_lib, orc, cap):
+    """Lattice faces held at sml_max: after pass 1 (cap 1.02) or 2 (cap 1.1) no h changes any
+    more, so the later tail passes repeat the counts of the pass before (the counting kernels
+    skip them).  The oracle iterates every pass; h, iteration and neighbour counts and, on the
+    RHS path, the neighbour sets are bit-exact."""
+    print_case(f"h-iteration sedov 24^3 lattice, sml_max {cap} x h0 (frozen tail passes)")
+    s = ics.sedov_lattice(24)
+    h0 = s["sml"].copy()
+    sml_max = cap * h0.max()
+    # the oracle's h after u passes' updates is its result with max_counts = u + 1 (the last
+    # pass keeps h); pass F is the first frozen one when pass F - 1 changed no h
+    H = [orc.h_iterate(s["x"], s["y"], s["z"], h0, max_counts=u + 1, sml_max=sml_max)[1] for u in range(4)]
+    st, oh, oiters, onoi = orc.h_iterate(s["x"], s["y"], s["z"], h0, sml_max=sml_max)
+    assert st == 0
+    F = 2 if cap < 1.05 else 3
+    assert np.array_equal(H[F], H[F - 1]) and not np.array_equal(H[F - 1], H[F - 2])
+    assert np.array_equal(oh, H[F - 1])
+    assert (oiters == 7).sum() > 100 and (oh == sml_max).sum() > 100
+    p = gpu_lib.default_params(curve=1, sml_max=sml_max)
+    with make_ctx(gpu_lib, {**s, "sml": h0}, p) as ctx:
+        ctx.bbox(); ctx.keys(); ctx.sort(); ctx.build_tree()
+        ctx.h_iterate()
+        g = {k: ctx.download(k) for k in ("sml", "h_iters", "noi")}
+    assert np.array_equal(g["h_iters"], oiters)
+    assert np.array_equal(g["noi"], onoi)
+    assert np.array_equal(g["sml"], oh)
+    flags = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML | gpu_lib.RHS_SPECULATIVE_NN
+    with make_ctx(gpu_lib, {**s, "sml": h0}, p) as ctx:
+        st_rhs = ctx.rhs(flags)
+        g = {k: ctx.download(k) for k in ("sml", "h_iters", "noi")}
+        lists = ctx.neighbour_lists()
+    assert np.array_equal(g["h_iters"], oiters)
+    assert np.array_equal(g["sml"], oh)
+    assert np.array_equal(lists[1], onoi) and np.array_equal(g["noi"], onoi)
+    st, onnl, onoi2 = orc.nn_grid(s["x"], s["y"], s["z"], oh, 180)
+    assert st == 0 and np.array_equal(onoi2, onoi)
+    assert sets(*lists) == sets(onnl, onoi)
+    assert st_rhs["n_h_counts"] == int(oiters.sum())
+
+
 @pytest.mark.parametrize("drop", [7, 507])
 def test_lists_ragged_last_window(gpu_lib, orc, sedov24, drop):
     """List walks stream each group's 512-particle window of the key order first and clip it
