@@ -40,8 +40,8 @@ def log(msg: str) -> None:
 def sph_kernel_table(nnb: float, precision_fp64: bool = True) -> dict:
     return {
         "ms_nn_walk": ("k_nn_walk<2>", 32 + 4 + 4 * nnb),            # read x,y,z,h; write list + count
-        "ms_density": ("k_density<true, true>", 40 + 4 + 4 * nnb + 8),   # read x,y,z,h,m + list; write rho
-        "ms_forces": ("k_forces<true, false>", 88 + 4 + 4 * nnb + 48),   # read 11 fields + list; write 6
+        "ms_density": ("k_density<true, true, true>", 40 + 4 + 4 * nnb + 8),   # read x,y,z,h,m + list; write rho
+        "ms_forces": ("k_forces<true, false, true>", 88 + 4 + 4 * nnb + 48),   # read 11 fields + list; write 6
     }
 
 

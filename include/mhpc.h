@@ -81,6 +81,18 @@ enum { MHPC_BH_BOX = 0, MHPC_BH_VOTE = 1, MHPC_BH_PER_PARTICLE = 2 };
 /* Pair-sum precision. */
 enum { MHPC_PREC_FP64 = 0, MHPC_PREC_FP32_KAHAN = 1 };
 
+/* SPH smoothing kernels, support radius h (reference Smoothing::Kernel, parameter.h:177-180;
+ * SPH::SmoothingKernel, kernel.cu:14-183, 3D).  The Wendland C4 gradient is the reference's
+ * expression (kernel.cu:140), which is not the derivative of that kernel's W; it is kept
+ * as written so that results equal the reference's. */
+enum {
+    MHPC_KERNEL_SPIKY = 0,
+    MHPC_KERNEL_CUBIC_SPLINE = 1,
+    MHPC_KERNEL_WENDLAND_C2 = 2,
+    MHPC_KERNEL_WENDLAND_C4 = 3,
+    MHPC_KERNEL_WENDLAND_C6 = 4
+};
+
 /* Runtime parameters.  Field names follow SimulationParameters (parameter.h:122-159)
  * and Material (material.cuh:123-160) where they have a counterpart. */
 typedef struct {
@@ -110,6 +122,8 @@ typedef struct {
                                reference).  On a free boundary a particle that cannot reach the
                                band under d < min(h_i, h_j) otherwise grows h on every pass. */
     int bh_accept;          /* MHPC_BH_VOTE (default), MHPC_BH_BOX or MHPC_BH_PER_PARTICLE */
+    int smoothing_kernel;   /* MHPC_KERNEL_* (SimulationParameters::smoothingKernelSelection);
+                               default MHPC_KERNEL_CUBIC_SPLINE */
 } mhpc_params;
 
 /* Field ids for upload / download / device pointers (reference Entry::Name, particles.cuh).

@@ -45,6 +45,7 @@ extern "C" void mhpc_default_params(mhpc_params *p) {
     p->precision = MHPC_PREC_FP64;
     p->cs_before_p = 1;
     p->bh_accept = MHPC_BH_VOTE;
+    p->smoothing_kernel = MHPC_KERNEL_CUBIC_SPLINE;
 }
 
 extern "C" const char *mhpc_status_string(int s) {
@@ -74,6 +75,7 @@ static int check_params(const mhpc_params *p) {
     if (!(p->sml_max >= 0.0)) return 0;
     if (p->bh_accept < MHPC_BH_BOX || p->bh_accept > MHPC_BH_PER_PARTICLE) return 0;
     if (p->bh_accept == MHPC_BH_PER_PARTICLE && (p->mac != MHPC_MAC_CLASSIC || p->quadrupole)) return 0;
+    if (p->smoothing_kernel < MHPC_KERNEL_SPIKY || p->smoothing_kernel > MHPC_KERNEL_WENDLAND_C6) return 0;
     return 1;
 }
 

@@ -1367,6 +1367,56 @@ __device__ __forceinline__ double rsq_nr(double x) {
     return fma(y, e, y);
 }
 
+// a10, the five kernels of SPH::SmoothingKernel (kernel.cu:14-183, 3D branches; Dehnen & Aly
+// 2012 for the Wendland functions), selected by mhpc_params.smoothing_kernel as the
+// reference's Smoothing::Kernel (parameter.h:177-180).  Normalised form: for q = r/h <= 1,
+// W = sigma/h^3 w(q) and dW/dr = sigma/h^4 w'(q); both are 0 beyond.  The cubic spline keeps
+// its own inlined form in the density and force loops; the others go through kernel_shape.
+// The Wendland C4 gradient is the reference's expression (kernel.cu:140), which is not the
+// derivative of its W; it is followed as written so that results are the reference's.
+__device__ __forceinline__ double kernel_sigma(int kind) {
+    switch (kind) {
+        case MHPC_KERNEL_SPIKY: return 15.0 / M_PI;
+        case MHPC_KERNEL_WENDLAND_C2: return 21.0 / (2.0 * M_PI);
+        case MHPC_KERNEL_WENDLAND_C4: return 495.0 / (32.0 * M_PI);
+        case MHPC_KERNEL_WENDLAND_C6: return 1365.0 / (64.0 * M_PI);
+        default: return 8.0 / M_PI;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void kernel_shape(int kind, T q, T &w, T &dw) {
+    const T t = T(1) - q;
+    const T t2 = t * t, t3 = t2 * t;
+    switch (kind) {
+        case MHPC_KERNEL_SPIKY:
+            w = t3;
+            dw = T(-3) * t2;
+            break;
+        case MHPC_KERNEL_WENDLAND_C2:
+            w = t2 * t2 * (T(1) + T(4) * q);
+            dw = T(-20) * q * t3;
+            break;
+        case MHPC_KERNEL_WENDLAND_C4:
+            w = t3 * t3 * (T(1) + q * (T(6) + T(35.0 / 3.0) * q));
+            dw = T(-6) * t3 * t2 * (T(1) + q * q * (T(-35) + T(105) * q));
+            break;
+        case MHPC_KERNEL_WENDLAND_C6:
+            w = t2 * t2 * t2 * t2 * (T(1) + q * (T(8) + q * (T(25) + T(32) * q)));
+            dw = T(-22) * q * t3 * t2 * t2 * (T(1) + q * (T(7) + T(16) * q));
+            break;
+        default:   // cubic spline
+            if (q <= T(0.5)) {
+                w = T(1) + T(6) * q * q * (q - T(1));
+                dw = T(6) * q * (T(3) * q - T(2));
+            } else {
+                w = T(2) * t3;
+                dw = T(-6) * t2;
+            }
+            break;
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // a12 / a13 sound speed (soundspeed.cu:38-97) and pressure (pressure.cu:6-87), fused;
 // also writes the records the force loop gathers: cs into {vx,vy,vz,cs} and
@@ -1446,9 +1496,34 @@ static EosArgs eos_args(mhpc_ctx *c) {
 
 // one density pair term: FP64 adds m_j W into acc; FP32 adds it (in FP32, from FP64
 // differences) into the Kahan accumulator acc32
-template <bool FP64>
+template <bool FP64, bool CUBIC>
 __device__ __forceinline__ void density_pair(const double4 &pi, double hi, const double4 &pj, double hj,
-                                             int sml_avg, double &acc, Kahan &acc32) {
+                                             int sml_avg, int kind, double &acc, Kahan &acc32) {
+    if constexpr (!CUBIC) {
+        if constexpr (FP64) {
+            const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            const double ih = rcp_nr(hb);
+            const double q = r2 * rsq_nr(r2) * ih;
+            if (q <= 1.0) {
+                double w, dw;
+                kernel_shape<double>(kind, q, w, dw);
+                acc = fma(pj.w, kernel_sigma(kind) * (ih * ih * ih) * w, acc);
+            }
+        } else {
+            const float hb = (float)(sml_avg ? 0.5 * (hi + hj) : hi);
+            const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y), dz = (float)(pi.z - pj.z);
+            const float ih = 1.0f / hb;
+            const float q = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz))) * ih;
+            if (q <= 1.0f) {
+                float w, dw;
+                kernel_shape<float>(kind, q, w, dw);
+                acc32.add((float)pj.w * ((float)kernel_sigma(kind) * ih * ih * ih * w));
+            }
+        }
+        return;
+    }
     if constexpr (FP64) {
         const double hb = sml_avg ? 0.5 * (hi + hj) : hi;
         const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -1489,12 +1564,13 @@ struct DensPub {
 #define DENS_WIN 512
 
 // EOS: the sound speed / pressure of the particle follow in the same kernel (the RHS path)
-template <bool FP64, bool EOS>
+template <bool FP64, bool EOS, bool CUBIC>
 __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *__restrict__ sxm,
                                                       const double *__restrict__ sh,
                                                       const int32_t *__restrict__ snoi,
                                                       const int32_t *__restrict__ nbr, int64_t stride,
-                                                      int max_nb, int sml_avg, const int32_t *__restrict__ perm,
+                                                      int max_nb, int sml_avg, int kind,
+                                                      const int32_t *__restrict__ perm,
                                                       double *__restrict__ srho, double *__restrict__ rho,
                                                       uint32_t *err, EosArgs E, DensPub P) {
     __shared__ double4 wp[DENS_WIN];
@@ -1546,9 +1622,10 @@ __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *
             }
     #pragma unroll
             for (int e = 0; e < 8; e++)
-                if (e < m) density_pair<FP64>(pi, hi, pj[e], hj[e], sml_avg, acc, acc32);
+                if (e < m) density_pair<FP64, CUBIC>(pi, hi, pj[e], hj[e], sml_avg, kind, acc, acc32);
         }
-        const double r = pi.w * (K8PI / (hi * hi * hi)) + (FP64 ? acc : acc32.value());
+        const double self = CUBIC ? K8PI / (hi * hi * hi) : kernel_sigma(kind) / (hi * hi * hi);   // W(0)
+        const double r = pi.w * self + (FP64 ? acc : acc32.value());
         srho[k] = r;
         const int32_t i = perm[k];
         rho[i] = r;
@@ -1586,21 +1663,28 @@ __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *
     }
 }
 
+template <bool FP64, bool EOS>
+static void launch_density_p(mhpc_ctx *c, const EosArgs &E, const DensPub &P) {
+    const unsigned g = (unsigned)ceil_div(c->n, DENS_WIN);
+    const int kind = c->p.smoothing_kernel;
+    if (kind == MHPC_KERNEL_CUBIC_SPLINE)
+        hipLaunchKernelGGL((k_density<FP64, EOS, true>), dim3(g), dim3(DENS_WIN), 0, c->stream, c->n, c->sxm, c->sh,
+                           c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, kind, c->perm, c->srho,
+                           c->f[MHPC_F_RHO], c->err, E, P);
+    else
+        hipLaunchKernelGGL((k_density<FP64, EOS, false>), dim3(g), dim3(DENS_WIN), 0, c->stream, c->n, c->sxm,
+                           c->sh, c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, kind, c->perm, c->srho,
+                           c->f[MHPC_F_RHO], c->err, E, P);
+}
+
 template <bool EOS>
 static void launch_density_t(mhpc_ctx *c, bool publish) {
     if (c->n <= 0) return;
-    const unsigned g = (unsigned)ceil_div(c->n, DENS_WIN);
     const EosArgs E = eos_args(c);
     DensPub P = {};
     if (publish) P = DensPub{c->siters, c->f[MHPC_F_SML], c->noi, c->h_iters, c->counters};
-    if (c->p.precision == MHPC_PREC_FP64)
-        hipLaunchKernelGGL((k_density<true, EOS>), dim3(g), dim3(DENS_WIN), 0, c->stream, c->n, c->sxm, c->sh,
-                           c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
-                           c->f[MHPC_F_RHO], c->err, E, P);
-    else
-        hipLaunchKernelGGL((k_density<false, EOS>), dim3(g), dim3(DENS_WIN), 0, c->stream, c->n, c->sxm, c->sh,
-                           c->snoi, c->nbr, c->cap_pad, c->p.max_nb, c->p.sml_avg, c->perm, c->srho,
-                           c->f[MHPC_F_RHO], c->err, E, P);
+    if (c->p.precision == MHPC_PREC_FP64) launch_density_p<true, EOS>(c, E, P);
+    else launch_density_p<false, EOS>(c, E, P);
 }
 
 void launch_density(mhpc_ctx *c) { launch_density_t<false>(c, false); }
@@ -1634,7 +1718,7 @@ struct ForceArgs {
     const double *sh;
     const int32_t *snoi, *nbr;
     int64_t stride;
-    int max_nb, sml_avg;
+    int max_nb, sml_avg, kind;
     double alpha, beta;
     const int32_t *perm;
     double *ax, *ay, *az, *drhodt, *dedt, *dsmldt, *muijmax;
@@ -1658,13 +1742,13 @@ struct ForceAcc<false> {
 struct ForceI {   // particle i's data and the constants of the pair loop
     double4 p, v, r;
     double h;
-    int sml_avg;
+    int sml_avg, kind;
     double alpha, beta;
 };
 
 // one pair (i, j) of the force loop; j's records {x,y,z,m}, {vx,vy,vz,cs},
 // {rho, p/rho^2, m/rho, h}
-template <bool FP64, bool FEAT>
+template <bool FP64, bool FEAT, bool CUBIC>
 __device__ __forceinline__ void force_pair(const ForceI &I, const double4 &pj, const double4 &vj, const double4 &rj,
                                            ForceAcc<FP64> &acc, double &mumax, double *feat) {
     const double4 pi = I.p, vi = I.v, ri = I.r;
@@ -1679,12 +1763,18 @@ __device__ __forceinline__ void force_pair(const ForceI &I, const double4 &pj, c
         const double ih = rcp_nr(sml);
         const double q = r2 * rinv * ih;
         const double ih2 = ih * ih;
-        const double fih = K8PI * (ih2 * ih2);
-        // grad W = (dW/dr / r) r_ij; for q <= 1/2, dW/dr / r = 6 f/h (3q - 2) q / r with
-        // q / r = 1/h
+        // grad W = (dW/dr / r) r_ij
         double gs = 0.0;
-        if (q <= 0.5) gs = fih * ih * fma(18.0, q, -12.0);
-        else if (q <= 1.0) { const double t = 1.0 - q; gs = -6.0 * fih * (t * t) * rinv; }
+        if constexpr (CUBIC) {
+            // for q <= 1/2, dW/dr / r = 6 f/h (3q - 2) q / r with q / r = 1/h
+            const double fih = K8PI * (ih2 * ih2);
+            if (q <= 0.5) gs = fih * ih * fma(18.0, q, -12.0);
+            else if (q <= 1.0) { const double t = 1.0 - q; gs = -6.0 * fih * (t * t) * rinv; }
+        } else if (q <= 1.0) {
+            double w, dw;
+            kernel_shape<double>(I.kind, q, w, dw);
+            gs = kernel_sigma(I.kind) * (ih2 * ih2) * dw * rinv;
+        }
         const double g0 = gs * d0, g1 = gs * d1, g2 = gs * d2;
         const double v0 = vi.x - vj.x, v1 = vi.y - vj.y, v2 = vi.z - vj.z;
         const double vv = fma(v0, g0, fma(v1, g1, v2 * g2));
@@ -1723,7 +1813,15 @@ __device__ __forceinline__ void force_pair(const ForceI &I, const double4 &pj, c
         const float r2 = __builtin_fmaf(d0, d0, __builtin_fmaf(d1, d1, d2 * d2));
         const float r = sqrtf(r2);
         float W, dWdr;
-        cubic32(r, sml, W, dWdr);
+        if constexpr (CUBIC) {
+            cubic32(r, sml, W, dWdr);
+        } else {
+            const float ih = 1.0f / sml;
+            const float q = r * ih;
+            float w = 0.f, dw = 0.f;
+            if (q <= 1.0f) kernel_shape<float>(I.kind, q, w, dw);
+            dWdr = (float)kernel_sigma(I.kind) * (ih * ih) * (ih * ih) * dw;
+        }
         const float gs = dWdr / r;
         const float g0 = gs * d0, g1 = gs * d1, g2 = gs * d2;
         const float vv = __builtin_fmaf(v0, g0, __builtin_fmaf(v1, g1, v2 * g2));
@@ -1763,7 +1861,7 @@ __device__ __forceinline__ void force_pair(const ForceI &I, const double4 &pj, c
 #define FORCE_WIN 512
 static_assert(FORCE_WIN == NN_CWIN && DENS_WIN == NN_CWIN, "lists hold the neighbours of the consumers' window first");
 
-template <bool FP64, bool FEAT>
+template <bool FP64, bool FEAT, bool CUBIC>
 __global__ __launch_bounds__(FORCE_WIN) void k_forces(ForceArgs A) {
     __shared__ double4 wp[FORCE_WIN], wv[FORCE_WIN], wr[FORCE_WIN];
     const int64_t base = xcd_block(blockIdx.x, gridDim.x) * (int64_t)FORCE_WIN;
@@ -1782,6 +1880,7 @@ __global__ __launch_bounds__(FORCE_WIN) void k_forces(ForceArgs A) {
     I.r = wr[tid];
     I.h = A.sh[k];
     I.sml_avg = A.sml_avg;
+    I.kind = A.kind;
     I.alpha = A.alpha;
     I.beta = A.beta;
     int cnt = A.snoi[k];
@@ -1820,7 +1919,7 @@ __global__ __launch_bounds__(FORCE_WIN) void k_forces(ForceArgs A) {
             }
 #pragma unroll
             for (int e = 0; e < 2; e++)
-                if (h + e < m) force_pair<FP64, FEAT>(I, pj[e], vj[e], rj[e], acc, mumax, feat);
+                if (h + e < m) force_pair<FP64, FEAT, CUBIC>(I, pj[e], vj[e], rj[e], acc, mumax, feat);
         }
     }
     double out[6];
@@ -1849,13 +1948,22 @@ __global__ __launch_bounds__(FORCE_WIN) void k_forces(ForceArgs A) {
     }
 }
 
+template <bool FP64, bool FEAT>
+static void launch_forces_p(mhpc_ctx *c, const ForceArgs &a) {
+    const unsigned g = (unsigned)ceil_div(c->n, FORCE_WIN);
+    if (a.kind == MHPC_KERNEL_CUBIC_SPLINE)
+        hipLaunchKernelGGL((k_forces<FP64, FEAT, true>), dim3(g), dim3(FORCE_WIN), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_forces<FP64, FEAT, false>), dim3(g), dim3(FORCE_WIN), 0, c->stream, a);
+}
+
 void launch_forces(mhpc_ctx *c, bool features) {
     if (c->n <= 0) return;
     ForceArgs a;
     a.n = c->n;
     a.sxm = c->sxm; a.sv4 = c->sv4; a.sr4 = c->sr4; a.sh = c->sh;
     a.snoi = c->snoi; a.nbr = c->nbr; a.stride = c->cap_pad;
-    a.max_nb = c->p.max_nb; a.sml_avg = c->p.sml_avg;
+    a.max_nb = c->p.max_nb; a.sml_avg = c->p.sml_avg; a.kind = c->p.smoothing_kernel;
     a.alpha = c->p.alpha; a.beta = c->p.beta;
     a.perm = c->perm;
     a.ax = c->f[MHPC_F_AX]; a.ay = c->f[MHPC_F_AY]; a.az = c->f[MHPC_F_AZ];
@@ -1863,13 +1971,12 @@ void launch_forces(mhpc_ctx *c, bool features) {
     a.muijmax = c->f[MHPC_F_MUIJMAX];
     a.curlv = c->f[MHPC_F_CURLV];
     a.gradrho = c->f[MHPC_F_GRADRHO];
-    const unsigned g = (unsigned)ceil_div(c->n, FORCE_WIN);
     if (c->p.precision == MHPC_PREC_FP64) {
-        if (features) hipLaunchKernelGGL((k_forces<true, true>), dim3(g), dim3(FORCE_WIN), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_forces<true, false>), dim3(g), dim3(FORCE_WIN), 0, c->stream, a);
+        if (features) launch_forces_p<true, true>(c, a);
+        else launch_forces_p<true, false>(c, a);
     } else {
-        if (features) hipLaunchKernelGGL((k_forces<false, true>), dim3(g), dim3(FORCE_WIN), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_forces<false, false>), dim3(g), dim3(FORCE_WIN), 0, c->stream, a);
+        if (features) launch_forces_p<false, true>(c, a);
+        else launch_forces_p<false, false>(c, a);
     }
 }
 

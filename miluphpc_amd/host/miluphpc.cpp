@@ -20,6 +20,7 @@ static mhpc_params to_params(const SimulationParameters &sp, const Material &m) 
     p.curve = sp.sfcSelection == Curve::hilbert ? MHPC_CURVE_HILBERT : MHPC_CURVE_LEBESGUE;
     p.precision = sp.precision;
     p.sml_max = sp.smlMax;
+    p.smoothing_kernel = sp.smoothingKernelSelection;
     p.eos = m.eos.type;
     p.gamma = m.eos.polytropic_gamma;
     p.K = m.eos.polytropic_K;

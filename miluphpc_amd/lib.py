@@ -31,6 +31,7 @@ EOS_LOCALLY_ISOTHERMAL, EOS_ISOTHERMAL_CONST_CS = 12, 100
 MAC_CLASSIC, MAC_SQRT_THETA = 0, 1
 BH_BOX, BH_VOTE, BH_PER_PARTICLE = 0, 1, 2
 PREC_FP64, PREC_FP32_KAHAN = 0, 1
+KERNEL_SPIKY, KERNEL_CUBIC_SPLINE, KERNEL_WENDLAND_C2, KERNEL_WENDLAND_C4, KERNEL_WENDLAND_C6 = 0, 1, 2, 3, 4
 
 RHS_GRAVITY = 1 << 0
 RHS_SPH = 1 << 1
@@ -83,6 +84,7 @@ class Params(C.Structure):
         ("cs_before_p", C.c_int),
         ("sml_max", C.c_double),
         ("bh_accept", C.c_int),
+        ("smoothing_kernel", C.c_int),
     ]
 
     def as_dict(self) -> dict:

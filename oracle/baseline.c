@@ -44,12 +44,12 @@ int orc_bench_sph_rhs(int64_t n, const double *x, const double *y, const double 
         st = orc_h_iterate(n, x, y, z, h, n_target, band_lo, band_hi, max_counts, max_nb, table, BENCH_TMAX,
                            sml_max, iters, noi);
         if (st == ORC_OK) st = orc_nn_grid(n, x, y, z, h, max_nb, nnl, noi);
-        if (st == ORC_OK) st = orc_density(n, x, y, z, m, h, nnl, noi, max_nb, 1, rho);
+        if (st == ORC_OK) st = orc_density(n, x, y, z, m, h, nnl, noi, max_nb, 1, ORC_KERNEL_CUBIC_SPLINE, rho);
         if (st == ORC_OK) {
             orc_soundspeed(n, eos, gamma, K, rho, p, cs);
             orc_pressure(n, eos, gamma, K, rho, e, cs, p);
-            orc_forces(n, x, y, z, vx, vy, vz, m, h, rho, p, cs, nnl, noi, max_nb, 1, alpha, beta, out,
-                       out + N, out + 2 * N, out + 3 * N, out + 4 * N, out + 5 * N, out + 6 * N);
+            orc_forces(n, x, y, z, vx, vy, vz, m, h, rho, p, cs, nnl, noi, max_nb, 1, ORC_KERNEL_CUBIC_SPLINE, alpha,
+                       beta, out, out + N, out + 2 * N, out + 3 * N, out + 4 * N, out + 5 * N, out + 6 * N);
         }
         seconds[r] = now_s() - t0;
     }

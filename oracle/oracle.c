@@ -852,24 +852,75 @@ void orc_cubic_spline(double r2, double dx, double dy, double dz, double h, doub
     dWdx[2] = *dWdr / r * dz;
 }
 
+/* a10, the other four kernels of SPH::SmoothingKernel (kernel.cu:14-48 spiky, :86-113
+ * wendlandc2, :116-145 wendlandc4, :149-183 wendlandc6; Dehnen & Aly 2012), their 3D
+ * branches, selected as the reference's Smoothing::Kernel (parameter.h:177-180,
+ * kernel_handler.cu:13-36).  Each is W = 0, dW/dr = 0 beyond r = h and grad W = dW/dr / r dx.
+ * The Wendland C4 gradient restates the reference's expression (kernel.cu:140) as written:
+ * it is not the derivative of that kernel's W (which is -(56/3) sigma/h^4 q (1-q)^5 (1+5q)),
+ * so a finite-difference check cannot pin it -- parity unpinned for that gradient.
+ * Returns ORC_E_ARG for an unknown kind. */
+int orc_sph_kernel(int kind, double r2, double dx, double dy, double dz, double h, double *W,
+                   double dWdx[3], double *dWdr) {
+    if (kind == ORC_KERNEL_CUBIC_SPLINE) {
+        orc_cubic_spline(r2, dx, dy, dz, h, W, dWdx, dWdr);
+        return ORC_OK;
+    }
+    const double r = sqrt(r2);
+    const double q = r / h;
+    const double h3 = h * h * h, h4 = h3 * h;
+    const double t = 1.0 - q;
+    *W = 0.0;
+    *dWdr = 0.0;
+    if (q <= 1.0) {
+        switch (kind) {
+            case ORC_KERNEL_SPIKY:
+                *W = 15. / (M_PI * h3) * t * t * t;
+                *dWdr = -45. / (M_PI * h4) * t * t;
+                break;
+            case ORC_KERNEL_WENDLAND_C2:
+                *W = 21. / (2 * M_PI * h3) * t * t * t * t * (1 + 4 * q);
+                *dWdr = -210. / (M_PI * h4) * q * t * t * t;
+                break;
+            case ORC_KERNEL_WENDLAND_C4:
+                *W = 495. / (32. * M_PI * h3) * t * t * t * t * t * t * (1. + 6. * q + 35. / 3. * q * q);
+                *dWdr = -1485. / (16. * M_PI * h4) * t * t * t * t * t * (1. - 35. * q * q + 105. * q * q * q);
+                break;
+            case ORC_KERNEL_WENDLAND_C6:
+                *W = 1365. / (64. * M_PI * h3) * t * t * t * t * t * t * t * t *
+                     (1. + 8. * q + 25. * q * q + 32 * q * q * q);
+                *dWdr = -15015. / (32. * M_PI * h4) * q * t * t * t * t * t * t * t * (1. + 7 * q + 16 * q * q);
+                break;
+            default: return ORC_E_ARG;
+        }
+    } else if (kind < ORC_KERNEL_SPIKY || kind > ORC_KERNEL_WENDLAND_C6) {
+        return ORC_E_ARG;
+    }
+    dWdx[0] = *dWdr / r * dx;
+    dWdx[1] = *dWdr / r * dy;
+    dWdx[2] = *dWdr / r * dz;
+    return ORC_OK;
+}
+
 /* a11: SPH::Kernel::calculateDensity (density.cu:8-72): self term m_i W(0, h_i) plus
  * m_j W(r_ij, h) over the list, h = (h_i + h_j)/2 when sml_avg (VARIABLE_SML ||
  * INTEGRATE_SML) else h_i.  rho <= 0 is the reference's termination (density.cu:68). */
 int orc_density(int64_t n, const double *x, const double *y, const double *z,
                 const double *m, const double *h, const int32_t *nnl, const int32_t *noi,
-                int max_nb, int sml_avg, double *rho) {
+                int max_nb, int sml_avg, int kernel, double *rho) {
+    if (kernel < ORC_KERNEL_SPIKY || kernel > ORC_KERNEL_WENDLAND_C6) return ORC_E_ARG;
     int err = ORC_OK;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         double W, dWdx[3], dWdr;
-        orc_cubic_spline(0.0, 0.0, 0.0, 0.0, h[i], &W, dWdx, &dWdr);
+        orc_sph_kernel(kernel, 0.0, 0.0, 0.0, 0.0, h[i], &W, dWdx, &dWdr);
         double r = m[i] * W;
         for (int k = 0; k < noi[i]; k++) {
             int32_t j = nnl[i * max_nb + k];
             double sml = sml_avg ? 0.5 * (h[i] + h[j]) : h[i];
             double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
             double r2 = dx * dx + dy * dy + dz * dz;
-            orc_cubic_spline(r2, dx, dy, dz, sml, &W, dWdx, &dWdr);
+            orc_sph_kernel(kernel, r2, dx, dy, dz, sml, &W, dWdx, &dWdr);
             r += m[j] * W;
         }
         rho[i] = r;
@@ -918,7 +969,7 @@ void orc_pressure(int64_t n, int eos, double gamma, double K, const double *rho,
 void orc_forces(int64_t n, const double *x, const double *y, const double *z,
                 const double *vx, const double *vy, const double *vz, const double *m,
                 const double *h, const double *rho, const double *p, const double *cs,
-                const int32_t *nnl, const int32_t *noi, int max_nb, int sml_avg,
+                const int32_t *nnl, const int32_t *noi, int max_nb, int sml_avg, int kernel,
                 double alpha, double beta, double *ax, double *ay, double *az,
                 double *drhodt, double *dedt, double *dsmldt, double *muijmax) {
 #pragma omp parallel for schedule(static)
@@ -932,7 +983,7 @@ void orc_forces(int64_t n, const double *x, const double *y, const double *z,
             double r2 = 0.0;
             for (int e = 0; e < 3; e++) r2 += dr[e] * dr[e];
             double W, dWdx[3], dWdr;
-            orc_cubic_spline(r2, dr[0], dr[1], dr[2], sml, &W, dWdx, &dWdr);
+            orc_sph_kernel(kernel, r2, dr[0], dr[1], dr[2], sml, &W, dWdx, &dWdr);
             double dv[3] = {vx[i] - vx[j], vy[i] - vy[j], vz[i] - vz[j]};
             double vvnablaW = dv[0] * dWdx[0];
             vvnablaW += dv[1] * dWdx[1];
@@ -1075,7 +1126,7 @@ void orc_time_step_terms(int64_t n, const double *ax, const double *ay, const do
 void orc_sph_features(int64_t n, const double *x, const double *y, const double *z,
                       const double *vx, const double *vy, const double *vz, const double *m,
                       const double *h, const double *rho, const int32_t *nnl, const int32_t *noi,
-                      int max_nb, int sml_avg, double *curl, double *gradrho) {
+                      int max_nb, int sml_avg, int kernel, double *curl, double *gradrho) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         double c[3] = {0, 0, 0}, g[3] = {0, 0, 0};
@@ -1085,7 +1136,7 @@ void orc_sph_features(int64_t n, const double *x, const double *y, const double 
             const double dr[3] = {x[i] - x[j], y[i] - y[j], z[i] - z[j]};
             const double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
             double W, gW[3], dWdr;
-            orc_cubic_spline(r2, dr[0], dr[1], dr[2], sml, &W, gW, &dWdr);
+            orc_sph_kernel(kernel, r2, dr[0], dr[1], dr[2], sml, &W, gW, &dWdr);
             const double V = m[j] / rho[j];
             const double dv[3] = {vx[j] - vx[i], vy[j] - vy[i], vz[j] - vz[i]};
             c[0] += V * (gW[1] * dv[2] - gW[2] * dv[1]);

@@ -6,8 +6,10 @@
  *
  * Every function cites the reference file:line it restates (paths relative to the
  * milupHPC source tree).  Functions that no golden vector, closed form, invariant or
- * brute-force check pins are marked "parity unpinned" in their comment; at present
- * none is (the tests under tests/test_oracle_*.py pin each one).
+ * brute-force check pins are marked "parity unpinned" in their comment; at present that
+ * is only the gradient of the Wendland C4 kernel (orc_sph_kernel, ORC_KERNEL_WENDLAND_C4),
+ * which restates the reference's expression but is not the derivative of its W; everything
+ * else is pinned by the tests under tests/test_oracle_*.py.
  *
  * Node numbering of the oracle's octree: indices [0, n) are particles, [n, n+ncells)
  * are cells, and n is the root.  (The reference aliases the root with particle 0's
@@ -119,11 +121,16 @@ int orc_h_iterate(int64_t n, const double *x, const double *y, const double *z, 
                   const double *table, int tmax, double sml_max, int32_t *iters, int32_t *noi);
 
 /* ---- a10-a14: SPH (FP64) ---- */
+/* smoothing kernels: the reference's Smoothing::Kernel (parameter.h:177-180) */
+enum { ORC_KERNEL_SPIKY = 0, ORC_KERNEL_CUBIC_SPLINE = 1, ORC_KERNEL_WENDLAND_C2 = 2,
+       ORC_KERNEL_WENDLAND_C4 = 3, ORC_KERNEL_WENDLAND_C6 = 4 };
 void orc_cubic_spline(double r2, double dx, double dy, double dz, double h, double *W,
                       double dWdx[3], double *dWdr);
+int orc_sph_kernel(int kind, double r2, double dx, double dy, double dz, double h, double *W,
+                   double dWdx[3], double *dWdr);
 int orc_density(int64_t n, const double *x, const double *y, const double *z,
                 const double *m, const double *h, const int32_t *nnl, const int32_t *noi,
-                int max_nb, int sml_avg, double *rho);
+                int max_nb, int sml_avg, int kernel, double *rho);
 void orc_soundspeed(int64_t n, int eos, double gamma, double K, const double *rho,
                     const double *p, double *cs);
 void orc_pressure(int64_t n, int eos, double gamma, double K, const double *rho,
@@ -131,7 +138,7 @@ void orc_pressure(int64_t n, int eos, double gamma, double K, const double *rho,
 void orc_forces(int64_t n, const double *x, const double *y, const double *z,
                 const double *vx, const double *vy, const double *vz, const double *m,
                 const double *h, const double *rho, const double *p, const double *cs,
-                const int32_t *nnl, const int32_t *noi, int max_nb, int sml_avg,
+                const int32_t *nnl, const int32_t *noi, int max_nb, int sml_avg, int kernel,
                 double alpha, double beta, double *ax, double *ay, double *az,
                 double *drhodt, double *dedt, double *dsmldt, double *muijmax);
 
@@ -162,7 +169,7 @@ void orc_time_step_terms(int64_t n, const double *ax, const double *ay, const do
 void orc_sph_features(int64_t n, const double *x, const double *y, const double *z,
                       const double *vx, const double *vy, const double *vz, const double *m,
                       const double *h, const double *rho, const int32_t *nnl, const int32_t *noi,
-                      int max_nb, int sml_avg, double *curl, double *gradrho);
+                      int max_nb, int sml_avg, int kernel, double *curl, double *gradrho);
 /* 7 -> H -> H -> 1 tanh MLP, FP64 forward/backward */
 void orc_mlp_forward(int64_t nb, int nin, int nh, const double *X, const double *W1,
                      const double *b1, const double *W2, const double *b2, const double *W3,

@@ -53,17 +53,18 @@ def _load() -> C.CDLL:
         "orc_h_factor_table": (None, [i32, i32, vp]),
         "orc_h_iterate": (i32, [i64, vp, vp, vp, vp, i32, i32, i32, i32, i32, vp, i32, d, vp, vp]),
         "orc_cubic_spline": (None, [d, d, d, d, d, vp, vp, vp]),
-        "orc_density": (i32, [i64, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp]),
+        "orc_sph_kernel": (i32, [i32, d, d, d, d, d, vp, vp, vp]),
+        "orc_density": (i32, [i64, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, vp]),
         "orc_soundspeed": (None, [i64, i32, d, d, vp, vp, vp]),
         "orc_pressure": (None, [i64, i32, d, d, vp, vp, vp, vp]),
-        "orc_forces": (None, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32,
+        "orc_forces": (None, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32,
                               d, d, vp, vp, vp, vp, vp, vp, vp]),
         "orc_pc_predictor": (None, [i64, d, vp, vp]),
         "orc_pc_corrector": (None, [i64, d, vp, vp]),
         "orc_time_step_terms": (None, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, d, vp]),
         "orc_leapfrog_update_x": (None, [i64, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "orc_leapfrog_update_v": (None, [i64, d, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
-        "orc_sph_features": (None, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, vp]),
+        "orc_sph_features": (None, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp]),
         "orc_mlp_forward": (None, [i64, i32, i32, vp, vp, vp, vp, vp, vp, d, vp, vp, vp]),
         "orc_mlp_backward": (d, [i64, i32, i32, vp, vp, vp, vp, vp, vp, vp, d, vp, vp, vp, vp,
                                  vp, vp]),
@@ -254,12 +255,27 @@ def cubic_spline(r2, dx, dy, dz, h):
     return float(W[0]), g, float(dWdr[0])
 
 
-def density(x, y, z, m, h, nnl, noi, sml_avg=1):
+# smoothing kernels (the reference's Smoothing::Kernel ids)
+KERNEL_SPIKY, KERNEL_CUBIC_SPLINE, KERNEL_WENDLAND_C2, KERNEL_WENDLAND_C4, KERNEL_WENDLAND_C6 = 0, 1, 2, 3, 4
+
+
+def sph_kernel(kind, r2, dx, dy, dz, h):
+    W = np.zeros(1)
+    g = np.zeros(3)
+    dWdr = np.zeros(1)
+    st = LIB.orc_sph_kernel(int(kind), float(r2), float(dx), float(dy), float(dz), float(h), P(W), P(g), P(dWdr))
+    if st != 0:
+        raise ValueError(f"unknown smoothing kernel {kind}")
+    return float(W[0]), g, float(dWdr[0])
+
+
+def density(x, y, z, m, h, nnl, noi, sml_avg=1, kernel=KERNEL_CUBIC_SPLINE):
     x, y, z, m, h = f64(x), f64(y), f64(z), f64(m), f64(h)
     nnl = np.ascontiguousarray(nnl, np.int32)
     noi = np.ascontiguousarray(noi, np.int32)
     rho = np.empty(len(x))
-    st = LIB.orc_density(len(x), P(x), P(y), P(z), P(m), P(h), P(nnl), P(noi), nnl.shape[1], int(sml_avg), P(rho))
+    st = LIB.orc_density(len(x), P(x), P(y), P(z), P(m), P(h), P(nnl), P(noi), nnl.shape[1], int(sml_avg),
+                         int(kernel), P(rho))
     return st, rho
 
 
@@ -277,18 +293,19 @@ def pressure(eos, gamma, K, rho, e, cs):
     return p
 
 
-def forces(x, y, z, vx, vy, vz, m, h, rho, p, cs, nnl, noi, alpha=1.0, beta=2.0, sml_avg=1):
+def forces(x, y, z, vx, vy, vz, m, h, rho, p, cs, nnl, noi, alpha=1.0, beta=2.0, sml_avg=1,
+           kernel=KERNEL_CUBIC_SPLINE):
     arrs = [f64(a) for a in (x, y, z, vx, vy, vz, m, h, rho, p, cs)]
     nnl = np.ascontiguousarray(nnl, np.int32)
     noi = np.ascontiguousarray(noi, np.int32)
     n = len(arrs[0])
     out = [np.empty(n) for _ in range(7)]
-    LIB.orc_forces(n, *[P(a) for a in arrs], P(nnl), P(noi), nnl.shape[1], int(sml_avg), float(alpha),
-                   float(beta), *[P(o) for o in out])
+    LIB.orc_forces(n, *[P(a) for a in arrs], P(nnl), P(noi), nnl.shape[1], int(sml_avg), int(kernel),
+                   float(alpha), float(beta), *[P(o) for o in out])
     return dict(zip(["ax", "ay", "az", "drhodt", "dedt", "dsmldt", "muijmax"], out))
 
 
-def sph_features(x, y, z, vx, vy, vz, m, h, rho, nnl, noi, sml_avg=1):
+def sph_features(x, y, z, vx, vy, vz, m, h, rho, nnl, noi, sml_avg=1, kernel=KERNEL_CUBIC_SPLINE):
     """(curl v, grad rho) as (n, 3) arrays over the neighbour lists."""
     arrs = [f64(a) for a in (x, y, z, vx, vy, vz, m, h, rho)]
     nnl = np.ascontiguousarray(nnl, np.int32)
@@ -296,7 +313,8 @@ def sph_features(x, y, z, vx, vy, vz, m, h, rho, nnl, noi, sml_avg=1):
     n = len(arrs[0])
     curl = np.empty((n, 3))
     grad = np.empty((n, 3))
-    LIB.orc_sph_features(n, *[P(a) for a in arrs], P(nnl), P(noi), nnl.shape[1], int(sml_avg), P(curl), P(grad))
+    LIB.orc_sph_features(n, *[P(a) for a in arrs], P(nnl), P(noi), nnl.shape[1], int(sml_avg), int(kernel),
+                         P(curl), P(grad))
     return curl, grad
 
 
@@ -368,7 +386,8 @@ def leapfrog_update_v(dt, s: dict) -> None:
 
 # ---- composite passes used by the tests and the host-core baseline ------------------
 def sph_rhs(s: dict, n_target=64, band=(50, 80), max_counts=7, max_nb=180, eos=9, gamma=5.0 / 3.0,
-            K=0.0, alpha=1.0, beta=2.0, variable_sml=True, cs_before_p=True, sml_max=0.0):
+            K=0.0, alpha=1.0, beta=2.0, variable_sml=True, cs_before_p=True, sml_max=0.0,
+            kernel=KERNEL_CUBIC_SPLINE):
     """h-iteration -> neighbours -> density -> sound speed -> pressure -> forces, FP64."""
     h = f64(s["sml"])
     out = {}
@@ -378,7 +397,7 @@ def sph_rhs(s: dict, n_target=64, band=(50, 80), max_counts=7, max_nb=180, eos=9
         out["h_iters"] = iters
     st, nnl, noi = nn_grid(s["x"], s["y"], s["z"], h, max_nb)
     out["nn_status"] = st
-    st, rho = density(s["x"], s["y"], s["z"], s["mass"], h, nnl, noi)
+    st, rho = density(s["x"], s["y"], s["z"], s["mass"], h, nnl, noi, kernel=kernel)
     out["rho_status"] = st
     cs = f64(s["cs"])
     p = f64(s["p"])
@@ -389,7 +408,7 @@ def sph_rhs(s: dict, n_target=64, band=(50, 80), max_counts=7, max_nb=180, eos=9
         p = pressure(eos, gamma, K, rho, s["e"], cs)
         cs = soundspeed(eos, gamma, K, rho, p, cs)
     f = forces(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"], s["mass"], h, rho, p, cs, nnl, noi,
-               alpha, beta)
+               alpha, beta, kernel=kernel)
     out.update({"sml": h, "nnl": nnl, "noi": noi, "rho": rho, "p": p, "cs": cs, **f})
     return out
 

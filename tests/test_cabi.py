@@ -48,8 +48,8 @@ def test_struct_layout_matches_header():
 #include <stddef.h>
 #include "mhpc.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(mhpc_params), offsetof(mhpc_params, cs_before_p),
-         offsetof(mhpc_params, sml_max),
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(mhpc_params), offsetof(mhpc_params, cs_before_p),
+         offsetof(mhpc_params, sml_max), offsetof(mhpc_params, smoothing_kernel),
          sizeof(mhpc_rhs_stats), offsetof(mhpc_rhs_stats, num_cells), offsetof(mhpc_rhs_stats, h_passes));
   return 0;
 }
@@ -60,7 +60,7 @@ int main(void) {
     subprocess.run(["gcc", "-I", str(REPO / "include"), str(tmp / "t.c"), "-o", str(tmp / "t")], check=True)
     vals = list(map(int, subprocess.run([str(tmp / "t")], capture_output=True, text=True).stdout.split()))
     assert vals == [C.sizeof(lib.Params), lib.Params.cs_before_p.offset, lib.Params.sml_max.offset,
-                    C.sizeof(lib.RhsStats),
+                    lib.Params.smoothing_kernel.offset, C.sizeof(lib.RhsStats),
                     lib.RhsStats.num_cells.offset, lib.RhsStats.h_passes.offset]
 
 

@@ -86,19 +86,20 @@ def test_mlp_gradient_matches_oracle(gpu_lib, orc, n):
         assert cos > 0.999, (name, cos)
 
 
-def sedov_ctx(gpu_lib, nside):
+def sedov_ctx(gpu_lib, nside, **params):
     s = ics.sedov_lattice(nside)
     n = len(s["x"])
     for k in ("ax", "ay", "az", "dedt", "g_ax", "g_ay", "g_az", "muijmax", "drhodt", "dsmldt"):
         s.setdefault(k, np.zeros(n))
-    ctx = gpu_lib.Context(n, ics.params_sedov())
+    ctx = gpu_lib.Context(n, ics.params_sedov(**params))
     ctx.load(**{k: v for k, v in s.items() if k in gpu_lib.FIELD_ID})
     return ctx, s
 
 
-def test_features_curl_gradrho_match_oracle(gpu_lib, orc):
-    print_case("learned-h features sedov 16^3 after 2 pc steps")
-    ctx, s = sedov_ctx(gpu_lib, 16)
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_features_curl_gradrho_match_oracle(gpu_lib, orc, kernel):
+    print_case(f"learned-h features sedov 16^3 after 2 pc steps, smoothing kernel {kernel}")
+    ctx, s = sedov_ctx(gpu_lib, 16, smoothing_kernel=kernel)
     with ctx:
         flags = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML
         ctx.mlp_init(3)
@@ -109,7 +110,7 @@ def test_features_curl_gradrho_match_oracle(gpu_lib, orc):
                                            "gradrho", "noi")}
         nnl, noi = ctx.neighbour_lists(180)
     curl, grad = orc.sph_features(f["x"], f["y"], f["z"], f["vx"], f["vy"], f["vz"], f["mass"], f["sml"],
-                                  f["rho"], nnl, noi)
+                                  f["rho"], nnl, noi, kernel=kernel)
     cm = np.linalg.norm(curl, axis=1)
     gm = np.linalg.norm(grad, axis=1)
     assert np.abs(f["gradrho"] - gm).max() <= 1e-9 * gm.max()

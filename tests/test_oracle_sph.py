@@ -95,6 +95,54 @@ def test_cubic_spline_closed_forms(orc):
         assert np.allclose(g, dWdr / r0 * np.array([0.6 * r0, 0.8 * r0, 0.0]), rtol=1e-14)
 
 
+# W(0) h^3 of each kernel from its published normalisation (3D; Desbrun & Cani 1996 for the
+# spiky kernel, Dehnen & Aly 2012 Table 1 for the Wendland functions), and W(q) h^3 in closed
+# form -- the shapes the oracle's expressions must reproduce
+KERNEL_SHAPES = {
+    0: (15 / math.pi, lambda q: (1 - q) ** 3),
+    1: (8 / math.pi, lambda q: 1 - 6 * q * q + 6 * q ** 3 if q <= 0.5 else 2 * (1 - q) ** 3),
+    2: (21 / (2 * math.pi), lambda q: (1 - q) ** 4 * (1 + 4 * q)),
+    3: (495 / (32 * math.pi), lambda q: (1 - q) ** 6 * (1 + 6 * q + 35 / 3 * q * q)),
+    4: (1365 / (64 * math.pi), lambda q: (1 - q) ** 8 * (1 + 8 * q + 25 * q * q + 32 * q ** 3)),
+}
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4])
+def test_smoothing_kernels_normalised(orc, kind):
+    """Every kernel: W(0) = sigma / h^3, the published shape at interior points, W = 0 from
+    r = h on, int W dV = 1 by quadrature, and grad W = dW/dr dx / r.  dW/dr equals the
+    central difference of W for all but the Wendland C4 kernel, whose gradient expression in
+    the reference is not its derivative: there the difference of W matches the analytic
+    derivative -(56/3) sigma/h^4 q (1-q)^5 (1+5q) instead, and the restated expression stays
+    unpinned."""
+    h = 0.37
+    sigma, shape = KERNEL_SHAPES[kind]
+    W0 = orc.sph_kernel(kind, 0.0, 0, 0, 0, h)[0]
+    assert W0 == pytest.approx(sigma / h ** 3, rel=1e-14)
+    for q in (0.1, 0.3, 0.5, 0.62, 0.9):
+        W = orc.sph_kernel(kind, (q * h) ** 2, q * h, 0, 0, h)[0]
+        assert W == pytest.approx(sigma / h ** 3 * shape(q), rel=1e-12)
+    assert orc.sph_kernel(kind, h * h, h, 0, 0, h)[0] == 0.0
+    assert orc.sph_kernel(kind, (1.01 * h) ** 2, 1.01 * h, 0, 0, h)[0] == 0.0
+    rr = np.linspace(0, h, 4001)
+    Wr = np.array([orc.sph_kernel(kind, r * r, r, 0, 0, h)[0] for r in rr])
+    assert np.trapezoid(4 * math.pi * rr * rr * Wr, rr) == pytest.approx(1.0, abs=2e-5)
+    for q in (0.2, 0.45, 0.7, 0.95):
+        r0, dr = q * h, 1e-7
+        Wp = orc.sph_kernel(kind, (r0 + dr) ** 2, r0 + dr, 0, 0, h)[0]
+        Wm = orc.sph_kernel(kind, (r0 - dr) ** 2, r0 - dr, 0, 0, h)[0]
+        fd = (Wp - Wm) / (2 * dr)
+        W, g, dWdr = orc.sph_kernel(kind, r0 * r0, r0 * 0.6, r0 * 0.8, 0.0, h)
+        assert np.allclose(g, dWdr / r0 * np.array([0.6 * r0, 0.8 * r0, 0.0]), rtol=1e-14)
+        if kind == 3:
+            assert fd == pytest.approx(-(56 / 3) * sigma / h ** 4 * q * (1 - q) ** 5 * (1 + 5 * q), rel=1e-6)
+            assert abs(dWdr - fd) > 1e-3 * abs(fd)
+        else:
+            assert dWdr == pytest.approx(fd, rel=1e-6)
+    with pytest.raises(ValueError):
+        orc.sph_kernel(5, 0.01, 0.1, 0, 0, h)
+
+
 def lattice_state(nside, n_nb=64):
     s = ics.sedov_lattice(nside, n_nb=n_nb)
     s["e"] = np.full(len(s["x"]), 1.0)  # uniform pressure
@@ -106,13 +154,15 @@ def interior(s, margin):
             & (np.abs(s["z"]) < 0.5 - margin))
 
 
-def test_lattice_density_and_zero_force(orc):
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4])
+def test_lattice_density_and_zero_force(orc, kernel):
     s = lattice_state(24)
-    out = orc.sph_rhs(s, variable_sml=False)
+    out = orc.sph_rhs(s, variable_sml=False, kernel=kernel)
     inner = interior(s, 2.5 * s["sml"][0])
     assert out["rho_status"] == 0
-    # rho = 1 in the interior up to the kernel's discretisation error
-    assert np.all(np.abs(out["rho"][inner] - 1.0) < 0.03)
+    # rho = 1 in the interior up to the kernel's discretisation error (the spiky kernel,
+    # peaked at r = 0, has the largest)
+    assert np.all(np.abs(out["rho"][inner] - 1.0) < (0.12 if kernel == 0 else 0.03))
     # uniform density and pressure: the forces cancel by symmetry
     a = np.sqrt(out["ax"] ** 2 + out["ay"] ** 2 + out["az"] ** 2)
     scale = np.abs(out["p"]).max() / out["rho"].max() / s["sml"][0]
@@ -151,9 +201,10 @@ def sedov_small():
     return s
 
 
-def test_forces_conserve_momentum_and_energy(orc, sedov_small):
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4])
+def test_forces_conserve_momentum_and_energy(orc, sedov_small, kernel):
     s = sedov_small
-    out = orc.sph_rhs(s, variable_sml=True)
+    out = orc.sph_rhs(s, variable_sml=True, kernel=kernel)
     m = s["mass"]
     for k in ("ax", "ay", "az"):
         tot = (m * out[k]).sum()
@@ -171,9 +222,7 @@ def test_forces_conserve_momentum_and_energy(orc, sedov_small):
             h = 0.5 * (out["sml"][i] + out["sml"][j])
             dr = np.array([s["x"][i] - s["x"][j], s["y"][i] - s["y"][j], s["z"][i] - s["z"][j]])
             r = np.sqrt(dr @ dr)
-            q = r / h
-            f = 8 / math.pi / h ** 3
-            dWdr = 6 * f / h * (3 * q * q - 2 * q) if q <= 0.5 else -6 * f / h * (1 - q) ** 2
+            dWdr = orc.sph_kernel(kernel, r * r, dr[0], dr[1], dr[2], h)[2]
             dv = np.array([s["vx"][i] - s["vx"][j], s["vy"][i] - s["vy"][j], s["vz"][i] - s["vz"][j]])
             acc += out["rho"][i] / out["rho"][j] * m[j] * (dv @ (dWdr / r * dr))
         assert acc == pytest.approx(out["drhodt"][i], rel=1e-10, abs=1e-12)
