@@ -8,15 +8,15 @@
 // the corrector combines the two.
 #include "mhpc_internal.h"
 
-// backup slots
-enum { PB_X, PB_Y, PB_Z, PB_VX, PB_VY, PB_VZ, PB_AX, PB_AY, PB_AZ, PB_E, PB_DEDT, PB_SML,
-       PB_DSMLDT, PB_RHO, PB_DRHODT, PB_COUNT_ };
+// backup slots: what the corrector reads (h, rho and their rates are not integrated:
+// INTEGRATE_SML and INTEGRATE_DENSITY are 0, so the predicted state's values stay live)
+enum { PB_X, PB_Y, PB_Z, PB_VX, PB_VY, PB_VZ, PB_AX, PB_AY, PB_AZ, PB_E, PB_DEDT, PB_COUNT_ };
 
 struct PCArgs {
     int64_t n;
     double dt;
     double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *gx, *gy, *gz;
-    double *e, *dedt, *sml, *dsmldt, *rho, *drhodt;
+    double *e, *dedt;
     double *b[PB_COUNT_];
 };
 
@@ -31,8 +31,6 @@ __global__ __launch_bounds__(256) void k_pc_predictor(PCArgs A) {
     A.b[PB_VX][i] = vx; A.b[PB_VY][i] = vy; A.b[PB_VZ][i] = vz;
     A.b[PB_AX][i] = ax; A.b[PB_AY][i] = ay; A.b[PB_AZ][i] = az;
     A.b[PB_E][i] = e; A.b[PB_DEDT][i] = dedt;
-    A.b[PB_SML][i] = A.sml[i]; A.b[PB_DSMLDT][i] = A.dsmldt[i];
-    A.b[PB_RHO][i] = A.rho[i]; A.b[PB_DRHODT][i] = A.drhodt[i];
     A.x[i] = x + A.dt * vx;
     A.vx[i] = vx + A.dt * (ax + A.gx[i]);
     A.y[i] = y + A.dt * vy;
@@ -162,8 +160,7 @@ static PCArgs pc_args(mhpc_ctx *c, double dt) {
     a.vx = f[MHPC_F_VX]; a.vy = f[MHPC_F_VY]; a.vz = f[MHPC_F_VZ];
     a.ax = f[MHPC_F_AX]; a.ay = f[MHPC_F_AY]; a.az = f[MHPC_F_AZ];
     a.gx = f[MHPC_F_G_AX]; a.gy = f[MHPC_F_G_AY]; a.gz = f[MHPC_F_G_AZ];
-    a.e = f[MHPC_F_E]; a.dedt = f[MHPC_F_DEDT]; a.sml = f[MHPC_F_SML]; a.dsmldt = f[MHPC_F_DSMLDT];
-    a.rho = f[MHPC_F_RHO]; a.drhodt = f[MHPC_F_DRHODT];
+    a.e = f[MHPC_F_E]; a.dedt = f[MHPC_F_DEDT];
     for (int k = 0; k < PB_COUNT_; k++) a.b[k] = c->pc_backup + (int64_t)k * c->cap;
     return a;
 }
