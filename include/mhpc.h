@@ -297,6 +297,15 @@ int mhpc_time_step(mhpc_ctx *ctx, double courant_fact, double forces_fact, doubl
 int mhpc_pc_predictor(mhpc_ctx *ctx, double dt);
 /* Kernel::Launch::corrector: combines the saved state with the predicted derivatives */
 int mhpc_pc_corrector(mhpc_ctx *ctx, double dt);
+/* One sub-step of PredictorCorrectorEuler::integrate (predictor_corrector_euler.cpp): rhs
+ * (flags), setTimeStep -- dt = min(courant_fact q1, forces_fact q0, courant_fact q2, max h /
+ * max |v|, dt_max if > 0, t_left if > 0) as mhpc_time_step -- predictor, rhs (flags without
+ * MHPC_RHS_GRAVITY), corrector; the same results as those five calls in turn, queued without
+ * waiting in between (dt is computed and used on the device).  *dt receives the step; stats0 /
+ * stats1 (may be NULL) the two RHS's statistics.  Flags with MHPC_RHS_LEARNED_H or
+ * MHPC_RHS_FEATURES are refused (MHPC_E_ARG): use the separate calls. */
+int mhpc_pc_euler_step(mhpc_ctx *ctx, uint32_t flags, double courant_fact, double forces_fact, double dt_max,
+                       double t_left, double *dt, mhpc_rhs_stats *stats0, mhpc_rhs_stats *stats1);
 
 /* -------- learned smoothing length (build extension, no reference counterpart): a
  * 7 -> 64 -> 64 -> 1 tanh MLP, BF16 weights on MFMA with FP32 accumulation, predicting

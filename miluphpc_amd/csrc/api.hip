@@ -121,6 +121,7 @@ static void free_all(mhpc_ctx *c) {
     for (int f = 0; f < MHPC_F_NOI; f++)
         if (c->f_own[f]) (void)hipFree(c->f_own[f]);
     if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->h_status1) (void)hipHostFree(c->h_status1);
     for (int e = 0; e < MHPC_NUM_EVENTS; e++)
         if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
     if (c->sync_ev) (void)hipEventDestroy(c->sync_ev);
@@ -136,6 +137,8 @@ static int create_impl(mhpc_ctx *c, int64_t cap, int64_t cap_cells) {
     HIPC(hipMemset(c->dstatus, 0, sizeof(DevStatus)));
     HIPC(hipHostMalloc((void **)&c->h_status, sizeof(DevStatus), hipHostMallocDefault));
     memset(c->h_status, 0, sizeof(DevStatus));
+    HIPC(hipHostMalloc((void **)&c->h_status1, sizeof(DevStatus), hipHostMallocDefault));
+    memset(c->h_status1, 0, sizeof(DevStatus));
     c->err = &c->dstatus->err;
     c->counters = c->dstatus->counters;
     c->sort_check = &c->dstatus->sort_check;
@@ -420,11 +423,11 @@ hipError_t ctx_fetch_status(mhpc_ctx *c) {
     return e != hipSuccess ? e : ctx_sync(c);
 }
 
-static int finish(mhpc_ctx *c) {
-    HIPC(hipGetLastError());
-    HIPC(ctx_fetch_status(c));
-    const uint32_t e = c->h_status->err;
-    if (c->h_status->counters[CNT_TREE_DEPTH] > 0) c->com_depth_hint = (int)c->h_status->counters[CNT_TREE_DEPTH];
+// the status of a call from a fetched copy of the device status block (and the tree depth
+// the next sort starts from)
+static int status_of(mhpc_ctx *c, const DevStatus &hs) {
+    const uint32_t e = hs.err;
+    if (hs.counters[CNT_TREE_DEPTH] > 0) c->com_depth_hint = (int)hs.counters[CNT_TREE_DEPTH];
     if (e == 0) return MHPC_OK;
     int st = MHPC_OK;
     if (e & ERRF_DEPTH) st = MHPC_E_DEPTH;
@@ -436,11 +439,19 @@ static int finish(mhpc_ctx *c) {
     return st;
 }
 
-static float elapsed(mhpc_ctx *c, int a, int b) {
+static int finish(mhpc_ctx *c) {
+    HIPC(hipGetLastError());
+    HIPC(ctx_fetch_status(c));
+    return status_of(c, *c->h_status);
+}
+
+static float elapsed_ev(const hipEvent_t *ev, int a, int b) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) ms = 0.f;
     return ms;
 }
+
+static float elapsed(mhpc_ctx *c, int a, int b) { return elapsed_ev(c->ev, a, b); }
 
 // one stage: reset flags, launch, time with events, sync, map errors
 template <typename F>
@@ -579,14 +590,12 @@ extern "C" int mhpc_time_step(mhpc_ctx *c, double courant_fact, double forces_fa
                               double *dt, double parts[5]) {
     if (!c || !dt) return MHPC_E_ARG;
     double q[5];
-    launch_time_step(c, c->diag_partial, c->diag_out);
+    const DtRule rule = {courant_fact, forces_fact, dt_max, -1.0, nullptr};
+    launch_time_step(c, c->diag_partial, c->diag_out, rule);
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(q, c->diag_out, sizeof(q), hipMemcpyDeviceToHost, c->stream));
     HIPC(ctx_sync(c));
-    double t = fmin(courant_fact * q[1], forces_fact * q[0]);
-    t = fmin(t, courant_fact * q[2]);
-    if (q[3] > 0.0 && q[4] > 0.0) t = fmin(t, q[4] / q[3]);
-    if (dt_max > 0.0) t = fmin(t, dt_max);
+    const double t = dt_rule(rule, q[0], q[1], q[2], q[3], q[4]);
     *dt = t;
     c->last_dt = t;
     if (parts)
@@ -934,42 +943,50 @@ extern "C" int mhpc_download_tree(mhpc_ctx *c, int32_t *cell_level, int32_t *cel
 // Miluphpc::rhs (miluphpc.cpp:387-562) for one subdomain: reset -> bounding box -> keys ->
 // sort -> tree -> [COM -> Barnes-Hut] -> [h-iteration -> neighbours -> density -> sound
 // speed / pressure -> internal forces].  No host synchronisation until the end.
-extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
-    if (!c) return MHPC_E_ARG;
-    const bool grav = flags & MHPC_RHS_GRAVITY;
-    const bool sph = flags & MHPC_RHS_SPH;
-    const bool hiter = sph && (flags & MHPC_RHS_VARIABLE_SML);
-    const bool pot = grav && (flags & MHPC_RHS_POTENTIAL);
-    const bool learned = hiter && (flags & MHPC_RHS_LEARNED_H);
+// what one RHS call does, from its flags
+struct RhsPlan {
+    bool grav, sph, hiter, pot, learned, feats, spec, predict, timing;
+};
+
+static int rhs_plan(mhpc_ctx *c, uint32_t flags, RhsPlan &P) {
+    P.grav = flags & MHPC_RHS_GRAVITY;
+    P.sph = flags & MHPC_RHS_SPH;
+    P.hiter = P.sph && (flags & MHPC_RHS_VARIABLE_SML);
+    P.pot = P.grav && (flags & MHPC_RHS_POTENTIAL);
+    P.learned = P.hiter && (flags & MHPC_RHS_LEARNED_H);
     // learned-h features (MLP inputs and curl v / grad rho) without the prediction:
     // the plain steps that produce training pairs
-    const bool feats = hiter && (flags & (MHPC_RHS_LEARNED_H | MHPC_RHS_FEATURES));
+    P.feats = P.hiter && (flags & (MHPC_RHS_LEARNED_H | MHPC_RHS_FEATURES));
     // speculative lists: pass 0 of the h-iteration writes the lists, and only particles
     // whose list can differ from the final one are re-listed (same neighbour sets)
-    const bool spec = hiter && (learned || (flags & MHPC_RHS_SPECULATIVE_NN));
-    if (sph) {
+    P.spec = P.hiter && (P.learned || (flags & MHPC_RHS_SPECULATIVE_NN));
+    P.timing = flags & MHPC_RHS_TIMING;
+    if (P.sph) {
         int st = ensure_nbr(c);
         if (st != MHPC_OK) return st;
     }
-    if (feats) {
+    if (P.feats) {
         int st = ensure_mlp(c);
         if (st != MHPC_OK) return st;
     }
     // the learned start applies once the model has feature statistics; before that the
     // features are still taken (raw) so that a following mhpc_mlp_train can use them
-    const bool predict = learned && c->mlp_norm_set;
-    int e = 0;
-    // per-phase events only with MHPC_RHS_TIMING; the total (13 -> 12), the pass-0 walk
-    // (14 -> 15), the density (9 -> 10) and the forces (11 -> 12) are always timed
-    const bool timing = flags & MHPC_RHS_TIMING;
+    P.predict = P.learned && c->mlp_norm_set;
+    return MHPC_OK;
+}
+
+// Queue one RHS on the context stream (no host synchronisation besides the sort's check).
+// ev: 16 events; per-phase events only with MHPC_RHS_TIMING; the total (13 -> 12), the pass-0
+// walk (14 -> 15), the density (9 -> 10) and the forces (11 -> 12) are always timed.
+static void rhs_launch(mhpc_ctx *c, const RhsPlan &P, hipEvent_t *ev) {
     auto mark = [&](int idx) {
-        if (timing || idx >= 9) (void)hipEventRecord(c->ev[idx], c->stream);
+        if (P.timing || idx >= 9) (void)hipEventRecord(ev[idx], c->stream);
     };
     reset_flags(c);
     mark(13);
-    if (feats) {
+    if (P.feats) {
         launch_mlp_features(c, c->last_dt);
-        if (predict) launch_mlp_predict(c);
+        if (P.predict) launch_mlp_predict(c);
     }
     mark(0);
     launch_bbox(c);
@@ -978,68 +995,130 @@ extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
     mark(2);
     launch_sort(c);
     mark(3);
-    launch_gather(c, grav, sph, true);
+    launch_gather(c, P.grav, P.sph, true);
     mark(4);
     launch_tree(c);
     mark(5);
-    launch_bottom_up(c, grav, sph);
+    launch_bottom_up(c, P.grav, P.sph);
     mark(6);
-    if (grav) launch_gravity(c, pot);
+    if (P.grav) launch_gravity(c, P.pot);
     mark(7);
-    if (hiter) {
-        launch_h_iterate(c, c->p.max_counts, spec, c->ev + 14);
-        if (!spec) launch_scatter_h(c);   // with lists from pass 0, the list fix-up publishes h
+    if (P.hiter) {
+        launch_h_iterate(c, c->p.max_counts, P.spec, ev + 14);
+        if (!P.spec) launch_scatter_h(c);   // with lists from pass 0, the list fix-up publishes h
     }
     mark(8);
-    if (spec) launch_neighbours_fixup(c);
-    else if (sph) launch_neighbours(c);
+    if (P.spec) launch_neighbours_fixup(c);
+    else if (P.sph) launch_neighbours(c);
     mark(9);
     // sound speed and pressure fused (ms_eos stays ~0); after the list fix-up it also publishes
     // h and the neighbour counts
-    if (sph) launch_density_eos(c, spec);
+    if (P.sph) launch_density_eos(c, P.spec);
     mark(10);
     mark(11);
-    if (sph) launch_forces(c, feats);
+    if (P.sph) launch_forces(c, P.feats);
     mark(12);
-    (void)e;
-    int st = finish(c);
+}
+
+// the stage flags after an RHS that ended with status st
+static void rhs_valid(mhpc_ctx *c, const RhsPlan &P, int st) {
     c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = (st == MHPC_OK);
-    c->com_valid = c->com_valid && grav;
-    c->nn_valid = c->rho_valid = c->eos_valid = (st == MHPC_OK) && sph;
-    if (feats) c->mlp_x_valid = st == MHPC_OK;
-    if (stats) {
-        memset(stats, 0, sizeof(*stats));
-        if (timing) {
-            stats->ms_bbox = elapsed(c, 0, 1);
-            stats->ms_keys = elapsed(c, 1, 2);
-            stats->ms_sort = elapsed(c, 2, 3);
-            stats->ms_gather = elapsed(c, 3, 4);
-            stats->ms_tree = elapsed(c, 4, 5);
-            stats->ms_com = elapsed(c, 5, 6);
-            stats->ms_gravity = elapsed(c, 6, 7);
-            stats->ms_hiter = elapsed(c, 7, 8);
-            stats->ms_nn = elapsed(c, 8, 9);
-            stats->ms_eos = elapsed(c, 10, 11);
-            stats->ms_mlp = elapsed(c, 13, 0);
-        }
-        stats->ms_density = sph ? elapsed(c, 9, 10) : 0.f;
-        stats->ms_forces = sph ? elapsed(c, 11, 12) : 0.f;
-        stats->ms_total = elapsed(c, 13, 12);
-        const DevStatus &hs = *c->h_status;   // fetched by finish()
-        stats->num_cells = hs.cell_base[MHPC_LEVELS + 1];
-        stats->n_cell_interactions = (int64_t)hs.counters[CNT_CELL_INT];
-        stats->n_part_interactions = (int64_t)hs.counters[CNT_PART_INT];
-        stats->n_neighbours = (int64_t)hs.counters[CNT_NEIGHBOURS];
-        stats->max_noi = (int32_t)hs.counters[CNT_MAX_NOI];
-        stats->n_h_counts = (int64_t)hs.counters[CNT_H_COUNTS];
-        stats->ms_nn_walk = hiter ? elapsed(c, 14, 15) : 0.f;
-        stats->h_passes = 0;
-        if (hiter) {
-            // pass 0, then every pass whose input list (the previous pass's unconverged
-            // particles) was not empty
-            stats->h_passes = 1;
-            for (int pass = 1; pass < c->p.max_counts; pass++) stats->h_passes += hs.hlist_count[pass - 1] > 0 ? 1 : 0;
-        }
+    c->com_valid = c->com_valid && P.grav;
+    c->nn_valid = c->rho_valid = c->eos_valid = (st == MHPC_OK) && P.sph;
+    if (P.feats) c->mlp_x_valid = st == MHPC_OK;
+}
+
+// statistics of a finished RHS from its events and its copy of the device status block
+static void rhs_stats(mhpc_ctx *c, const RhsPlan &P, const hipEvent_t *ev, const DevStatus &hs,
+                      mhpc_rhs_stats *stats) {
+    if (!stats) return;
+    memset(stats, 0, sizeof(*stats));
+    if (P.timing) {
+        stats->ms_bbox = elapsed_ev(ev, 0, 1);
+        stats->ms_keys = elapsed_ev(ev, 1, 2);
+        stats->ms_sort = elapsed_ev(ev, 2, 3);
+        stats->ms_gather = elapsed_ev(ev, 3, 4);
+        stats->ms_tree = elapsed_ev(ev, 4, 5);
+        stats->ms_com = elapsed_ev(ev, 5, 6);
+        stats->ms_gravity = elapsed_ev(ev, 6, 7);
+        stats->ms_hiter = elapsed_ev(ev, 7, 8);
+        stats->ms_nn = elapsed_ev(ev, 8, 9);
+        stats->ms_eos = elapsed_ev(ev, 10, 11);
+        stats->ms_mlp = elapsed_ev(ev, 13, 0);
     }
+    stats->ms_density = P.sph ? elapsed_ev(ev, 9, 10) : 0.f;
+    stats->ms_forces = P.sph ? elapsed_ev(ev, 11, 12) : 0.f;
+    stats->ms_total = elapsed_ev(ev, 13, 12);
+    stats->num_cells = hs.cell_base[MHPC_LEVELS + 1];
+    stats->n_cell_interactions = (int64_t)hs.counters[CNT_CELL_INT];
+    stats->n_part_interactions = (int64_t)hs.counters[CNT_PART_INT];
+    stats->n_neighbours = (int64_t)hs.counters[CNT_NEIGHBOURS];
+    stats->max_noi = (int32_t)hs.counters[CNT_MAX_NOI];
+    stats->n_h_counts = (int64_t)hs.counters[CNT_H_COUNTS];
+    stats->ms_nn_walk = P.hiter ? elapsed_ev(ev, 14, 15) : 0.f;
+    stats->h_passes = 0;
+    if (P.hiter) {
+        // pass 0, then every pass whose input list (the previous pass's unconverged
+        // particles) was not empty
+        stats->h_passes = 1;
+        for (int pass = 1; pass < c->p.max_counts; pass++) stats->h_passes += hs.hlist_count[pass - 1] > 0 ? 1 : 0;
+    }
+}
+
+extern "C" int mhpc_rhs(mhpc_ctx *c, uint32_t flags, mhpc_rhs_stats *stats) {
+    if (!c) return MHPC_E_ARG;
+    RhsPlan P;
+    int st = rhs_plan(c, flags, P);
+    if (st != MHPC_OK) return st;
+    rhs_launch(c, P, c->ev);
+    st = finish(c);
+    rhs_valid(c, P, st);
+    rhs_stats(c, P, c->ev, *c->h_status, stats);
     return st;
+}
+
+// One sub-step of PredictorCorrectorEuler::integrate (predictor_corrector_euler.cpp): rhs with
+// `flags`, the CFL time step, the predictor, rhs without self-gravity on the predicted state,
+// the corrector -- queued back to back on the context stream.  The time step is computed and
+// consumed on the device (k_time_step_final writes it, the predictor and corrector read it),
+// the first RHS's status block is copied aside without waiting, and the host waits once at the
+// end (besides each sort's check): the same kernels and results as mhpc_rhs, mhpc_time_step,
+// mhpc_pc_predictor, mhpc_rhs and mhpc_pc_corrector called in turn, without their four
+// round trips.  Returns the first RHS's error, else the second's; on an error the state is the
+// caller's to discard (the reference stops).
+extern "C" int mhpc_pc_euler_step(mhpc_ctx *c, uint32_t flags, double courant_fact, double forces_fact,
+                                  double dt_max, double t_left, double *dt, mhpc_rhs_stats *stats0,
+                                  mhpc_rhs_stats *stats1) {
+    if (!c || !dt) return MHPC_E_ARG;
+    // the learned-h features use the previous time step on the host side: the separate calls
+    if (flags & (MHPC_RHS_LEARNED_H | MHPC_RHS_FEATURES)) return MHPC_E_ARG;
+    int st = ensure_pc_backup(c);
+    if (st != MHPC_OK) return st;
+    RhsPlan P0, P1;
+    st = rhs_plan(c, flags, P0);
+    if (st == MHPC_OK) st = rhs_plan(c, flags & ~(uint32_t)MHPC_RHS_GRAVITY, P1);
+    if (st != MHPC_OK) return st;
+    double *dt_dev = &c->dstatus->dt;
+    rhs_launch(c, P0, c->ev);
+    HIPC(hipMemcpyAsync(c->h_status1, c->dstatus, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
+    launch_time_step(c, c->diag_partial, c->diag_out, DtRule{courant_fact, forces_fact, dt_max, t_left, dt_dev});
+    launch_pc_predictor(c, 0.0, dt_dev);
+    rhs_launch(c, P1, c->ev + 16);
+    launch_pc_corrector(c, 0.0, dt_dev);
+    HIPC(hipGetLastError());
+    HIPC(ctx_fetch_status(c));
+    const int st0 = status_of(c, *c->h_status1);
+    const int st1 = status_of(c, *c->h_status);   // last: the next sort starts from this tree's depth
+    *dt = c->h_status->dt;
+    c->last_dt = *dt;
+    rhs_stats(c, P0, c->ev, *c->h_status1, stats0);
+    rhs_stats(c, P1, c->ev + 16, *c->h_status, stats1);
+    // the corrector moved the particles
+    c->keys_valid = c->sorted_valid = c->tree_valid = c->com_valid = false;
+    c->nn_valid = c->rho_valid = c->eos_valid = false;
+    if (st0 != MHPC_OK) {
+        c->last_error = mhpc_status_string(st0);
+        return st0;
+    }
+    return st1;
 }

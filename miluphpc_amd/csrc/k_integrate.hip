@@ -15,6 +15,7 @@ enum { PB_X, PB_Y, PB_Z, PB_VX, PB_VY, PB_VZ, PB_AX, PB_AY, PB_AZ, PB_E, PB_DEDT
 struct PCArgs {
     int64_t n;
     double dt;
+    const double *dt_dev;   // when set, the time step is read from here (computed on the device)
     double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *gx, *gy, *gz;
     double *e, *dedt;
     double *b[PB_COUNT_];
@@ -25,19 +26,20 @@ struct PCArgs {
 __global__ __launch_bounds__(256) void k_pc_predictor(PCArgs A) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= A.n) return;
+    const double dt = A.dt_dev ? *A.dt_dev : A.dt;
     const double x = A.x[i], y = A.y[i], z = A.z[i], vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
     const double ax = A.ax[i], ay = A.ay[i], az = A.az[i], e = A.e[i], dedt = A.dedt[i];
     A.b[PB_X][i] = x; A.b[PB_Y][i] = y; A.b[PB_Z][i] = z;
     A.b[PB_VX][i] = vx; A.b[PB_VY][i] = vy; A.b[PB_VZ][i] = vz;
     A.b[PB_AX][i] = ax; A.b[PB_AY][i] = ay; A.b[PB_AZ][i] = az;
     A.b[PB_E][i] = e; A.b[PB_DEDT][i] = dedt;
-    A.x[i] = x + A.dt * vx;
-    A.vx[i] = vx + A.dt * (ax + A.gx[i]);
-    A.y[i] = y + A.dt * vy;
-    A.vy[i] = vy + A.dt * (ay + A.gy[i]);
-    A.z[i] = z + A.dt * vz;
-    A.vz[i] = vz + A.dt * (az + A.gz[i]);
-    double ep = e + A.dt * dedt;
+    A.x[i] = x + dt * vx;
+    A.vx[i] = vx + dt * (ax + A.gx[i]);
+    A.y[i] = y + dt * vy;
+    A.vy[i] = vy + dt * (ay + A.gy[i]);
+    A.z[i] = z + dt * vz;
+    A.vz[i] = vz + dt * (az + A.gz[i]);
+    double ep = e + dt * dedt;
     if (ep < 1e-6) ep = 1e-6;
     A.e[i] = ep;
 }
@@ -47,7 +49,7 @@ __global__ __launch_bounds__(256) void k_pc_predictor(PCArgs A) {
 __global__ __launch_bounds__(256) void k_pc_corrector(PCArgs A) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= A.n) return;
-    const double dt = A.dt;
+    const double dt = A.dt_dev ? *A.dt_dev : A.dt;
     const double vxp = A.vx[i], vyp = A.vy[i], vzp = A.vz[i];
     const double axp = A.ax[i], ayp = A.ay[i], azp = A.az[i];
     const double vx0 = A.b[PB_VX][i], vy0 = A.b[PB_VY][i], vz0 = A.b[PB_VZ][i];
@@ -129,9 +131,11 @@ __global__ __launch_bounds__(256) void k_time_step_partial(int64_t n, const doub
 }
 
 // one wavefront folds the block partials: each lane a strided share, then the wave (minima and
-// maxima are exact, so the result does not depend on the order)
+// maxima are exact, so the result does not depend on the order).  With R.dt_out set, lane 0
+// also applies the time-step rule of mhpc_time_step (the same operations in the same order)
+// and writes dt there, for a predictor and corrector that read it on the device.
 __global__ __launch_bounds__(64) void k_time_step_final(const double *__restrict__ partial, int nb,
-                                                        double *__restrict__ out) {
+                                                        double *__restrict__ out, DtRule R) {
     const int l = threadIdx.x;
     double v0 = 1.7976931348623157e308, v1 = v0, v2 = v0, v3 = 0.0, v4 = 0.0;
     for (int b = l; b < nb; b += 64) {
@@ -148,14 +152,16 @@ __global__ __launch_bounds__(64) void k_time_step_final(const double *__restrict
     v4 = wave_max(v4);
     if (l == 0) {
         out[0] = v0; out[1] = v1; out[2] = v2; out[3] = v3; out[4] = v4;
+        if (R.dt_out) *R.dt_out = dt_rule(R, v0, v1, v2, v3, v4);
     }
 }
 
-static PCArgs pc_args(mhpc_ctx *c, double dt) {
+static PCArgs pc_args(mhpc_ctx *c, double dt, const double *dt_dev) {
     PCArgs a;
     double **f = c->f;
     a.n = c->n;
     a.dt = dt;
+    a.dt_dev = dt_dev;
     a.x = f[MHPC_F_X]; a.y = f[MHPC_F_Y]; a.z = f[MHPC_F_Z];
     a.vx = f[MHPC_F_VX]; a.vy = f[MHPC_F_VY]; a.vz = f[MHPC_F_VZ];
     a.ax = f[MHPC_F_AX]; a.ay = f[MHPC_F_AY]; a.az = f[MHPC_F_AZ];
@@ -167,19 +173,19 @@ static PCArgs pc_args(mhpc_ctx *c, double dt) {
 
 int pc_backup_fields() { return PB_COUNT_; }
 
-void launch_pc_predictor(mhpc_ctx *c, double dt) {
+void launch_pc_predictor(mhpc_ctx *c, double dt, const double *dt_dev) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_pc_predictor, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream,
-                       pc_args(c, dt));
+                       pc_args(c, dt, dt_dev));
 }
 
-void launch_pc_corrector(mhpc_ctx *c, double dt) {
+void launch_pc_corrector(mhpc_ctx *c, double dt, const double *dt_dev) {
     if (c->n <= 0) return;
     hipLaunchKernelGGL(k_pc_corrector, dim3((unsigned)ceil_div(c->n, 256)), dim3(256), 0, c->stream,
-                       pc_args(c, dt));
+                       pc_args(c, dt, dt_dev));
 }
 
-void launch_time_step(mhpc_ctx *c, double *partial, double *out) {
+void launch_time_step(mhpc_ctx *c, double *partial, double *out, const DtRule &rule) {
     int nb = (int)ceil_div(c->n, 256 * 4);
     if (nb < 1) nb = 1;
     if (nb > TS_BLOCKS) nb = TS_BLOCKS;
@@ -188,5 +194,5 @@ void launch_time_step(mhpc_ctx *c, double *partial, double *out) {
                        f[MHPC_F_AY], f[MHPC_F_AZ], f[MHPC_F_G_AX], f[MHPC_F_G_AY], f[MHPC_F_G_AZ],
                        f[MHPC_F_SML], f[MHPC_F_CS], f[MHPC_F_MUIJMAX], f[MHPC_F_VX], f[MHPC_F_VY],
                        f[MHPC_F_VZ], c->p.alpha, c->p.beta, partial);
-    hipLaunchKernelGGL(k_time_step_final, dim3(1), dim3(64), 0, c->stream, partial, nb, out);
+    hipLaunchKernelGGL(k_time_step_final, dim3(1), dim3(64), 0, c->stream, partial, nb, out, rule);
 }

@@ -32,7 +32,7 @@
 #define MHPC_TREE_TILE 1024     // positions per tree block (256 threads x 4)
 #define MHPC_HTABLE_MAX 4096    // h-iteration factor table size
 #define MHPC_MAX_PASSES 64       // upper bound of max_counts
-#define MHPC_NUM_EVENTS 24
+#define MHPC_NUM_EVENTS 32   // two sets of 16: an RHS, and the second RHS of a fused PC-Euler step
 
 // neighbour-walk start table: entries of levels 1 .. K-1 come before level K
 #define MHPC_NN_KMAX 8
@@ -68,6 +68,7 @@ struct DevStatus {
     uint32_t hlist_count[2 * (MHPC_MAX_PASSES + 1)];
     uint32_t hchanged[MHPC_MAX_PASSES + 1];      // set when a pass changed some h (follows
                                                  // hlist_count: k_h_init zeroes both)
+    double dt;                                   // the fused PC-Euler step's time step
 };
 
 struct mhpc_ctx {
@@ -157,6 +158,8 @@ struct mhpc_ctx {
     double *box = nullptr;                       // [6]
     DevStatus *dstatus = nullptr;                // device block: err, counters, cell_base, list lengths
     DevStatus *h_status = nullptr;               // its pinned host mirror
+    DevStatus *h_status1 = nullptr;              // a second pinned copy: the first RHS of a fused
+                                                 // PC-Euler step
     uint32_t *err = nullptr;                     // = &dstatus->err
     unsigned long long *counters = nullptr;      // = dstatus->counters
     hipEvent_t ev[MHPC_NUM_EVENTS] = {};
@@ -217,9 +220,26 @@ void launch_mlp_targets(mhpc_ctx *c);
 void launch_mlp_sample(mhpc_ctx *c, int64_t nsamples, uint64_t seed);
 void launch_mlp_renorm(mhpc_ctx *c);
 int pc_backup_fields();
-void launch_pc_predictor(mhpc_ctx *c, double dt);
-void launch_pc_corrector(mhpc_ctx *c, double dt);
-void launch_time_step(mhpc_ctx *c, double *partial, double *out);
+// The CFL time-step rule of setTimeStep (device_predictor_corrector_euler.cu:315-540) from the
+// five folded terms q = {min sqrt(h/|a+g|), min h/cs, min h/(cs + 1.2 (alpha cs + beta mu)),
+// max |v|, max h}: min(courant q1, forces q0, courant q2, q4 / q3 (when both > 0), dt_max (when
+// > 0), t_left (when > 0)).  Host (mhpc_time_step) and device (the fused step) use this one
+// function, so both give the same dt.
+struct DtRule {
+    double courant, forces, dt_max, t_left;
+    double *dt_out;   // device: where k_time_step_final writes dt (nullptr: not computed)
+};
+__host__ __device__ inline double dt_rule(const DtRule &R, double q0, double q1, double q2, double q3, double q4) {
+    double t = fmin(R.courant * q1, R.forces * q0);
+    t = fmin(t, R.courant * q2);
+    if (q3 > 0.0 && q4 > 0.0) t = fmin(t, q4 / q3);
+    if (R.dt_max > 0.0) t = fmin(t, R.dt_max);
+    if (R.t_left > 0.0) t = fmin(t, R.t_left);
+    return t;
+}
+void launch_pc_predictor(mhpc_ctx *c, double dt, const double *dt_dev = nullptr);
+void launch_pc_corrector(mhpc_ctx *c, double dt, const double *dt_dev = nullptr);
+void launch_time_step(mhpc_ctx *c, double *partial, double *out, const DtRule &rule);
 int diag_num_quantities();
 int diag_partial_size();
 void launch_diagnostics(mhpc_ctx *c, double *partial, mhpc_diagnostics *out);

@@ -183,6 +183,11 @@ void Miluphpc::check(int status, const char *where) {
 real Miluphpc::rhs(int step, bool selfGravity, bool assignParticlesToProcess) {
     (void)step;
     (void)assignParticlesToProcess;
+    check(mhpc_rhs(ctx_, rhsFlags(selfGravity), &stats_), "rhs");
+    return (real)stats_.ms_total;
+}
+
+uint32_t Miluphpc::rhsFlags(bool selfGravity) const {
     uint32_t flags = 0;
     if (simulationParameters.gravitySim && selfGravity) flags |= MHPC_RHS_GRAVITY;
     if (simulationParameters.sphSim) flags |= MHPC_RHS_SPH;
@@ -192,8 +197,7 @@ real Miluphpc::rhs(int step, bool selfGravity, bool assignParticlesToProcess) {
         if (simulationParameters.learnedSML) flags |= MHPC_RHS_LEARNED_H;
     }
     if (simulationParameters.calculateEnergy && (flags & MHPC_RHS_GRAVITY)) flags |= MHPC_RHS_POTENTIAL;
-    check(mhpc_rhs(ctx_, flags, &stats_), "rhs");
-    return (real)stats_.ms_total;
+    return flags;
 }
 
 real Miluphpc::energy() {

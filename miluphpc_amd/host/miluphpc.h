@@ -82,4 +82,6 @@ protected:
     mhpc_ctx *ctx_ = nullptr;
     mhpc_rhs_stats stats_{};
     void check(int status, const char *where);
+    // the MHPC_RHS_* flags of rhs(step, selfGravity, ...) under the simulation parameters
+    uint32_t rhsFlags(bool selfGravity) const;
 };

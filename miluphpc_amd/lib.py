@@ -193,6 +193,8 @@ def _load() -> C.CDLL:
         "mhpc_time_step": (i32, [vp, C.c_double, C.c_double, C.c_double, dp, dp]),
         "mhpc_pc_predictor": (i32, [vp, C.c_double]),
         "mhpc_pc_corrector": (i32, [vp, C.c_double]),
+        "mhpc_pc_euler_step": (i32, [vp, u32, C.c_double, C.c_double, C.c_double, C.c_double, dp,
+                                     C.POINTER(RhsStats), C.POINTER(RhsStats)]),
         "mhpc_leapfrog_update_x": (i32, [vp, C.c_double]),
         "mhpc_leapfrog_update_v": (i32, [vp, C.c_double]),
         "mhpc_mlp_init": (i32, [vp, C.c_uint64]),
@@ -403,7 +405,16 @@ class Context:
         `flags`, CFL dt, predictor, rhs without self-gravity, corrector.  Returns
         (dt, stats of the first rhs, stats of the predictor rhs).  With RHS_LEARNED_H in
         `flags` and `train` given (mlp_train keyword arguments), the network is trained on
-        the pairs each rhs's h-iteration emits, right after that rhs."""
+        the pairs each rhs's h-iteration emits, right after that rhs.  Without training and
+        learned-h features the whole step is one call (mhpc_pc_euler_step), with no wait between
+        its kernels."""
+        if train is None and not flags & (RHS_LEARNED_H | RHS_FEATURES):
+            dt = C.c_double(0.0)
+            a, b = RhsStats(), RhsStats()
+            self._check(LIB.mhpc_pc_euler_step(self._h, C.c_uint32(flags), float(courant_fact), float(forces_fact),
+                                               float(dt_max), -1.0 if t_left is None else float(t_left),
+                                               C.byref(dt), C.byref(a), C.byref(b)), "mhpc_pc_euler_step")
+            return float(dt.value), a.as_dict(), b.as_dict()
         s0 = self.rhs(flags)
         if train is not None:
             s0["mlp_loss"] = self.mlp_train(**train)

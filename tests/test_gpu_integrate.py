@@ -48,3 +48,62 @@ def test_pc_euler_sedov_matches_oracle(gpu_lib, orc):
         assert np.max(np.abs(g[k] - o[k])) <= 1e-10 * scale, k
     # the blast is moving: energy went from the centre into kinetic motion
     assert np.abs(g["vx"]).max() > 0.0
+
+
+def separate_pc_step(ctx, lib, flags, courant, t_left=None):
+    """The five calls mhpc_pc_euler_step stands for, each waiting for its work."""
+    s0 = ctx.rhs(flags)
+    dt, _ = ctx.time_step(courant, 0.2, -1.0)
+    if t_left is not None:
+        dt = min(dt, t_left)
+    ctx.pc_predictor(dt)
+    s1 = ctx.rhs(flags & ~lib.RHS_GRAVITY)
+    ctx.pc_corrector(dt)
+    return dt, s0, s1
+
+
+@pytest.mark.parametrize("speculative", [False, True])
+def test_fused_pc_step_equals_separate_calls(gpu_lib, speculative):
+    """mhpc_pc_euler_step queues rhs, the CFL step (computed on the device), predictor, rhs and
+    corrector without waiting in between: every field, every dt and the statistics equal those
+    of the five calls made one by one, bit for bit, over four steps (the last one limited by
+    the remaining time)."""
+    print_case(f"fused pc-euler step vs separate calls, sedov 16^3, speculative={speculative}")
+    s = ics.sedov_lattice(16)
+    n = len(s["x"])
+    for k in ("ax", "ay", "az", "dedt", "g_ax", "g_ay", "g_az", "muijmax", "drhodt", "dsmldt"):
+        s.setdefault(k, np.zeros(n))
+    flags = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML | (gpu_lib.RHS_SPECULATIVE_NN if speculative else 0)
+    out = {}
+    for fused in (True, False):
+        with gpu_lib.Context(n, ics.params_sedov(sml_max=4.0 * float(s["sml"].max()))) as ctx:
+            ctx.load(**{k: v for k, v in s.items() if k in gpu_lib.FIELD_ID})
+            res = []
+            for i in range(4):
+                t_left = 1e-5 if i == 3 else None
+                if fused:
+                    res.append(ctx.pc_euler_step(flags, courant_fact=0.3, forces_fact=0.2, t_left=t_left))
+                else:
+                    res.append(separate_pc_step(ctx, gpu_lib, flags, 0.3, t_left))
+            out[fused] = (res, {k: ctx.download(k) for k in FIELDS + ("noi", "h_iters")})
+    (rf, gf), (rs, gs) = out[True], out[False]
+    for (dtf, a0, a1), (dts, b0, b1) in zip(rf, rs):
+        assert dtf == dts
+        for a, b in ((a0, b0), (a1, b1)):
+            for k in ("n_neighbours", "n_h_counts", "max_noi", "num_cells", "h_passes"):
+                assert a[k] == b[k], k
+    assert rf[3][0] == 1e-5
+    for k in gf:
+        assert np.array_equal(gf[k], gs[k]), k
+
+
+def test_fused_pc_step_refuses_learned_flags(gpu_lib):
+    s = ics.sedov_lattice(8)
+    with gpu_lib.Context(len(s["x"]), ics.params_sedov()) as ctx:
+        ctx.load(**{k: v for k, v in s.items() if k in gpu_lib.FIELD_ID})
+        import ctypes as C
+        dt = C.c_double(0.0)
+        st = gpu_lib.LIB.mhpc_pc_euler_step(ctx._h, C.c_uint32(gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML |
+                                                             gpu_lib.RHS_FEATURES),
+                                            0.3, 0.2, -1.0, -1.0, C.byref(dt), None, None)
+        assert st == gpu_lib.E_ARG
