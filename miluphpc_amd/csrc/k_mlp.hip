@@ -4,10 +4,11 @@
 // particle's state; the plain h-iteration starts from the prediction and stays the judge.
 // No counterpart in the reference.
 //
-// Work split: one wavefront owns 64 samples.  Per layer it computes a 64 x 64 tile product
-// as 4 x 4 output blocks of 16 x 16 with K in steps of 16; activations are staged in LDS as
-// BF16 row-major [sample][feature].  Training writes per-wave gradient partials that one
-// block folds in wave order (deterministic), then Adam updates the weights.
+// Work split, training: one wavefront owns 64 samples.  Per layer it computes a 64 x 64 tile
+// product as 4 x 4 output blocks of 16 x 16 with K in steps of 16; activations are staged in
+// LDS as BF16 row-major [sample][feature] (the backward pass reads them).  Training writes
+// per-wave gradient partials that one block folds in wave order (deterministic), then Adam
+// updates the weights.  Inference keeps the activations in registers (mlp_forward_16).
 #include <math.h>
 
 #include "mhpc_internal.h"
@@ -183,9 +184,84 @@ void launch_mlp_renorm(mhpc_ctx *c) {
                        c->mlp_norm, c->mlp_x);
 }
 
+// Inference with the activations kept in registers.  The layers are computed transposed --
+// hidden units as rows, samples as columns -- so the fragment a lane holds after one layer,
+// D[4q + r][c] (q = lane / 16, c = lane % 16, r = 0..3), is exactly its B operand of the next
+// layer, B[k = 4q + r][c]: no LDS round trip and no layout change between the layers.  The
+// weights are the A operands and stay in registers for the whole kernel.  One step takes 16
+// samples (one column block): layer 1 is 4 MFMAs (K = 16, features 8..15 zero), layer 2 is 16
+// (4 row blocks x 4 K steps), and the output a per-lane partial over the lane's 16 hidden
+// units plus two cross-lane sums.  Same BF16 roundings as the training kernel's forward
+// (inputs, weights and both layers' activations), FP32 accumulation.
+struct MlpRegs {
+    bf16x4 w1[4];                         // W1[mt*16 + c][4q .. 4q+3]
+    bf16x4 w2[4][4];                      // W2[mt*16 + c][kt*16 + 4q .. +3]
+    float b1[4][4], b2[4][4], w3[4][4];   // [mt][r]: hidden unit mt*16 + 4q + r
+    float b3;
+};
+
+__device__ __forceinline__ void load_regs(MlpRegs &R, const float *__restrict__ p) {
+    const int lane = wave_lane(), c = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = 4 * q + j;
+            R.w1[mt][j] = (short)(k < MLP_IN ? f2bf(p[P_W1 + (mt * 16 + c) * MLP_IN + k]) : 0);
+        }
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                R.w2[mt][kt][j] = (short)f2bf(p[P_W2 + (mt * 16 + c) * MLP_H + kt * 16 + 4 * q + j]);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int hu = mt * 16 + 4 * q + r;
+            R.b1[mt][r] = p[P_B1 + hu];
+            R.b2[mt][r] = p[P_B2 + hu];
+            R.w3[mt][r] = p[P_W3 + hu];
+        }
+    }
+    R.b3 = p[P_B3];
+}
+
+// the network's output for samples s0 .. s0 + 15 (lane c and its copies c + 16, c + 32, c + 48
+// return the value of sample s0 + c; samples at or beyond n read zero features)
+__device__ __forceinline__ float mlp_forward_16(const MlpRegs &R, const float *__restrict__ X, int64_t s0,
+                                                int64_t n) {
+    const int lane = wave_lane(), c = lane & 15, q = lane >> 4;
+    bf16x4 xb = {0, 0, 0, 0};
+    const int64_t s = s0 + c;
+    if (q < 2 && s < n) {
+        const float4 v = *reinterpret_cast<const float4 *>(X + s * MLP_IN + 4 * q);
+        xb[0] = (short)f2bf(v.x);
+        xb[1] = (short)f2bf(v.y);
+        xb[2] = (short)f2bf(v.z);
+        xb[3] = (short)f2bf(v.w);
+    }
+    bf16x4 a1[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const f32x4 d = mfma16(R.w1[mt], xb, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+        for (int r = 0; r < 4; r++) a1[mt][r] = (short)f2bf(tanh_fast(d[r] + R.b1[mt][r]));
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) d = mfma16(R.w2[mt][kt], a1[kt], d);
+#pragma unroll
+        for (int r = 0; r < 4; r++) part += bf2f(f2bf(tanh_fast(d[r] + R.b2[mt][r]))) * R.w3[mt][r];
+    }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    return part + R.b3;
+}
+
 // inference: h = h_prev * exp(y * sigma_t + mu_t), clamped to [1/4, 4] x h_prev.  A bounded
-// grid whose wavefronts loop over 64-sample tiles: each block converts the weights into LDS once
-// (4,801 loads and BF16 conversions), not once per 256 samples.
+// grid of wavefronts looping over 16-sample steps; each wave converts its weight fragments once.
 #define MLP_INF_WAVES 4
 #define MLP_INF_BLOCKS 1024
 __global__ __launch_bounds__(64 * MLP_INF_WAVES) void k_mlp_predict(int64_t n, const float *__restrict__ X,
@@ -193,26 +269,19 @@ __global__ __launch_bounds__(64 * MLP_INF_WAVES) void k_mlp_predict(int64_t n, c
                                                                    const float *__restrict__ norm,
                                                                    const double *__restrict__ hprev,
                                                                    double *__restrict__ h_out) {
-    __shared__ MlpLds W;
-    __shared__ MlpWave Sw[MLP_INF_WAVES];
-    load_weights(W, params);
-    __syncthreads();
+    MlpRegs R;
+    load_regs(R, params);
     const int w = threadIdx.x / 64, lane = wave_lane();
-    MlpWave &S = Sw[w];
-    for (int64_t base = ((int64_t)blockIdx.x * MLP_INF_WAVES + w) * 64; base < n;
-         base += (int64_t)gridDim.x * MLP_INF_WAVES * 64) {
-        const int64_t i = base + lane;
-        const bool valid = i < n;
-#pragma unroll
-        for (int k = 0; k < MLP_KPAD; k++) S.x[lane][k] = (valid && k < MLP_IN) ? f2bf(X[i * MLP_IN + k]) : 0;
-        wave_sync();
-        const float y = mlp_forward_wave(W, S);
-        if (valid) {
-            double t = (double)y * norm[16 + 1] + norm[16];
+    const double mu = norm[16], sd = norm[16 + 1];
+    for (int64_t s0 = ((int64_t)blockIdx.x * MLP_INF_WAVES + w) * 16; s0 < n;
+         s0 += (int64_t)gridDim.x * MLP_INF_WAVES * 16) {
+        const float y = mlp_forward_16(R, X, s0, n);
+        const int64_t i = s0 + lane;
+        if (lane < 16 && i < n) {
+            double t = (double)y * sd + mu;
             t = fmin(fmax(t, -1.3862943611198906), 1.3862943611198906);   // ln 4
             h_out[i] = hprev[i] * exp(t);
         }
-        wave_sync();   // the tile's activations are read before the next tile overwrites them
     }
 }
 
@@ -220,21 +289,15 @@ __global__ __launch_bounds__(64 * MLP_INF_WAVES) void k_mlp_predict(int64_t n, c
 __global__ __launch_bounds__(64 * MLP_INF_WAVES) void k_mlp_forward_raw(int64_t n, const float *__restrict__ X,
                                                                        const float *__restrict__ params,
                                                                        float *__restrict__ Y) {
-    __shared__ MlpLds W;
-    __shared__ MlpWave Sw[MLP_INF_WAVES];
-    load_weights(W, params);
-    __syncthreads();
+    MlpRegs R;
+    load_regs(R, params);
     const int w = threadIdx.x / 64, lane = wave_lane();
-    MlpWave &S = Sw[w];
-    const int64_t base = ((int64_t)blockIdx.x * MLP_INF_WAVES + w) * 64;
-    if (base >= n) return;
-    const int64_t i = base + lane;
-    const bool valid = i < n;
-#pragma unroll
-    for (int k = 0; k < MLP_KPAD; k++) S.x[lane][k] = (valid && k < MLP_IN) ? f2bf(X[i * MLP_IN + k]) : 0;
-    wave_sync();
-    const float y = mlp_forward_wave(W, S);
-    if (valid) Y[i] = y;
+    for (int64_t s0 = ((int64_t)blockIdx.x * MLP_INF_WAVES + w) * 16; s0 < n;
+         s0 += (int64_t)gridDim.x * MLP_INF_WAVES * 16) {
+        const float y = mlp_forward_16(R, X, s0, n);
+        const int64_t i = s0 + lane;
+        if (lane < 16 && i < n) Y[i] = y;
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -423,13 +486,13 @@ void launch_mlp_features(mhpc_ctx *c, double dt) {
 
 void launch_mlp_predict(mhpc_ctx *c) {
     if (c->n <= 0) return;
-    const unsigned g = (unsigned)std::min<int64_t>(ceil_div(c->n, 64 * MLP_INF_WAVES), MLP_INF_BLOCKS);
+    const unsigned g = (unsigned)std::min<int64_t>(ceil_div(c->n, 16 * MLP_INF_WAVES), MLP_INF_BLOCKS);
     hipLaunchKernelGGL(k_mlp_predict, dim3(g), dim3(64 * MLP_INF_WAVES), 0, c->stream, c->n, c->mlp_x,
                        c->mlp_params, c->mlp_norm, c->mlp_hprev, c->f[MHPC_F_SML]);
 }
 
 void launch_mlp_forward_raw(mhpc_ctx *c, int64_t n, const float *X, float *Y) {
-    const unsigned g = (unsigned)ceil_div(n, 64 * MLP_INF_WAVES);
+    const unsigned g = (unsigned)std::min<int64_t>(ceil_div(n, 16 * MLP_INF_WAVES), MLP_INF_BLOCKS);
     hipLaunchKernelGGL(k_mlp_forward_raw, dim3(g), dim3(64 * MLP_INF_WAVES), 0, c->stream, n, X,
                        c->mlp_params, Y);
 }
