@@ -77,10 +77,10 @@ def median_stats(stats):
 
 
 def time_steps(step, steps: int, warmup: int, label: str, ctx=None):
-    """W untimed steps, then K steps each bracketed by device synchronisations; returns the
-    per-step wall times (s), the total wall time of the K steps and the steps' results.  With
-    ctx, the context's own (spinning) synchronisation runs first, so the device-wide one that
-    follows finds the work done."""
+    """W untimed steps, then K steps, the timed region bracketed by device synchronisations;
+    returns the per-step wall times (s), the total wall time of the K steps and the steps'
+    results.  Each step ends with a synchronisation: the context's own (spinning) one when ctx
+    is given -- all of a step's work is on the context's stream -- else the device-wide one."""
     for i in range(warmup):
         step()
         log(f"{label}: warmup {i + 1}/{warmup} done")
@@ -92,8 +92,10 @@ def time_steps(step, steps: int, warmup: int, label: str, ctx=None):
         res.append(step())
         if ctx is not None:
             ctx.synchronize()
-        sync()
+        else:
+            sync()
         walls.append(time.perf_counter() - t0)
+    sync()
     total = time.perf_counter() - t_all
     log(f"{label}: {steps} timed steps, median {np.median(walls) * 1e3:.3f} ms/step")
     return np.array(walls), total, res
