@@ -1177,7 +1177,7 @@ __global__ __launch_bounds__(256) void k_hmax_raise(const int32_t *__restrict__ 
 }
 
 static void launch_hmax_raise(mhpc_ctx *c, const int32_t *list, const uint32_t *count) {
-    hipLaunchKernelGGL(k_hmax_raise, dim3(256), dim3(256), 0, c->stream, list, count, c->sh, c->ppar, c->cpar,
+    hipLaunchKernelGGL(k_hmax_raise, dim3(64), dim3(256), 0, c->stream, list, count, c->sh, c->ppar, c->cpar,
                        c->chmax, c->cap_cells);
 }
 
@@ -1254,12 +1254,13 @@ void launch_h_iterate(mhpc_ctx *c, int passes, bool speculative, hipEvent_t *wal
         hipLaunchKernelGGL(k_nn_count_direct, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, d);
         a.slot_list = c->hhard;
         a.slot_count = hard_count;
-        hipLaunchKernelGGL(k_nn_count_split, dim3(g1), dim3(64 * NN1_WAVES), 0, c->stream, a);
+        // the split work is 8 wavefronts per particle the direct form declined: usually none
+        hipLaunchKernelGGL(k_nn_count_split, dim3(std::min(g1, 256u)), dim3(64 * NN1_WAVES), 0, c->stream, a);
         NNArgs f = a;
         f.chmax_raise = c->chmax;
         f.ppar = c->ppar;
         f.cpar = c->cpar;
-        hipLaunchKernelGGL(k_h_finish, dim3(256), dim3(256), 0, c->stream, f);
+        hipLaunchKernelGGL(k_h_finish, dim3(64), dim3(256), 0, c->stream, f);
     }
 }
 

@@ -405,6 +405,10 @@ void launch_bottom_up(mhpc_ctx *c, bool gravity, bool sph) {
     if (deep < 3) deep = 3;
     if (deep > MHPC_LEVELS) deep = MHPC_LEVELS;
     if (deep < MHPC_LEVELS) run(MHPC_LEVELS, deep + 1, 1, nullptr);
-    for (int L = deep; L >= 3; L--) run(L, L, grid, nullptr);
+    for (int L = deep; L >= 3; L--) {
+        // level L holds at most min(8^L, n) cells: no more workgroups than that needs
+        const int64_t cells = L < 7 ? std::min<int64_t>((int64_t)1 << (3 * L), c->n) : c->n;
+        run(L, L, (int)std::min<int64_t>(grid, ceil_div(cells * 8, BU_BLOCK * 4)), nullptr);
+    }
     run(2, 0, 1, c->counters + CNT_TREE_DEPTH);
 }
