@@ -310,7 +310,7 @@ int mhpc_pc_euler_step(mhpc_ctx *ctx, uint32_t flags, double courant_fact, doubl
 /* -------- learned smoothing length (build extension, no reference counterpart): a
  * 7 -> 64 -> 64 -> 1 tanh MLP, BF16 weights on MFMA with FP32 accumulation, predicting
  * log(h_converged / h_previous); the plain h-iteration starts from the prediction and
- * remains the judge (MHPC_RHS_LEARNED_H).  Parameters: flat FP32 vector of
+ * has the last word (MHPC_RHS_LEARNED_H).  Parameters: flat FP32 vector of
  * MHPC_MLP_NPARAM = W1[64][8] (input 8 is padding), b1[64], W2[64][64], b2[64], W3[64], b3.
  * norm[18]: feature means [0..6], feature stds [8..14], target mean [16], target std [17]. */
 #define MHPC_MLP_NPARAM 4801

@@ -1,7 +1,7 @@
 // k_mlp.hip -- learned smoothing length: a 7 -> 64 -> 64 -> 1 tanh MLP with BF16 weights
 // and activations on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32 accumulation),
 // FP32 master weights and Adam.  It predicts log(h_converged / h_previous) from a
-// particle's state; the plain h-iteration starts from the prediction and stays the judge.
+// particle's state; the plain h-iteration starts from the prediction and has the last word.
 // No counterpart in the reference.
 //
 // Work split, training: one wavefront owns 64 samples.  Per layer it computes a 64 x 64 tile

@@ -12,7 +12,7 @@ for p in (REPO, REPO / "oracle"):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs a real MI355X (run on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs a real MI355X")
     config.addinivalue_line("markers", "slow: CPU test that takes more than a few seconds")
 
 
