@@ -445,9 +445,15 @@ static int finish(mhpc_ctx *c) {
     return status_of(c, *c->h_status);
 }
 
+// ms between two events of a finished call; 0 for an event the call did not record (a stage
+// with nothing to do, e.g. no particles), whose query error is cleared here so that the next
+// call's error check does not report it
 static float elapsed_ev(const hipEvent_t *ev, int a, int b) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) {
+        ms = 0.f;
+        (void)hipGetLastError();
+    }
     return ms;
 }
 
