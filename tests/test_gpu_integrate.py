@@ -62,37 +62,47 @@ def separate_pc_step(ctx, lib, flags, courant, t_left=None):
     return dt, s0, s1
 
 
-@pytest.mark.parametrize("speculative", [False, True])
-def test_fused_pc_step_equals_separate_calls(gpu_lib, speculative):
+@pytest.mark.parametrize("case", ["sedov", "sedov-speculative", "boss-bodenheimer"])
+def test_fused_pc_step_equals_separate_calls(gpu_lib, case):
     """mhpc_pc_euler_step queues rhs, the CFL step (computed on the device), predictor, rhs and
     corrector without waiting in between: every field, every dt and the statistics equal those
     of the five calls made one by one, bit for bit, over four steps (the last one limited by
-    the remaining time)."""
-    print_case(f"fused pc-euler step vs separate calls, sedov 16^3, speculative={speculative}")
-    s = ics.sedov_lattice(16)
+    the remaining time).  The Boss-Bodenheimer case has self-gravity in the first RHS only and
+    FP32 + compensation pair sums."""
+    print_case(f"fused pc-euler step vs separate calls, {case}")
+    if case == "boss-bodenheimer":
+        s = ics.boss_bodenheimer(32768)
+        params = ics.params_boss_bodenheimer()
+        flags = gpu_lib.RHS_GRAVITY | gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML | gpu_lib.RHS_SPECULATIVE_NN
+    else:
+        s = ics.sedov_lattice(16)
+        params = ics.params_sedov(sml_max=4.0 * float(s["sml"].max()))
+        flags = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML
+        if case == "sedov-speculative":
+            flags |= gpu_lib.RHS_SPECULATIVE_NN
     n = len(s["x"])
     for k in ("ax", "ay", "az", "dedt", "g_ax", "g_ay", "g_az", "muijmax", "drhodt", "dsmldt"):
         s.setdefault(k, np.zeros(n))
-    flags = gpu_lib.RHS_SPH | gpu_lib.RHS_VARIABLE_SML | (gpu_lib.RHS_SPECULATIVE_NN if speculative else 0)
     out = {}
     for fused in (True, False):
-        with gpu_lib.Context(n, ics.params_sedov(sml_max=4.0 * float(s["sml"].max()))) as ctx:
+        with gpu_lib.Context(n, params) as ctx:
             ctx.load(**{k: v for k, v in s.items() if k in gpu_lib.FIELD_ID})
             res = []
             for i in range(4):
-                t_left = 1e-5 if i == 3 else None
+                t_left = 0.25 * res[2][0] if i == 3 else None
                 if fused:
                     res.append(ctx.pc_euler_step(flags, courant_fact=0.3, forces_fact=0.2, t_left=t_left))
                 else:
                     res.append(separate_pc_step(ctx, gpu_lib, flags, 0.3, t_left))
-            out[fused] = (res, {k: ctx.download(k) for k in FIELDS + ("noi", "h_iters")})
+            out[fused] = (res, {k: ctx.download(k) for k in FIELDS + ("noi", "h_iters", "g_ax", "g_ay", "g_az")})
     (rf, gf), (rs, gs) = out[True], out[False]
     for (dtf, a0, a1), (dts, b0, b1) in zip(rf, rs):
         assert dtf == dts
         for a, b in ((a0, b0), (a1, b1)):
-            for k in ("n_neighbours", "n_h_counts", "max_noi", "num_cells", "h_passes"):
+            for k in ("n_neighbours", "n_h_counts", "max_noi", "num_cells", "h_passes", "n_cell_interactions",
+                      "n_part_interactions"):
                 assert a[k] == b[k], k
-    assert rf[3][0] == 1e-5
+    assert rf[3][0] == 0.25 * rf[2][0]
     for k in gf:
         assert np.array_equal(gf[k], gs[k]), k
 
