@@ -21,9 +21,9 @@
 #define NN_STACK_PHYS (NN_STACK + 192)
 #define NN_RANGES 128
 #define NN_LEAF 16
-#define NN_OUTLIER 1.5       // group-mode outlier: h above this factor x the group's smallest
+#define NN_OUTLIER 4.0       // group-mode outlier: h above this factor x the group's smallest
 #define NN_OUTLIER_MAX 32    // at most this many outliers per group are split off
-#define NN_SPARSE 32         // re-lists: groups with at most this many flagged particles go per particle
+#define NN_SPARSE 64         // re-lists: groups with at most this many flagged particles go per particle
 #define NN_DRANGES 256       // direct form: candidate ranges per wavefront
 #define NN_DSPAN 5           // direct form: start cells per axis - 1 (<= 216 cells)
 #define NN_DIRECT_MAX 32768  // direct form: candidates per wavefront

@@ -317,11 +317,11 @@ def test_outlier_particles_match_oracle(gpu_lib, orc, sedov24):
     """Particles whose h is several times their group's smallest are split off the group
     walk into the per-particle mode: neighbour sets, h-iteration and speculative lists stay
     exactly the oracle's."""
-    print_case("outliers: 5% of particles with 4x h, sedov 24^3")
+    print_case("outliers: 5% of particles with 6x h, sedov 24^3")
     rng = np.random.default_rng(77)
     s = dict(sedov24)
     big = rng.uniform(size=len(s["x"])) < 0.05
-    h0 = s["sml"] * np.where(big, 4.0, 1.0)
+    h0 = s["sml"] * np.where(big, 6.0, 1.0)   # above the split threshold (4x the group's smallest)
     p = gpu_lib.default_params(curve=1, max_nb=2000)
     with make_ctx(gpu_lib, {**s, "sml": h0}, p) as ctx:
         ctx.bbox(); ctx.keys(); ctx.sort(); ctx.build_tree()
