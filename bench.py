@@ -61,6 +61,23 @@ def measured_traffic(kernel_name: str, workload: str = "sedov128"):
     return None
 
 
+def measured_busy(kernel_name: str, workload: str = "sedov128"):
+    """Busy shares of `kernel_name` from the committed PMC summary (texture addresser, VALU,
+    LDS), or None if there is none: what bounds a kernel that is not HBM-bound."""
+    path = REPO / "profiles" / f"busy_{workload}.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split("\t")
+        if len(parts) >= 6 and parts[0].strip() == kernel_name:
+            try:
+                return {"valu": float(parts[3]), "texture_addresser": float(parts[2]), "lds": float(parts[5]),
+                        "valu_instructions_per_wave": float(parts[4]), "source": f"profiles/busy_{workload}.txt"}
+            except ValueError:
+                return None
+    return None
+
+
 def sync():
     import torch
 
@@ -302,7 +319,8 @@ def run_sedov(args, lib, ics, result):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name),
                      "algorithmic_bytes_per_launch": alg_bytes, "algorithmic_bytes_per_particle": bpp,
                      "kernel_ms": med[dom], "timing": "hipEvents around the kernel on the context stream, "
-                                                      "median over the timed RHS"},
+                                                      "median over the timed RHS",
+                     "busy": measured_busy(name)},
         "sph_kernels": kernels,
         "phases_ms": {k[3:]: round(v, 4) for k, v in phase_med.items() if k.startswith("ms_")},
         "phases_note": "per-phase times from two steps after the timed ones, with every phase timed",
