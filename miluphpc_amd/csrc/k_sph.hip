@@ -20,7 +20,9 @@
 #define NN_STACK 512
 #define NN_STACK_PHYS (NN_STACK + 192)
 #define NN_RANGES 128
-#define NN_LEAF 16
+#define NN_LEAF 16           // per-particle walks: cells of at most this many particles are taken whole
+#define NN_LEAF_GROUP 64     // group walks: the same for the 64-particle groups' boxes
+static_assert(NN_LEAF_GROUP < NN_CWIN, "a clipped range crosses at most one end of the window");
 #define NN_OUTLIER 4.0       // group-mode outlier: h above this factor x the group's smallest
 #define NN_OUTLIER_MAX 32    // at most this many outliers per group are split off
 #define NN_SPARSE 64         // re-lists: groups with at most this many flagged particles go per particle
@@ -205,16 +207,16 @@ __device__ __forceinline__ NNStart nn_start_level(const NNArgs &A, double lox, d
 }
 
 // Consumer window of a list walk: the particles [w0, w1) of the density and force kernels'
-// LDS window that holds the walking group (w1 - w0 >= NN_LEAF; w0 == w1: none).  Their
+// LDS window that holds the walking group (w1 - w0 >= leaf; w0 == w1: none).  Their
 // candidates are streamed first as one range, so every list holds its in-window neighbours
 // first, and the tree walk skips them: a node or range inside the window is dropped (false)
-// and a range of at most NN_LEAF particles is clipped to its part outside (it crosses at most
-// one end of the window).  Larger nodes that cross an end are opened as usual.
-__device__ __forceinline__ bool nn_win_clip(int w0, int w1, int &rs, int &rc) {
+// and a range of at most `leaf` (< NN_CWIN) particles is clipped to its part outside (it
+// crosses at most one end of the window).  Larger nodes that cross an end are opened as usual.
+__device__ __forceinline__ bool nn_win_clip(int w0, int w1, int &rs, int &rc, int leaf) {
     if (w0 == w1) return true;
     const int64_t e = (int64_t)rs + rc;
     if (rs >= w0 && e <= w1) return false;
-    if (rc <= NN_LEAF) {
+    if (rc <= leaf) {
         if (rs < w0 && e > w0) rc = w0 - rs;
         else if (rs < w1 && e > w1) { rc = (int)(e - w1); rs = w1; }
     }
@@ -226,7 +228,7 @@ __device__ __forceinline__ bool nn_win_clip(int w0, int w1, int &rs, int &rc) {
 // K, a particle alone in it, or none).  Every particle inside the box lies in one of these
 // cells (the keys use the same bisection), and the nodes' subtrees are disjoint, so the walk
 // sees the same candidates as one from the root, in another order.  A node of at most
-// NN_LEAF particles goes straight to the range list (its particles are candidates, as for a
+// `leaf` particles goes straight to the range list (its particles are candidates, as for a
 // leaf the walk would take), any other onto the stack.  With nparts = 8, only cells of
 // index = part (mod 8).  With a consumer window [w0, w1), its ranges come first -- the walking
 // group's own particles [g0, g0 + 64) before the rest of the window -- and the nodes are
@@ -234,7 +236,7 @@ __device__ __forceinline__ bool nn_win_clip(int w0, int w1, int &rs, int &rc) {
 template <typename LdsT>
 __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
                                              double hiy, double loz, double hiz, const NNCore &q, int part,
-                                             int nparts, int w0, int w1, int g0, int &nr) {
+                                             int nparts, int w0, int w1, int g0, int &nr, int leaf) {
     const int lane = wave_lane();
     int node = -1;
     int2 rng = make_int2(0, 0x7fffffff);
@@ -260,7 +262,7 @@ __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lo
         }
     }
     if (node >= A.n && !nn_cell_reaches(A, q, node - A.n)) node = -1;
-    if (node >= 0 && !nn_win_clip(w0, w1, rng.x, rng.y)) node = -1;
+    if (node >= 0 && !nn_win_clip(w0, w1, rng.x, rng.y, leaf)) node = -1;
     int r0 = 0;   // the window's ranges: own group, then the window before and after it
     if (w1 > w0) {
         const int g1 = g0 + 64 < w1 ? g0 + 64 : w1;
@@ -281,7 +283,7 @@ __device__ __forceinline__ int nn_start_push(LdsT &L, const NNArgs &A, double lo
         r0 = 1 + (g0 > w0 ? 1 : 0) + (w1 > g1 ? 1 : 0);
     }
     const uint64_t lt = lanemask_lt();
-    const bool asRange = node >= 0 && rng.y <= NN_LEAF;
+    const bool asRange = node >= 0 && rng.y <= leaf;
     const bool asNode = node >= 0 && !asRange;
     const uint64_t mR = wave_ballot(asRange), mN = wave_ballot(asNode);
     if (asRange) {
@@ -389,22 +391,27 @@ __device__ __forceinline__ bool nn_direct(LdsT &L, const NNArgs &A, double lox, 
 // (the core q grown by the search radius) and hand the candidate particles to
 // `tile` in LDS tiles of up to 64 (L.tx/ty/tz/th2/tj, count m).  Cells that cannot hold a
 // neighbour of the core (nn_cell_reaches) are skipped with their subtrees.  Leaves of at most
-// NN_LEAF particles and single particles are taken as ranges of sorted positions; larger
-// cells are opened.  With FILTER, only candidates with keep(x, y, z, h^2) true reach the
-// tiles (compacted in walk order, so the tile sequence is the unfiltered one minus the
-// dropped candidates).  Returns false on stack overflow.  Wave-uniform control flow.
+// NN_LEAF particles (NN_LEAF_GROUP with FILTER: the group walks) and single particles are
+// taken as ranges of sorted positions; larger cells are opened.  With FILTER, only candidates
+// with keep(x, y, z, h^2) true reach the tiles (compacted in walk order, so the tile sequence
+// is the unfiltered one minus the dropped candidates).  Returns false on stack overflow.
+// Wave-uniform control flow.
 template <bool FILTER, typename LdsT, typename KeepFn, typename TileFn>
 __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox, double hix, double loy,
                                             double hiy, double loz, double hiz, const NNCore &q, int start,
                                             int w0, int w1, int g0, KeepFn &&keep, TileFn &&tile) {
     const int lane = wave_lane();
     const int64_t n = A.n;
+    // a group's box spans a few 64-particle cells per axis: taken whole, their candidates come
+    // in the order of the start cells, which is also the order the density and force kernels
+    // gather best (same lists as sets, fewer distinct lines per gather step)
+    constexpr int leaf = FILTER ? NN_LEAF_GROUP : NN_LEAF;
     int top = 1, nr = 0;
     if (start >= 0) {
         if (lane == 0) L.stack[0] = start;
     } else {
         top = nn_start_push(L, A, lox, hix, loy, hiy, loz, hiz, q, start == NN_START_ALL ? 0 : NN_START_PART0 - start,
-                            start == NN_START_ALL ? 1 : 8, w0, w1, g0, nr);
+                            start == NN_START_ALL ? 1 : 8, w0, w1, g0, nr, leaf);
     }
     wave_sync();
     const uint64_t lt = lanemask_lt();
@@ -428,7 +435,7 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
                 take = px >= lox && px <= hix && py >= loy && py <= hiy && pz >= loz && pz <= hiz;
                 rs = node;
                 rc = 1;
-                take = take && nn_win_clip(w0, w1, rs, rc);
+                take = take && nn_win_clip(w0, w1, rs, rc, leaf);
             } else if (node >= n) {
                 c = node - n;
                 const int64_t cc = A.cap_cells;
@@ -439,8 +446,8 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
                 if (overlap) {
                     rs = A.cstart[c];
                     rc = A.cend[c] - rs + 1;
-                    if (nn_win_clip(w0, w1, rs, rc)) {
-                        if (rc <= NN_LEAF) take = true; else open = true;
+                    if (nn_win_clip(w0, w1, rs, rc, leaf)) {
+                        if (rc <= leaf) take = true; else open = true;
                     }
                 }
             }
@@ -681,7 +688,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     if (MODE != 1) {
         w0 = (int)(group * 64 / NN_CWIN * NN_CWIN);
         w1 = (int)(w0 + NN_CWIN < n ? w0 + NN_CWIN : n);
-        if (w1 - w0 < NN_LEAF) w0 = w1 = 0;
+        if (w1 - w0 < NN_LEAF_GROUP) w0 = w1 = 0;
     }
     // with the window on, the group's own particles are the first candidates streamed, so all of
     // them that pass the filter are in the first tile: only its pair tests must skip j == k
