@@ -317,8 +317,24 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
             if (A.mac == MHPC_MAC_CLASSIC) return (d2 + eps2) * theta2 > s2_0 * ldexp(1.0, -2 * L);
             return dp_1 * ldexp(1.0, -2 * (L - 1)) <= d2 + eps2;
         };
+        // everything the round may need of its node is requested together, before the test
+        // decides which of it is used: one memory round trip per round instead of one for the
+        // test and one more for the children or the list entries
+        int4 cha = make_int4(-1, -1, -1, -1), chb = cha;
+        double cmass = 0.0;
+        float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = qa;
+        double4 prec = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (isPart) prec = A.sxm[node];
         if (isCell) {
             c = node - n;
+            const int4 *cp = reinterpret_cast<const int4 *>(A.cchild + 8 * c);
+            cha = cp[0];
+            chb = cp[1];
+            cmass = A.cmass[c];
+            if (QUAD) {
+                qa = A.cquadf[2 * c];
+                qb = A.cquadf[2 * c + 1];
+            }
             ccx = A.cx[c];
             ccy = A.cy[c];
             ccz = A.cz[c];
@@ -348,7 +364,7 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
         const uint64_t mP = wave_ballot(isPart), mA = wave_ballot(accept);
         if (isPart) {
             const int pos = pl_n + popc64(mP & lt);
-            const double4 r = A.sxm[node];
+            const double4 r = prec;
             if constexpr (FP64) {
                 L.px[pos] = r.x;
                 L.py[pos] = r.y;
@@ -365,15 +381,14 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
                 L.cx[pos] = ccx;
                 L.cy[pos] = ccy;
                 L.cz[pos] = ccz;
-                L.cm[pos] = A.cmass[c];
+                L.cm[pos] = cmass;
             } else {
                 L.c[pos] = make_float4((float)(ccx - refx), (float)(ccy - refy), (float)(ccz - refz),
-                                       (float)A.cmass[c]);
+                                       (float)cmass);
             }
             if (QUAD) {
-                const float4 a = A.cquadf[2 * c], b = A.cquadf[2 * c + 1];
-                L.q0[pos] = a;
-                L.q1[pos] = make_float2(b.x, b.y);
+                L.q0[pos] = qa;
+                L.q1[pos] = make_float2(qb.x, qb.y);
             }
         }
         pl_n += popc64(mP);
@@ -384,10 +399,8 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
         int ch[8];
         int nc = 0;
         if (open) {
-            const int4 *cp = reinterpret_cast<const int4 *>(A.cchild + 8 * c);
-            const int4 a = cp[0], b = cp[1];
-            ch[0] = a.x; ch[1] = a.y; ch[2] = a.z; ch[3] = a.w;
-            ch[4] = b.x; ch[5] = b.y; ch[6] = b.z; ch[7] = b.w;
+            ch[0] = cha.x; ch[1] = cha.y; ch[2] = cha.z; ch[3] = cha.w;
+            ch[4] = chb.x; ch[5] = chb.y; ch[6] = chb.z; ch[7] = chb.w;
 #pragma unroll
             for (int s = 0; s < 8; s++) nc += ch[s] != -1;
         }
