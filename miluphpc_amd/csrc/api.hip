@@ -408,7 +408,7 @@ extern "C" int mhpc_get_box(mhpc_ctx *c, double box[6]) {
 // ------------------------------------------------------------------------------------
 static void reset_flags(mhpc_ctx *c) {
     launch_fill_u32(c, c->err, 0u, 1);
-    launch_fill_u64(c, c->counters, 0ull, CNT_COUNT_);
+    launch_fill_u64(c, c->counters, 0ull, CNT_SLOTS * CNT_LINE);
 }
 
 hipError_t ctx_sync(mhpc_ctx *c) {
@@ -427,7 +427,8 @@ hipError_t ctx_fetch_status(mhpc_ctx *c) {
 // the next sort starts from)
 static int status_of(mhpc_ctx *c, const DevStatus &hs) {
     const uint32_t e = hs.err;
-    if (hs.counters[CNT_TREE_DEPTH] > 0) c->com_depth_hint = (int)hs.counters[CNT_TREE_DEPTH];
+    const unsigned long long depth = cnt_max(hs, CNT_TREE_DEPTH);
+    if (depth > 0) c->com_depth_hint = (int)depth;
     if (e == 0) return MHPC_OK;
     int st = MHPC_OK;
     if (e & ERRF_DEPTH) st = MHPC_E_DEPTH;
@@ -1056,11 +1057,11 @@ static void rhs_stats(mhpc_ctx *c, const RhsPlan &P, const hipEvent_t *ev, const
     stats->ms_forces = P.sph ? elapsed_ev(ev, 11, 12) : 0.f;
     stats->ms_total = elapsed_ev(ev, 13, 12);
     stats->num_cells = hs.cell_base[MHPC_LEVELS + 1];
-    stats->n_cell_interactions = (int64_t)hs.counters[CNT_CELL_INT];
-    stats->n_part_interactions = (int64_t)hs.counters[CNT_PART_INT];
-    stats->n_neighbours = (int64_t)hs.counters[CNT_NEIGHBOURS];
-    stats->max_noi = (int32_t)hs.counters[CNT_MAX_NOI];
-    stats->n_h_counts = (int64_t)hs.counters[CNT_H_COUNTS];
+    stats->n_cell_interactions = (int64_t)cnt_sum(hs, CNT_CELL_INT);
+    stats->n_part_interactions = (int64_t)cnt_sum(hs, CNT_PART_INT);
+    stats->n_neighbours = (int64_t)cnt_sum(hs, CNT_NEIGHBOURS);
+    stats->max_noi = (int32_t)cnt_max(hs, CNT_MAX_NOI);
+    stats->n_h_counts = (int64_t)cnt_sum(hs, CNT_H_COUNTS);
     stats->ms_nn_walk = P.hiter ? elapsed_ev(ev, 14, 15) : 0.f;
     stats->h_passes = 0;
     if (P.hiter) {

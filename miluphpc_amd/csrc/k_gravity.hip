@@ -482,8 +482,8 @@ __global__ __launch_bounds__(64 * BH_WAVES) void k_bh_walk(BHArgs A) {
         if (POT) A.u[i] = 0.5 * own.w * A.G * acc.phi.value();
     }
     if (lane == 0) {
-        atomicAdd(&A.counters[CNT_CELL_INT], (unsigned long long)(n_cells * nvalid));
-        atomicAdd(&A.counters[CNT_PART_INT], (unsigned long long)(n_parts * nvalid));
+        atomicAdd(&cnt_slot(A.counters)[CNT_CELL_INT], (unsigned long long)(n_cells * nvalid));
+        atomicAdd(&cnt_slot(A.counters)[CNT_PART_INT], (unsigned long long)(n_parts * nvalid));
     }
 }
 
@@ -590,8 +590,8 @@ __global__ __launch_bounds__(BH1_THREADS) void k_bh_v1(BHArgs A) {
     const unsigned long long sc = (unsigned long long)wave_sum(n_cells);
     const unsigned long long sp = (unsigned long long)wave_sum(n_parts);
     if (wave_lane() == 0) {
-        atomicAdd(&A.counters[CNT_CELL_INT], sc);
-        atomicAdd(&A.counters[CNT_PART_INT], sp);
+        atomicAdd(&cnt_slot(A.counters)[CNT_CELL_INT], sc);
+        atomicAdd(&cnt_slot(A.counters)[CNT_PART_INT], sp);
     }
 }
 

@@ -787,8 +787,8 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
         unsigned long long s = (unsigned long long)wave_sum(own_list ? count : 0);
         int mx = wave_max(own_list ? count : 0);
         if (lane == 0) {
-            atomicAdd(&A.counters[CNT_NEIGHBOURS], s);
-            atomicMax(&A.counters[CNT_MAX_NOI], (unsigned long long)mx);
+            atomicAdd(&cnt_slot(A.counters)[CNT_NEIGHBOURS], s);
+            atomicMax(&cnt_slot(A.counters)[CNT_MAX_NOI], (unsigned long long)mx);
         }
     } else {
         bool stay = false;
@@ -818,7 +818,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
                 if (A.h0_slot) A.h0_slot[pos] = hi;   // the h this pass's list was built with
             }
         }
-        if (lane == 0) atomicAdd(&A.counters[CNT_H_COUNTS], (unsigned long long)popc64(mActive));
+        if (lane == 0) atomicAdd(&cnt_slot(A.counters)[CNT_H_COUNTS], (unsigned long long)popc64(mActive));
     }
 }
 
@@ -884,8 +884,8 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
                 if (A.noi_out) A.noi_out[A.perm[k]] = count;
                 if (count > A.max_nb) set_err(A.err, ERRF_CAP_NEIGHBOURS);
                 if (A.counters) {
-                    atomicAdd(&A.counters[CNT_NEIGHBOURS], (unsigned long long)count);
-                    atomicMax(&A.counters[CNT_MAX_NOI], (unsigned long long)count);
+                    atomicAdd(&cnt_slot(A.counters)[CNT_NEIGHBOURS], (unsigned long long)count);
+                    atomicMax(&cnt_slot(A.counters)[CNT_MAX_NOI], (unsigned long long)count);
                 }
             }
         } else {
@@ -901,7 +901,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
                     A.list_out[pos] = (int32_t)k;
                     if (A.h0_slot) A.h0_slot[pos] = hi;
                 }
-                atomicAdd(&A.counters[CNT_H_COUNTS], 1ull);
+                atomicAdd(&cnt_slot(A.counters)[CNT_H_COUNTS], 1ull);
             }
         }
         wave_sync();
@@ -1075,7 +1075,7 @@ __global__ __launch_bounds__(256) void k_h_finish(NNArgs A) {
         b = __shfl(b, 0);
         if (stay) A.list_out[b + popc64(mS & lt)] = (int32_t)k;
         const uint64_t mV = wave_ballot(valid);
-        if (lane == 0 && mV != 0) atomicAdd(&A.counters[CNT_H_COUNTS], (unsigned long long)popc64(mV));
+        if (lane == 0 && mV != 0) atomicAdd(&cnt_slot(A.counters)[CNT_H_COUNTS], (unsigned long long)popc64(mV));
         if (A.changed && wave_ballot(changed) != 0 && lane == 0) *A.changed = 1u;
     }
 }
@@ -1665,8 +1665,8 @@ __global__ __launch_bounds__(DENS_WIN) void k_density(int64_t n, const double4 *
                 t += psum[w];
                 m = pmax[w] > m ? pmax[w] : m;
             }
-            atomicAdd(&P.counters[CNT_NEIGHBOURS], t);
-            atomicMax(&P.counters[CNT_MAX_NOI], (unsigned long long)m);
+            atomicAdd(&cnt_slot(P.counters)[CNT_NEIGHBOURS], t);
+            atomicMax(&cnt_slot(P.counters)[CNT_MAX_NOI], (unsigned long long)m);
         }
     }
 }

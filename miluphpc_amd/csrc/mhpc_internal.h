@@ -57,11 +57,18 @@ enum {
     CNT_TREE_DEPTH = 5,     // deepest level with cells (written by the bottom-up pass)
     CNT_COUNT_ = 8
 };
+// The counters are kept in CNT_SLOTS copies, one 128-byte line each, and a workgroup adds to
+// the copy of its index mod CNT_SLOTS: atomics on one address serialise (about 12 ns each on
+// MI355X), so a kernel whose 32k wavefronts each add once spent ~0.4 ms draining them.  The
+// host sums (or takes the maximum of) the copies.
+#define CNT_SLOTS 16
+#define CNT_LINE 16
+static_assert(CNT_COUNT_ <= CNT_LINE, "one line per copy");
 
 // The words the host reads after an RHS or a stage, in one device block so that one copy
 // brings them all to the pinned host mirror (one round trip per call).
 struct DevStatus {
-    unsigned long long counters[CNT_COUNT_];
+    unsigned long long counters[CNT_SLOTS * CNT_LINE];   // copy s at [s * CNT_LINE, + CNT_COUNT_)
     uint32_t err;
     uint32_t sort_check;                         // set when a partial sort left equal prefixes
     int32_t cell_base[MHPC_LEVELS + 3];          // level-major cell offsets, [23] = total
@@ -306,6 +313,22 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
         int w = __shfl_up(v, o);
         if (l >= o) v += w;
     }
+    return v;
+}
+
+// the calling workgroup's copy of the counters
+__device__ __forceinline__ unsigned long long *cnt_slot(unsigned long long *counters) {
+    return counters + (blockIdx.x % CNT_SLOTS) * CNT_LINE;
+}
+// a counter from a fetched status block: the sum of the copies (the maximum for max-counters)
+inline unsigned long long cnt_sum(const DevStatus &hs, int q) {
+    unsigned long long v = 0;
+    for (int s = 0; s < CNT_SLOTS; s++) v += hs.counters[s * CNT_LINE + q];
+    return v;
+}
+inline unsigned long long cnt_max(const DevStatus &hs, int q) {
+    unsigned long long v = 0;
+    for (int s = 0; s < CNT_SLOTS; s++) v = hs.counters[s * CNT_LINE + q] > v ? hs.counters[s * CNT_LINE + q] : v;
     return v;
 }
 
