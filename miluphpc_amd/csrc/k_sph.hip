@@ -121,14 +121,19 @@ struct NNCore {
     double x0, x1, y0, y1, z0, z1;
     double h;
 };
+
+// the squared distance of every neighbour test (and of the bounds that prune them):
+// fma(dx, dx, fma(dy, dy, dz^2)), the oracle's form (two fused multiply-adds per test)
+__device__ __forceinline__ double nn_d2(double dx, double dy, double dz) { return fma(dx, dx, fma(dy, dy, dz * dz)); }
 __device__ __forceinline__ NNCore nn_point(double x, double y, double z, double h) {
     return NNCore{x, x, y, y, z, z, h};
 }
 
 // Can cell c hold a neighbour j of some particle i of the core?  A pair needs d < h_i^2 and
 // d < h_j^2, and the squared distance from the core to the cell's particle box, taken per axis
-// as max(0, lo_cell - hi_core, lo_core - hi_cell) and summed in the pair test's order, is at
-// most every such pair's d (the roundings are monotone), while h_i <= q.h and h_j <= chmax: a
+// as max(0, lo_cell - hi_core, lo_core - hi_cell) and combined in the pair test's form
+// (nn_d2), is at most every such pair's d (each rounding is monotone in exact values that
+// grow with the per-axis distances), while h_i <= q.h and h_j <= chmax: a
 // cell that fails d2 < min(q.h, chmax)^2 holds no neighbour of the core and its subtree is
 // skipped.
 __device__ __forceinline__ bool nn_cell_reaches(const NNArgs &A, const NNCore &q, int64_t c) {
@@ -137,7 +142,7 @@ __device__ __forceinline__ bool nn_cell_reaches(const NNArgs &A, const NNCore &q
     const double ey = fmax(fmax(A.cbmin[cc + c] - q.y1, q.y0 - A.cbmax[cc + c]), 0.0);
     const double ez = fmax(fmax(A.cbmin[2 * cc + c] - q.z1, q.z0 - A.cbmax[2 * cc + c]), 0.0);
     const double hm = fmin(A.chmax[c], q.h);
-    return ex * ex + ey * ey + ez * ez < hm * hm;
+    return nn_d2(ex, ey, ez) < hm * hm;
 }
 
 // Raise the cells' h bounds (chmax) to a particle's new h: every ancestor cell whose bound is
@@ -661,14 +666,15 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
     const double loz = bz0 - grow, hiz = bz1 + grow;
     // candidate filter: squared distance from the box of the active positions, below both
     // hmax^2 and h_j^2.  Per axis fl(b0 - x) <= fl(x_i - x) for every x_i >= b0 (rounding is
-    // monotone), so this distance never exceeds the pair test's d for any active i, and every
+    // monotone), and nn_d2 is monotone in each |axis distance|, so this distance never exceeds
+    // the pair test's d for any active i, and every
     // pair with d < h_i^2 <= hmax^2 and d < h_j^2 passes: the tiles lose only non-pairs.
     const double hm2 = hmax * hmax;
     auto keep = [&](double x, double y, double z, double h2) {
         const double ex = fmax(fmax(bx0 - x, x - bx1), 0.0);
         const double ey = fmax(fmax(by0 - y, y - by1), 0.0);
         const double ez = fmax(fmax(bz0 - z, z - bz1), 0.0);
-        const double d = ex * ex + ey * ey + ez * ez;
+        const double d = nn_d2(ex, ey, ez);
         return d < hm2 && d < h2;
     };
 
@@ -702,7 +708,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
                 const double dx = xi - L.tx[e];
                 const double dy = yi - L.ty[e];
                 const double dz = zi - L.tz[e];
-                const double d = dx * dx + dy * dy + dz * dz;
+                const double d = nn_d2(dx, dy, dz);
                 count += (d < hi2 && d < L.th2[e] && L.tj[e] != (int)k) ? 1 : 0;
             }
             return;
@@ -732,7 +738,7 @@ __global__ __launch_bounds__(64 * NN_WAVES) void k_nn_walk(NNArgs A) {
                         const double dx = xi - xs[u];
                         const double dy = yi - ys[u];
                         const double dz = zi - zs[u];
-                        const double d = dx * dx + dy * dy + dz * dz;
+                        const double d = nn_d2(dx, dy, dz);
                         const bool hit =
                             (d < hi2) & (d < h2s[u]) & (!decltype(check_self)::value || j != (int)k);
                         if (hit && (count & 3) == 3) {
@@ -853,7 +859,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_one(NNArgs A) {
                 const double dx = own.x - x;
                 const double dy = own.y - y;
                 const double dz = own.z - z;
-                const double d = dx * dx + dy * dy + dz * dz;
+                const double d = nn_d2(dx, dy, dz);
                 hit = d < hi2 && d < h2 && j != (int)k;
             }
             const uint64_t mh = wave_ballot(hit);
@@ -933,7 +939,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_mark_dirty(NNArgs A) {
                 const double dx = own.x - x;
                 const double dy = own.y - y;
                 const double dz = own.z - z;
-                const double d = dx * dx + dy * dy + dz * dz;
+                const double d = nn_d2(dx, dy, dz);
                 if (((d < hi2) & (d < h02)) != ((d < hi2) & (d < h12))) A.dirty[i] = 1;
             }
         };
@@ -981,7 +987,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_direct(NNArgs A) {
                 const double dx = own.x - p.x;
                 const double dy = own.y - p.y;
                 const double dz = own.z - p.z;
-                const double d = dx * dx + dy * dy + dz * dz;
+                const double d = nn_d2(dx, dy, dz);
                 count += (d < hi2 && d < h * h && j != (int)k) ? 1 : 0;
             }
         });
@@ -1025,7 +1031,7 @@ __global__ __launch_bounds__(64 * NN1_WAVES) void k_nn_count_split(NNArgs A) {
                 const double dx = own.x - L.tx[lane];
                 const double dy = own.y - L.ty[lane];
                 const double dz = own.z - L.tz[lane];
-                const double d = dx * dx + dy * dy + dz * dz;
+                const double d = nn_d2(dx, dy, dz);
                 hit = d < hi2 && d < L.th2[lane] && L.tj[lane] != (int)k;
             }
             count += popc64(wave_ballot(hit));

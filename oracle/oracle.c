@@ -545,15 +545,19 @@ void orc_direct(int64_t n, const double *x, const double *y, const double *z,
 
 /* ------------------------------------------------------------------------------------
  * a8: neighbour criterion of SPH::Kernel::fixedRadiusNN (sph.cu:93-281): j != i is a
- * neighbour of i iff d^2 < h_i^2 and d^2 < h_j^2, d^2 = dx*dx + dy*dy + dz*dz in FP64
- * with dx = x_i - x_j.  Lists here are sorted by j.
+ * neighbour of i iff d^2 < h_i^2 and d^2 < h_j^2, d^2 = fma(dx, dx, fma(dy, dy, dz*dz))
+ * in FP64 with dx = x_i - x_j (nn_d2: the reference's CUDA build contracts products into fused
+ * multiply-adds by default, so its d^2 is some fused form; this one is fixed here and on the
+ * GPU).  Lists here are sorted by j.
  * ---------------------------------------------------------------------------------- */
+static inline double nn_d2(double dx, double dy, double dz) { return fma(dx, dx, fma(dy, dy, dz * dz)); }
+
 static inline int nn_pair(const double *x, const double *y, const double *z,
                           const double *h, int64_t i, int64_t j) {
     double dx = x[i] - x[j];
     double dy = y[i] - y[j];
     double dz = z[i] - z[j];
-    double d = dx * dx + dy * dy + dz * dz;
+    double d = nn_d2(dx, dy, dz);
     return d < h[i] * h[i] && d < h[j] * h[j];
 }
 
@@ -731,7 +735,7 @@ int orc_nn_refwalk(int64_t n, const double *x, const double *y, const double *z,
                     double dy = y[i] - ny[c];
                     double dz = z[i] - nz[c];
                     if (c < n) {
-                        double d = dx * dx + dy * dy + dz * dz;
+                        double d = nn_d2(dx, dy, dz);
                         if (d < sml * sml && d < h[c] * h[c]) {
                             if (k < max_nb) nnl[i * max_nb + k] = c;
                             k++;

@@ -103,7 +103,8 @@ void orc_direct(int64_t n, const double *x, const double *y, const double *z,
                 const double *m, double smoothing, int64_t ntarget, const int32_t *targets,
                 double *ax, double *ay, double *az, double *phi);
 
-/* ---- a8: neighbour search (sph.cu:93-281); lists are n x max_nb, row-major ---- */
+/* ---- a8: neighbour search (sph.cu:93-281); lists are n x max_nb, row-major; pair test
+ * d^2 < h_i^2 and d^2 < h_j^2 with d^2 = fma(dx, dx, fma(dy, dy, dz*dz)) ---- */
 int orc_nn_brute(int64_t n, const double *x, const double *y, const double *z,
                  const double *h, int max_nb, int32_t *nnl, int32_t *noi);
 int orc_nn_grid(int64_t n, const double *x, const double *y, const double *z,

@@ -55,6 +55,49 @@ def test_neighbour_sets_bitexact(gpu_lib, orc, sedov24):
     assert sets(nnl, noi) == sets(onnl, onoi)
 
 
+def test_neighbour_test_uses_the_fused_distance(gpu_lib, orc):
+    """Isolated pairs whose fused d^2 = fma(dx, dx, fma(dy, dy, dz^2)) and unfused
+    dx^2 + dy^2 + dz^2 straddle fl(h^2) (found by search from the pairs' own coordinate
+    differences): the GPU search lists exactly the pairs the fused form admits, as the
+    oracle does."""
+    import math
+    from test_oracle_sph import fma_exact
+    print_case("neighbours: fused-distance ties")
+    rng = np.random.default_rng(9)
+    xs, ys, zs, hs, expect = [], [], [], [], []
+    while len(expect) < 16:
+        a = np.array([4.0 * len(expect), 0.0, 0.0])
+        b = a + rng.uniform(0.1, 1.0, 3)
+        dx, dy, dz = (float(v) for v in (a - b))   # the kernels' x_i - x_j
+        df = fma_exact(dx, dx, fma_exact(dy, dy, dz * dz))
+        du = dx * dx + dy * dy + dz * dz
+        if df == du:
+            continue
+        hi = max(df, du)
+        h = math.sqrt(hi)
+        for _ in range(8):
+            if h * h == hi:
+                break
+            h = float(np.nextafter(h, np.inf) if h * h < hi else np.nextafter(h, -np.inf))
+        if h * h != hi:
+            continue
+        for p_ in (a, b):
+            xs.append(p_[0]); ys.append(p_[1]); zs.append(p_[2]); hs.append(h)
+        expect += [df < du]
+    s = {"x": np.array(xs), "y": np.array(ys), "z": np.array(zs), "sml": np.array(hs)}
+    n = len(xs)
+    s["mass"] = np.full(n, 1.0 / n)
+    p = gpu_lib.default_params(curve=1)
+    with make_ctx(gpu_lib, s, p) as ctx:
+        ctx.bbox(); ctx.keys(); ctx.sort(); ctx.build_tree()
+        ctx.neighbours()
+        nnl, noi = ctx.neighbour_lists()
+    want = np.repeat(np.array(expect, dtype=np.int32), 2)
+    assert np.array_equal(noi, want), (noi, want)
+    st, _, onoi = orc.nn_brute(s["x"], s["y"], s["z"], s["sml"], 180)
+    assert st == 0 and np.array_equal(onoi, want)
+
+
 @pytest.mark.parametrize("cap", [0.0, 1.3])
 def test_h_iteration_bitexact(gpu_lib, orc, sedov24, cap):
     print_case(f"h-iteration sedov 24^3 perturbed, sml_max {cap} x max h0")

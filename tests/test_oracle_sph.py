@@ -12,7 +12,16 @@ def as_sets(nnl, noi):
     return [frozenset(nnl[i, : noi[i]].tolist()) for i in range(len(noi))]
 
 
+def fma_exact(a, b, c):
+    """a * b + c rounded once (exact rationals, then Python's correctly rounded float())."""
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
 def numpy_brute_sets(x, y, z, h):
+    """Brute-force sets with the neighbour test d^2 = fma(dx, dx, fma(dy, dy, dz^2)) < h^2:
+    numpy's unfused d^2 differs from it by a few ulps at most, so only pairs that close to
+    either bound are decided again with the exactly rounded fused form."""
     n = len(x)
     out = []
     for i in range(n):
@@ -20,7 +29,12 @@ def numpy_brute_sets(x, y, z, h):
         dy = y[i] - y
         dz = z[i] - z
         d = dx * dx + dy * dy + dz * dz
-        ok = (d < h[i] * h[i]) & (d < h * h)
+        hi2, hj2 = h[i] * h[i], h * h
+        ok = (d < hi2) & (d < hj2)
+        near = np.nonzero((np.abs(d - hi2) <= 1e-12 * hi2) | (np.abs(d - hj2) <= 1e-12 * hj2))[0]
+        for j in near:
+            dj = fma_exact(dx[j], dx[j], fma_exact(dy[j], dy[j], dz[j] * dz[j]))
+            ok[j] = (dj < hi2) and (dj < hj2[j])
         ok[i] = False
         out.append(frozenset(np.nonzero(ok)[0].tolist()))
     return out
@@ -40,6 +54,41 @@ def test_nn_brute_matches_numpy(orc, blob):
     st, nnl, noi = orc.nn_brute(x, y, z, h, max_nb=400)
     assert st == 0
     assert as_sets(nnl, noi) == numpy_brute_sets(x, y, z, h)
+
+
+def fused_tie_cases(count, seed=5):
+    """Offsets (dx, dy, dz) and radii h for which the fused d^2 = fma(dx, dx, fma(dy, dy, dz^2))
+    and the unfused dx^2 + dy^2 + dz^2 fall on either side of fl(h * h): found by search, each
+    case decided one way by the fused form and the other way by the unfused one."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        dx, dy, dz = (float(v) for v in rng.uniform(0.1, 1.0, 3))
+        df = fma_exact(dx, dx, fma_exact(dy, dy, dz * dz))
+        du = dx * dx + dy * dy + dz * dz
+        if df == du:
+            continue
+        hi = max(df, du)   # want fl(h*h) == the larger one: then only the smaller one is below it
+        h = math.sqrt(hi)
+        for _ in range(8):
+            if h * h == hi:
+                out.append((dx, dy, dz, h, df < du))
+                break
+            h = np.nextafter(h, np.inf) if h * h < hi else np.nextafter(h, -np.inf)
+    return out
+
+
+def test_nn_pair_test_uses_the_fused_distance(orc):
+    """The neighbour test's d^2 is fma(dx, dx, fma(dy, dy, dz^2)): on pairs where that and the
+    unfused sum straddle fl(h^2), the oracle's brute force and grid search decide as the fused
+    form does."""
+    for dx, dy, dz, h, fused_is_pair in fused_tie_cases(12):
+        x, y, z = np.array([0.0, dx]), np.array([0.0, dy]), np.array([0.0, dz])
+        hh = np.array([h, h])
+        for nn in (orc.nn_brute, orc.nn_grid):
+            st, _, noi = nn(x, y, z, hh, max_nb=4)
+            assert st == 0
+            assert noi.tolist() == ([1, 1] if fused_is_pair else [0, 0]), (dx, dy, dz, h)
 
 
 def test_nn_grid_equals_brute(orc, blob):
