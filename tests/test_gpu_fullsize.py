@@ -116,7 +116,9 @@ def test_sedov128_benchmarked_rhs_equals_oracle(gpu_lib, orc):
     assert np.array_equal(g["sml"], o["sml"])
     assert np.array_equal(g["h_iters"], o["h_iters"])
     assert np.array_equal(g["noi"], o["noi"]) and np.array_equal(noi, o["noi"])
-    assert st["n_neighbours"] == int(o["noi"].sum())
+    # statistics: sums and the maximum over the counters' spread copies
+    assert st["n_neighbours"] == int(o["noi"].sum()) and st["max_noi"] == int(o["noi"].max())
+    assert st["n_h_counts"] == int(o["h_iters"].sum())
     assert np.array_equal(sorted_rows(nnl, noi), sorted_rows(o["nnl"], o["noi"]))
     del nnl
     for k in ("rho", "p", "cs"):
