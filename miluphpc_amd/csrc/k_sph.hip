@@ -494,13 +494,18 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
         const int v = lane < np ? L.rcnt[lane] : 0;
         const int incl = wave_incl_scan(v);
         const int total = __shfl(incl, 63);
-        L.rpre[lane] = incl - v;
+        const int pre = incl - v;
+        L.rpre[lane] = pre;
         wave_sync();
         for (int base = 0; base < total; base += 64) {
             const int t = base + lane;
+            // the ranges holding this chunk's first and last candidates bound every lane's
+            // search (prefixes are non-decreasing): usually one or two ranges apart
+            const int rf = popc64(wave_ballot(lane < np && pre <= base)) - 1;
+            const int rl = popc64(wave_ballot(lane < np && pre <= base + 63)) - 1;
             if constexpr (!FILTER) {
                 if (t < total) {
-                    int rlo = 0, rhi = np - 1;
+                    int rlo = rf, rhi = rl;
                     while (rlo < rhi) {
                         const int mid = (rlo + rhi + 1) >> 1;
                         if (L.rpre[mid] <= t) rlo = mid; else rhi = mid - 1;
@@ -522,7 +527,7 @@ __device__ __forceinline__ bool nn_walk_box(LdsT &L, const NNArgs &A, double lox
                 double rx = 0.0, ry = 0.0, rz = 0.0, h2 = 0.0;
                 int j = 0;
                 if (t < total) {
-                    int rlo = 0, rhi = np - 1;
+                    int rlo = rf, rhi = rl;
                     while (rlo < rhi) {
                         const int mid = (rlo + rhi + 1) >> 1;
                         if (L.rpre[mid] <= t) rlo = mid; else rhi = mid - 1;
